@@ -1,0 +1,202 @@
+"""Headline benchmark: SMA-TB (Gupta) bulk Au NVE at 1,000,188 atoms on one
+MI355X, mixed precision, beside the plain FP64 GPU form and a CPU baseline.
+
+A step is one velocity-Verlet timestep of the whole system: stepa (drift and
+half kick), the skin test and any neighbour rebuild, the two SMA-TB passes
+(density, then forces and energies) and stepb (half kick).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  `value` = atoms * K / wall time of the K timed steps
+(inputs already resident in HBM).  `roofline` is for the dominant kernel, the
+mixed-precision force pass, timed with HIP events on the library's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "SMA-TB atom-steps/s at 1M atoms (mixed vs FP64 GPU) + batched cluster-steps/s"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per atom (FP64 x 24 + type 4 + rho 8 in pass 1;
+# x 24 + type 4 + rho/eband 8 + F 24 + e 8 in pass 2)
+BYTES_PASS1 = 36
+BYTES_PASS2 = 68
+
+AU = dict(A=0.2061, xi=1.790, p=10.229, q=4.036, r0=2.884)
+A_LAT = 4.078
+MASS_AU = float(np.float32(196.97))
+
+
+def fcc(nx, a):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0, 0.5, 0.5], [0.5, 0, 0.5]])
+    i = np.arange(nx, dtype=np.float64)
+    g = np.stack(np.meshgrid(i, i, i, indexing="ij"), axis=-1)[..., ::-1].reshape(-1, 3)
+    return ((g[:, None, :] + base[None, :, :]).reshape(-1, 3)) * a
+
+
+def workload(nx, seed=20260101):
+    x = fcc(nx, A_LAT)
+    x = x + np.random.default_rng(seed).normal(0.0, 0.02, x.shape)
+    L = nx * A_LAT
+    return np.mod(x, L), L
+
+
+def make_ctx(hip, precision, x, L, skin, lcg_seed=4242, T=300.0, dt=0.002):
+    c = hip.Context(0, precision)
+    c.set_box([L] * 3, (1, 1, 1))
+    c.set_atoms(x)
+    c.set_masses([MASS_AU])
+    c.set_smatb(1, AU["A"], AU["xi"], AU["p"], AU["q"], AU["r0"], A_LAT, A_LAT * np.sqrt(1.5))
+    c.set_skin(skin)
+    c.prng_lcg(lcg_seed)
+    c.set_gdist(T)
+    c.integrator_vv(dt)
+    return c
+
+
+def timed_run(c, steps, warmup, timing=True):
+    c.run(warmup)
+    st0 = c.stats()
+    c.set_timing(timing)
+    t0 = time.perf_counter()
+    c.run(steps)  # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    tm = c.timing()
+    c.set_timing(False)
+    st1 = c.stats()
+    return t1 - t0, tm, st1["nupd"] - st0["nupd"]
+
+
+def cpu_baseline(x, L, skin, budget_s=20.0):
+    """The CPU oracle (C restatement of the reference's OpenMP build) on the
+    host cores, over as many steps of the same workload as fit the budget."""
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    s = O.smatb_params([(0, 0, AU["A"], AU["xi"], AU["p"], AU["q"], AU["r0"], A_LAT,
+                         A_LAT * np.sqrt(1.5))], 1)
+    box = O.make_box([L] * 3, (1, 1, 1))
+    mass = np.full(len(x), MASS_AU)
+    v = O.set_gdist(O.LCG(4242), 300.0, mass)
+    md = O.MDRun(x, v, mass, box, pot="smatb", tb=s, types=np.zeros(len(x)), skin=skin,
+                 integ="vv", dt=0.002, nthreads=cores, maxnb=96)
+    md.interact()  # first force evaluation (not timed, as in dinamic)
+    steps, t = 0, 0.0
+    while t < budget_s and steps < 200:
+        t0 = time.perf_counter()
+        md.run(1, each=0)
+        t += time.perf_counter() - t0
+        steps += 1
+    return dict(value=len(x) * steps / t, unit="atom-steps/s", cores=cores, kind="port",
+                sample="%d VV steps of the same %d-atom Au workload (oracle/gems_oracle.c, "
+                       "OpenMP, %d threads; reference semantics incl. its rebuild rule: "
+                       "%d rebuilds)" % (steps, len(x), cores, md.nupd - 1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=63)
+    ap.add_argument("--skin", type=float, default=1.0)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not run)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from gems_amd import hip
+
+    x, L = workload(args.nx)
+    n = len(x)
+
+    # mixed precision (the headline)
+    c = make_ctx(hip, hip.GH_MIXED, x, L, args.skin)
+    F_mx, _, et_mx = c.compute()
+    e0 = c.energies()
+    dt_mx, tm_mx, nupd_mx = timed_run(c, args.steps, args.warmup)
+    e1 = c.energies()
+    c.close()
+
+    # plain FP64 GPU form, same input
+    k64 = args.fp64_steps or max(10, args.steps // 2)
+    c = make_ctx(hip, hip.GH_FP64, x, L, args.skin)
+    F_64, _, et_64 = c.compute()
+    dt_64, tm_64, nupd_64 = timed_run(c, k64, max(2, args.warmup // 2))
+    c.close()
+
+    value = n * args.steps / dt_mx
+    value64 = n * k64 / dt_64
+    rms = float(np.sqrt(np.mean(F_64 ** 2)))
+    err_rms = float(np.sqrt(np.mean((F_mx - F_64) ** 2)) / rms)
+    err_max = float(np.max(np.abs(F_mx - F_64)) / rms)
+
+    ms_force, cnt_force = tm_mx["force"]
+    ms_rho, cnt_rho = tm_mx["rho"]
+    t_force = ms_force / cnt_force * 1e-3
+    t_rho = ms_rho / cnt_rho * 1e-3
+    achieved = BYTES_PASS2 * n / t_force / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_mx / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair terms, f64 positions/accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": "Au SMA-TB bulk NVE, %d atoms (%d^3 fcc cells, a=4.078 A, 0.02 A "
+                        "seeded jitter, 300 K Maxwell-Boltzmann via the reference lcg "
+                        "stream), velocity Verlet dt=2 fs, skin %.2f A, periodic"
+                        % (n, args.nx, args.skin),
+            "atoms": n,
+            "precision": "mixed",
+        },
+        "roofline": {
+            "kernel": "k_tb_force_mx (SMA-TB pass 2, mixed)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_atom": BYTES_PASS2,
+            "avg_launch_us": t_force * 1e6,
+        },
+        "pass1": {"kernel": "k_tb_rho_mx", "avg_launch_us": t_rho * 1e6,
+                  "achieved_GBs": BYTES_PASS1 * n / t_rho / 1e9},
+        "fp64": {"value": value64, "steps": k64, "ms_per_step": dt_64 / k64 * 1e3,
+                 "force_pass_us": tm_64["force"][0] / tm_64["force"][1] * 1e3,
+                 "rho_pass_us": tm_64["rho"][0] / tm_64["rho"][1] * 1e3,
+                 "rebuilds": nupd_64},
+        "mixed_vs_fp64": value / value64,
+        "force_err_vs_fp64": {"rms_over_rms": err_rms, "max_over_rms": err_max},
+        "epot_rel_err_vs_fp64": abs(et_mx / et_64 - 1.0),
+        "energy_change_ev_per_atom": (e1[0] + e1[1] - e0[0] - e0[1]) / n,
+        "rebuilds": nupd_mx,
+        "step_breakdown_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in tm_mx.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(x, L, args.skin, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,851 @@
+// Host side of the C ABI declared in include/gems_hip.h.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <new>
+
+#include "gh_internal.h"
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+int gh_fail(gh_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+static inline int roundup(int v, int m) { return (v + m - 1) / m * m; }
+
+template <typename T>
+static int dalloc(gh_ctx *c, T **p, size_t count) {
+  if (*p) {
+    GH_HIP(c, hipFree(*p));
+    *p = nullptr;
+  }
+  if (count == 0) return 0;
+  GH_HIP(c, hipMalloc((void **)p, sizeof(T) * count));
+  return 0;
+}
+
+#define GH_TRY(x)            \
+  do {                       \
+    int rc_ = (x);           \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+static int read_status(gh_ctx *c, GhStatus *h) {
+  GH_HIP(c, hipMemcpyAsync(h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+static int write_status(gh_ctx *c, const GhStatus *h) {
+  GH_HIP(c, hipMemcpyAsync(c->status, h, sizeof(GhStatus), hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// `prng lcg` (Random.F90:145-250): xorshift(13,-17,5) on ix combined with a
+// Park-Miller (Schrage) generator iy; Marsaglia polar normals in pairs.
+// ---------------------------------------------------------------------------
+void GhLcg::seed(int s) {
+  idum = -std::abs(s);
+  stored = false;
+}
+
+double GhLcg::ranu() {
+  const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+  // am is evaluated in default (single) real by the reference: the integer
+  // im rounds to 2^31 and am = (1 - 2^-24) * 2^-31 exactly.
+  const double am = (double)(nextafterf(1.0f, 0.0f) / (float)im);
+  if (idum <= 0 || iy < 0) {
+    iy = (888889999 ^ std::abs(idum)) | 1;
+    ix = 777755555 ^ std::abs(idum);
+    idum = std::abs(idum) + 1;
+  }
+  uint32_t u = (uint32_t)ix;
+  u ^= u << 13;
+  u ^= u >> 17;
+  u ^= u << 5;
+  ix = (int32_t)u;
+  const int32_t k = iy / iq;
+  iy = ia * (iy - k * iq) - ir * k;
+  if (iy < 0) iy += im;
+  return am * (double)((im & (ix ^ iy)) | 1);
+}
+
+double GhLcg::gasdev() {
+  if (stored) {
+    stored = false;
+    return g;
+  }
+  double v1, v2, rsq;
+  do {
+    v1 = 2.0 * ranu() - 1.0;
+    v2 = 2.0 * ranu() - 1.0;
+    rsq = v1 * v1 + v2 * v2;
+  } while (!(rsq > 0.0 && rsq < 1.0));
+  rsq = sqrt(-2.0 * log(rsq) / rsq);
+  g = v2 * rsq;
+  stored = true;
+  return v1 * rsq;
+}
+
+void GhLcg::rang(double *r1, double *r2) {
+  *r1 = gasdev();
+  const double aux = gasdev();
+  if (r2) *r2 = aux;
+}
+
+// Quintic tail of Logsdail et al. as smatb_set derives it (TB.F90:134-149).
+void gh_tb_coeffs(GhTbPair &c) {
+  const double dcut = c.rce - c.rci;
+  const double ab = -1.0 / (dcut * dcut * dcut);
+  double bb = -(c.q / c.r0) / (dcut * dcut);
+  double cb = -((c.q / c.r0) * (c.q / c.r0)) / dcut;
+  double aux = c.xi * exp(-c.q * (c.rci / c.r0 - 1.0));
+  c.b1 = aux * (12.0 * ab - 6.0 * bb + cb) / (2.0 * (dcut * dcut));
+  c.b2 = aux * (15.0 * ab - 7.0 * bb + cb) / dcut;
+  c.b3 = aux * (20.0 * ab - 8.0 * bb + cb) / 2.0;
+  aux = c.A * exp(-c.p * (c.rci / c.r0 - 1.0));
+  bb = -(c.p / c.r0) / (dcut * dcut);
+  cb = -((c.p / c.r0) * (c.p / c.r0)) / dcut;
+  c.r1 = aux * (12.0 * ab - 6.0 * bb + cb) / (2.0 * (dcut * dcut));
+  c.r2 = aux * (15.0 * ab - 7.0 * bb + cb) / dcut;
+  c.r3 = aux * (20.0 * ab - 8.0 * bb + cb) / 2.0;
+}
+
+int gh_upload_tb(gh_ctx *c) {
+  TbDev h[GH_MAXT * GH_MAXT];
+  memset(h, 0, sizeof(h));
+  const double log2e = 1.4426950408889634;
+  for (int a = 0; a < c->nt; ++a)
+    for (int b = 0; b < c->nt; ++b) {
+      const GhTbPair &t = c->tb[a * GH_MAXT + b];
+      TbDev &d = h[a * GH_MAXT + b];
+      d.ca = t.A;
+      d.eps2 = t.xi * t.xi;
+      d.p = t.p;
+      d.q = t.q;
+      d.r0 = t.r0;
+      d.one_r0 = 1.0 / t.r0;
+      d.rci = t.rci;
+      d.rce = t.rce;
+      d.rce2 = t.rce * t.rce;
+      d.b1 = t.b1;
+      d.b2 = t.b2;
+      d.b3 = t.b3;
+      d.r1 = t.r1;
+      d.r2 = t.r2;
+      d.r3 = t.r3;
+      d.p_r0 = t.p / t.r0;
+      d.q2_r0 = 2.0 * t.q / t.r0;
+      // A e^{-p(r/r0-1)} = 2^{log2(A) + p log2e - (p/r0) log2e r}
+      d.rep_a = (float)(log2(t.A) + t.p * log2e);
+      d.rep_b = (float)(-t.p / t.r0 * log2e);
+      d.rho_a = (float)(log2(t.xi * t.xi) + 2.0 * t.q * log2e);
+      d.rho_b = (float)(-2.0 * t.q / t.r0 * log2e);
+      d.frep = (float)(2.0 * t.p / t.r0);
+      d.fband = (float)(t.q / t.r0);  // xi^2/2 * 2q/r0 = xi^2 q/r0 (xi^2 inside band)
+      d.rci_f = (float)t.rci;
+      d.rce_f = (float)t.rce;
+      d.b1f = (float)t.b1;
+      d.b2f = (float)t.b2;
+      d.b3f = (float)t.b3;
+      d.r1f = (float)t.r1;
+      d.r2f = (float)t.r2;
+      d.r3f = (float)t.r3;
+    }
+  GH_TRY(dalloc(c, &c->tbd, GH_MAXT * GH_MAXT));
+  GH_HIP(c, hipMemcpy(c->tbd, h, sizeof(h), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// grid / list sizing
+// ---------------------------------------------------------------------------
+// Cell grid as intergroup_setcells chooses it (Neighbour.F90:587-664): cells
+// of edge >= rcut+skin; no box, or fewer than 4 cells on every axis, selects
+// the O(N^2) list.  Periodic axes need >= 3 cells for the 27-cell stencil to
+// visit distinct cells; otherwise the O(N^2) list is used.  Non-periodic axes
+// get one guard cell on each side so atoms that drift out of the box within
+// a cell length are still binned exactly.
+static void setup_grid(gh_ctx *c) {
+  const double rl = c->rcut + c->skin;
+  c->list_mode = 0;
+  for (int d = 0; d < 3; ++d) {
+    c->nc[d] = 1;
+    c->gn[d] = 1;
+    c->gofs[d] = 0;
+    c->cell[d] = c->L[d];
+  }
+  c->ncell = 1;
+  if (!c->boxed) return;
+  int nc[3];
+  bool all_small = true, pbc_small = false;
+  for (int d = 0; d < 3; ++d) {
+    nc[d] = (int)(c->L[d] / rl);
+    if (nc[d] >= 4) all_small = false;
+    if (c->pbc[d] && nc[d] < 3) pbc_small = true;
+    if (nc[d] < 1) nc[d] = 1;
+  }
+  if (all_small || pbc_small) return;
+  c->list_mode = 1;
+  c->ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    c->nc[d] = nc[d];
+    c->cell[d] = c->L[d] / nc[d];
+    c->gofs[d] = c->pbc[d] ? 0 : 1;
+    c->gn[d] = nc[d] + 2 * c->gofs[d];
+    c->ncell *= c->gn[d];
+  }
+}
+
+static int ensure_ready(gh_ctx *c) {
+  if (!c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms: call gh_set_atoms first");
+  if (c->pot == GH_POT_NONE) return gh_fail(c, GH_ESTATE, "no potential set");
+  if (c->ready) return 0;
+  setup_grid(c);
+  if (c->boxed) GH_TRY(gh_launch_wrap(c));
+  if (c->list_mode == 1 && c->ncell > c->cell_alloc) {
+    GH_TRY(dalloc(c, &c->cell_count, (size_t)c->ncell));
+    GH_TRY(dalloc(c, &c->cell_start, (size_t)c->ncell + 1));
+    GH_TRY(dalloc(c, &c->cell_cursor, (size_t)c->ncell));
+    c->cell_alloc = c->ncell;
+  }
+  // sizing pass: a rebuild with zero row capacity counts the rows
+  GhStatus h;
+  GH_TRY(read_status(c, &h));
+  const long long nupd0 = h.nupd;
+  c->cap = 0;
+  GH_TRY(gh_launch_rebuild(c, true));
+  GH_TRY(read_status(c, &h));
+  int maxnn = h.maxnn;
+  int cap = maxnn + std::max(16, maxnn / 4);
+  if (c->cap_req > cap) cap = c->cap_req;
+  cap = roundup(std::max(cap, 8), 8);
+  if ((size_t)cap * c->ncap > (size_t)1 << 31)
+    return gh_fail(c, GH_EINVAL, "neighbour list too large");
+  GH_TRY(dalloc(c, &c->list, (size_t)cap * c->ncap));
+  c->cap = cap;
+  h.overflow = 0;
+  h.nupd = nupd0;
+  GH_TRY(write_status(c, &h));
+  GH_TRY(gh_launch_rebuild(c, true));
+  GH_TRY(read_status(c, &h));
+  if (h.overflow) return gh_fail(c, GH_EOVERFLOW, "neighbour list overflow at sizing");
+  c->ready = true;
+  c->pos_dirty = false;
+  return 0;
+}
+
+static int check_status(gh_ctx *c) {
+  GhStatus h;
+  GH_TRY(read_status(c, &h));
+  if (h.overflow) {
+    c->ready = false;
+    c->cap_req = h.maxnn + std::max(16, h.maxnn / 4);
+    return gh_fail(c, GH_EOVERFLOW,
+                   "neighbour list row capacity exceeded (" + std::to_string(h.maxnn) + " > " +
+                       std::to_string(c->cap) + "); results of this call are invalid; the "
+                       "next call re-sizes the list");
+  }
+  if (h.outside)
+    return gh_fail(c, GH_ERANGE, "an atom left the cell grid of a non-periodic box axis");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// timing pool: 8 events per step
+//   0 start, 1 after stepa, 2 after rebuild, 3 after pass 1, 4 after pass 2,
+//   5 after stepb
+// ---------------------------------------------------------------------------
+static const int EV_PER_STEP = 6;
+static const int EV_STEPS = 128;
+
+static int timing_flush(gh_ctx *c) {
+  if (!c->timing || c->evused == 0) return 0;
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->evused; ++k) {
+    hipEvent_t *e = &c->evpool[(size_t)k * EV_PER_STEP];
+    float t;
+    GH_HIP(c, hipEventElapsedTime(&t, e[2], e[3]));
+    c->t_ms[0] += t;
+    c->t_cnt[0] += 1;
+    GH_HIP(c, hipEventElapsedTime(&t, e[3], e[4]));
+    c->t_ms[1] += t;
+    c->t_cnt[1] += 1;
+    float ta, tb;
+    GH_HIP(c, hipEventElapsedTime(&ta, e[0], e[1]));
+    GH_HIP(c, hipEventElapsedTime(&tb, e[4], e[5]));
+    c->t_ms[2] += ta + tb;
+    c->t_cnt[2] += 1;
+    GH_HIP(c, hipEventElapsedTime(&t, e[1], e[2]));
+    c->t_ms[3] += t;
+    c->t_cnt[3] += 1;
+  }
+  c->evused = 0;
+  return 0;
+}
+
+static hipEvent_t ev_at(gh_ctx *c, int k) {
+  if (!c->timing) return nullptr;
+  return c->evpool[(size_t)c->evused * EV_PER_STEP + k];
+}
+
+static int rec(gh_ctx *c, int k) {
+  if (!c->timing) return 0;
+  GH_HIP(c, hipEventRecord(ev_at(c, k), c->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int gh_create(gh_ctx **out, int device, int precision) {
+  if (!out) return GH_EINVAL;
+  *out = nullptr;
+  if (precision != GH_FP64 && precision != GH_MIXED) return GH_EINVAL;
+  gh_ctx *c = new (std::nothrow) gh_ctx();
+  if (!c) return GH_EINVAL;
+  c->device = device;
+  c->precision = precision;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    delete c;
+    return GH_EHIP;
+  }
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void **)&c->status, sizeof(GhStatus)) != hipSuccess ||
+      hipHostMalloc((void **)&c->red_h, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return GH_EHIP;
+  }
+  GhStatus h;
+  memset(&h, 0, sizeof(h));
+  if (hipMemcpy(c->status, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+    delete c;
+    return GH_EHIP;
+  }
+  c->lcg.seed(123456789);
+  *out = c;
+  return 0;
+}
+
+void gh_destroy(gh_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *dev[] = {c->pos, c->pos_t, c->vel, c->vel_t, c->acc, c->acc_t, c->frc, c->epa,
+                 c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
+                 c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
+                 c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->status,
+                 c->red};
+  for (void *p : dev)
+    if (p) (void)hipFree(p);
+  if (c->gauss_h) (void)hipHostFree(c->gauss_h);
+  if (c->red_h) (void)hipHostFree(c->red_h);
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *gh_last_error(const gh_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int gh_set_precision(gh_ctx *c, int precision) {
+  if (!c) return GH_EINVAL;
+  if (precision != GH_FP64 && precision != GH_MIXED)
+    return gh_fail(c, GH_EINVAL, "precision must be GH_FP64 or GH_MIXED");
+  c->precision = precision;
+  return 0;
+}
+
+int gh_set_box(gh_ctx *c, const double *L, const int *pbc) {
+  if (!c) return GH_EINVAL;
+  if (L) {
+    for (int d = 0; d < 3; ++d)
+      if (!(L[d] > 0.0)) return gh_fail(c, GH_EINVAL, "box lengths must be positive");
+    c->boxed = 1;
+    for (int d = 0; d < 3; ++d) {
+      c->L[d] = L[d];
+      c->pbc[d] = pbc ? (pbc[d] != 0) : 0;
+    }
+  } else {
+    c->boxed = 0;
+    for (int d = 0; d < 3; ++d) {
+      c->L[d] = 1.0e6;  // the reference's unboxed box (Program_Types.F90:69)
+      c->pbc[d] = 0;
+    }
+  }
+  c->ready = false;
+  c->pos_dirty = true;
+  return 0;
+}
+
+int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
+  if (!c || n <= 0 || !x) return gh_fail(c, GH_EINVAL, "gh_set_atoms: bad arguments");
+  if (n > (int)GH_JMASK_HOST) return gh_fail(c, GH_EINVAL, "too many atoms (max 134217727)");
+  GH_HIP(c, hipSetDevice(c->device));
+  const int ncap = roundup(n, 256);
+  if (ncap != c->ncap) {
+    c->ncap = ncap;
+    GH_TRY(dalloc(c, &c->pos, ncap));
+    GH_TRY(dalloc(c, &c->pos_t, ncap));
+    GH_TRY(dalloc(c, &c->vel, 3 * (size_t)ncap));
+    GH_TRY(dalloc(c, &c->vel_t, 3 * (size_t)ncap));
+    GH_TRY(dalloc(c, &c->acc, 3 * (size_t)ncap));
+    GH_TRY(dalloc(c, &c->acc_t, 3 * (size_t)ncap));
+    GH_TRY(dalloc(c, &c->frc, 3 * (size_t)ncap));
+    GH_TRY(dalloc(c, &c->epa, ncap));
+    GH_TRY(dalloc(c, &c->rho, ncap));
+    GH_TRY(dalloc(c, &c->eband, ncap));
+    GH_TRY(dalloc(c, &c->eband32, ncap));
+    GH_TRY(dalloc(c, &c->type, ncap));
+    GH_TRY(dalloc(c, &c->type_t, ncap));
+    GH_TRY(dalloc(c, &c->orig, ncap));
+    GH_TRY(dalloc(c, &c->orig_t, ncap));
+    GH_TRY(dalloc(c, &c->xold, 3 * (size_t)ncap));
+    GH_TRY(dalloc(c, &c->cellof, ncap));
+    GH_TRY(dalloc(c, &c->cellid, ncap));
+    GH_TRY(dalloc(c, &c->perm, ncap));
+    GH_TRY(dalloc(c, &c->nn, ncap));
+    c->nred = std::min(1024, std::max(1, (ncap + 255) / 256));
+    GH_TRY(dalloc(c, &c->red, (size_t)c->nred * 5 + 16));
+    if (c->gauss) {
+      GH_HIP(c, hipFree(c->gauss));
+      c->gauss = nullptr;
+    }
+    if (c->gauss_h) {
+      GH_HIP(c, hipHostFree(c->gauss_h));
+      c->gauss_h = nullptr;
+    }
+    if (c->list) {
+      GH_HIP(c, hipFree(c->list));
+      c->list = nullptr;
+    }
+    c->cap = 0;
+  }
+  c->n = n;
+  std::vector<double4> hp(ncap);
+  c->type_h.assign(n, 0);
+  std::vector<int> ho(ncap), ht(ncap, 0);
+  for (int i = 0; i < ncap; ++i) {
+    if (i < n) {
+      double v[3];
+      for (int d = 0; d < 3; ++d) {
+        v[d] = x[3 * i + d];
+        // do_pbc_1 (Set_Properties.f90:444-470): periodic axes start wrapped
+        if (c->boxed && c->pbc[d]) {
+          if (v[d] >= c->L[d]) v[d] -= c->L[d];
+          else if (v[d] < 0.0) v[d] += c->L[d];
+          if (v[d] < 0.0 || v[d] >= c->L[d])
+            return gh_fail(c, GH_EINVAL, "atom more than one box length outside a periodic box");
+        }
+      }
+      hp[i] = make_double4(v[0], v[1], v[2], 0.0);
+      int t = type ? type[i] : 0;
+      if (t < 0 || t >= GH_MAXT) return gh_fail(c, GH_EINVAL, "atom type out of range 0..7");
+      ht[i] = t;
+      c->type_h[i] = t;
+    } else {
+      hp[i] = make_double4(0, 0, 0, 0);
+    }
+    ho[i] = i;
+  }
+  GH_HIP(c, hipMemcpy(c->pos, hp.data(), sizeof(double4) * ncap, hipMemcpyHostToDevice));
+  GH_HIP(c, hipMemcpy(c->type, ht.data(), sizeof(int) * ncap, hipMemcpyHostToDevice));
+  GH_HIP(c, hipMemcpy(c->orig, ho.data(), sizeof(int) * ncap, hipMemcpyHostToDevice));
+  std::vector<double> z(3 * (size_t)ncap, 0.0);
+  GH_HIP(c, hipMemcpy(c->vel, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
+  GH_HIP(c, hipMemcpy(c->acc, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
+  GH_HIP(c, hipMemcpy(c->frc, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
+  GH_HIP(c, hipMemcpy(c->xold, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
+  GH_HIP(c, hipMemcpy(c->epa, z.data(), sizeof(double) * ncap, hipMemcpyHostToDevice));
+  c->have_atoms = true;
+  c->ready = false;
+  c->pos_dirty = true;
+  c->first_force = true;
+  return 0;
+}
+
+int gh_set_masses(gh_ctx *c, int nt, const double *mass) {
+  if (!c || nt <= 0 || nt > GH_MAXT || !mass) return gh_fail(c, GH_EINVAL, "gh_set_masses");
+  for (int t = 0; t < nt; ++t) {
+    if (!(mass[t] > 0.0)) return gh_fail(c, GH_EINVAL, "masses must be positive");
+    c->mass_t[t] = mass[t];
+  }
+  return 0;
+}
+
+// Host copies in caller order <-> device arrays in sorted order.
+static int fetch_orig(gh_ctx *c, std::vector<int> &orig) {
+  orig.resize(c->n);
+  GH_HIP(c, hipMemcpyAsync(orig.data(), c->orig, sizeof(int) * c->n, hipMemcpyDeviceToHost,
+                           c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int fetch_soa3(gh_ctx *c, const double *d, const std::vector<int> &orig, double *out) {
+  std::vector<double> h(3 * (size_t)c->ncap);
+  GH_HIP(c, hipMemcpyAsync(h.data(), d, sizeof(double) * 3 * c->ncap, hipMemcpyDeviceToHost,
+                           c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < c->n; ++s)
+    for (int k = 0; k < 3; ++k) out[3 * (size_t)orig[s] + k] = h[(size_t)k * c->ncap + s];
+  return 0;
+}
+
+static int put_soa3(gh_ctx *c, double *d, const std::vector<int> &orig, const double *in) {
+  std::vector<double> h(3 * (size_t)c->ncap, 0.0);
+  for (int s = 0; s < c->n; ++s)
+    for (int k = 0; k < 3; ++k) h[(size_t)k * c->ncap + s] = in ? in[3 * (size_t)orig[s] + k] : 0.0;
+  GH_HIP(c, hipMemcpyAsync(d, h.data(), sizeof(double) * 3 * c->ncap, hipMemcpyHostToDevice,
+                           c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gh_set_velocities(gh_ctx *c, const double *v) {
+  if (!c || !c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms");
+  std::vector<int> orig;
+  GH_TRY(fetch_orig(c, orig));
+  return put_soa3(c, c->vel, orig, v);
+}
+
+int gh_set_smatb(gh_ctx *c, int nt, const double *A, const double *xi, const double *p,
+                 const double *q, const double *r0, const double *rci, const double *rce) {
+  if (!c || nt <= 0 || nt > GH_MAXT || !A || !xi || !p || !q || !r0 || !rci || !rce)
+    return gh_fail(c, GH_EINVAL, "gh_set_smatb: bad arguments");
+  double rmax = 0.0;
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const int k = a * nt + b;
+      if (!(rce[k] > rci[k]) || !(r0[k] > 0) || !(A[k] > 0) || !(xi[k] > 0))
+        return gh_fail(c, GH_EINVAL, "SMA-TB parameters need A, xi, r0 > 0 and rce > rci");
+      GhTbPair &t = c->tb[a * GH_MAXT + b];
+      t.A = A[k];
+      t.xi = xi[k];
+      t.p = p[k];
+      t.q = q[k];
+      t.r0 = r0[k];
+      t.rci = rci[k];
+      t.rce = rce[k];
+      gh_tb_coeffs(t);
+      rmax = std::max(rmax, rce[k]);
+    }
+  c->nt = nt;
+  c->pot = GH_POT_SMATB;
+  c->rcut = rmax;
+  GH_TRY(gh_upload_tb(c));
+  c->ready = false;
+  return 0;
+}
+
+int gh_set_lj(gh_ctx *c, double eps, double sigma, double rc) {
+  if (!c || !(sigma > 0) || !(rc > 0)) return gh_fail(c, GH_EINVAL, "gh_set_lj: bad arguments");
+  c->lj_eps = eps;
+  c->lj_sigma = sigma;
+  c->lj_rc = rc;
+  c->pot = GH_POT_LJ;
+  c->rcut = rc;
+  c->ready = false;
+  return 0;
+}
+
+int gh_set_skin(gh_ctx *c, double skin) {
+  if (!c || !(skin >= 0.0)) return gh_fail(c, GH_EINVAL, "skin must be >= 0");
+  c->skin = skin;
+  c->ready = false;
+  return 0;
+}
+
+static int interact(gh_ctx *c) {
+  GH_TRY(ensure_ready(c));
+  if (c->pos_dirty) {
+    GH_TRY(gh_launch_rebuild(c, true));
+    c->pos_dirty = false;
+  } else {
+    GH_TRY(gh_launch_rebuild(c, false));
+  }
+  GH_TRY(gh_launch_forces(c, nullptr));
+  c->first_force = false;
+  return 0;
+}
+
+int gh_compute(gh_ctx *c, double *F, double *e_atom, double *e_tot) {
+  if (!c) return GH_EINVAL;
+  GH_HIP(c, hipSetDevice(c->device));
+  GH_TRY(interact(c));
+  GH_TRY(check_status(c));
+  std::vector<int> orig;
+  GH_TRY(fetch_orig(c, orig));
+  if (F) GH_TRY(fetch_soa3(c, c->frc, orig, F));
+  if (e_atom || e_tot) {
+    std::vector<double> h(c->n);
+    GH_HIP(c, hipMemcpyAsync(h.data(), c->epa, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                             c->stream));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    if (e_atom)
+      for (int s = 0; s < c->n; ++s) e_atom[orig[s]] = h[s];
+    if (e_tot) {
+      double ep, ek;
+      GH_TRY(gh_launch_energies(c, &ep, &ek));
+      *e_tot = ep;
+    }
+  }
+  return 0;
+}
+
+int gh_neighbors(gh_ctx *c, int *nn_out, int *idx, int cap) {
+  if (!c || !nn_out || !idx || cap <= 0) return gh_fail(c, GH_EINVAL, "gh_neighbors: bad arguments");
+  if (!c->ready) GH_TRY(interact(c));
+  std::vector<int> orig;
+  GH_TRY(fetch_orig(c, orig));
+  std::vector<int> nn(c->n);
+  std::vector<unsigned> lst((size_t)c->cap * c->ncap);
+  GH_HIP(c, hipMemcpyAsync(nn.data(), c->nn, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+  GH_HIP(c, hipMemcpyAsync(lst.data(), c->list, sizeof(unsigned) * lst.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = 0;
+  std::vector<int> row;
+  for (int s = 0; s < c->n; ++s) {
+    row.clear();
+    for (int k = 0; k < nn[s]; ++k) row.push_back(orig[lst[(size_t)k * c->ncap + s] & 0x07FFFFFFu]);
+    std::sort(row.begin(), row.end());
+    const int i = orig[s];
+    const int m = std::min((int)row.size(), cap);
+    if ((int)row.size() > cap) rc = GH_EOVERFLOW;
+    nn_out[i] = m;
+    for (int k = 0; k < m; ++k) idx[(size_t)i * cap + k] = row[k];
+  }
+  if (rc) return gh_fail(c, rc, "gh_neighbors: cap too small");
+  return 0;
+}
+
+int gh_prng_lcg(gh_ctx *c, int seed) {
+  if (!c) return GH_EINVAL;
+  c->lcg.seed(seed);
+  return 0;
+}
+
+// set_gdist + set_scal_vel + CM removal (Set_Properties.f90:301-377).
+int gh_set_gdist(gh_ctx *c, double T) {
+  if (!c || !c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms");
+  const int n = c->n;
+  std::vector<int> orig;
+  GH_TRY(fetch_orig(c, orig));
+  std::vector<double> v(3 * (size_t)n);
+  GH_TRY(fetch_soa3(c, c->vel, orig, v.data()));
+  std::vector<double> m(n);
+  for (int i = 0; i < n; ++i) m[i] = c->mass_t[c->type_h[i]];
+  auto cmv = [&](double cm[3]) {
+    double M = 0.0;
+    for (int i = 0; i < n; ++i) M = M + m[i];
+    if (n == 1) {
+      for (int d = 0; d < 3; ++d) cm[d] = v[d];
+      return;
+    }
+    cm[0] = cm[1] = cm[2] = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) cm[d] = cm[d] + m[i] * v[3 * (size_t)i + d];
+    for (int d = 0; d < 3; ++d) cm[d] = cm[d] / M;
+  };
+  const double kB_ui = ghc::kB_ui();
+  double cm[3];
+  cmv(cm);
+  const double factor = kB_ui * T;
+  for (int i = 0; i < n; ++i) {
+    const double vel_med = sqrt(factor * (1.0 / m[i]));
+    for (int d = 0; d < 3; ++d) {
+      double r1;
+      c->lcg.rang(&r1, nullptr);
+      v[3 * (size_t)i + d] = vel_med * r1 + cm[d];
+    }
+  }
+  cmv(cm);
+  double ek = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double w[3];
+    for (int d = 0; d < 3; ++d) w[d] = v[3 * (size_t)i + d] - cm[d];
+    ek = ek + (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) * m[i];
+  }
+  ek = ek * 0.5;
+  const double temp = 2.0 * ek / (kB_ui * (3 * n));
+  if (temp != 0.0) {
+    const double f = sqrt(T / temp);
+    for (auto &x : v) x = x * f;
+  }
+  cmv(cm);
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = v[3 * (size_t)i + d] + (-cm[d]);
+  return put_soa3(c, c->vel, orig, v.data());
+}
+
+int gh_integrator_vv(gh_ctx *c, double dt) {
+  if (!c || !(dt != 0.0)) return gh_fail(c, GH_EINVAL, "dt must be non-zero");
+  c->integ = GH_INT_VV;
+  c->dt = dt;
+  return 0;
+}
+
+int gh_integrator_ermak(gh_ctx *c, double dt, double T, double gamma, int rng,
+                        uint64_t philox_seed) {
+  if (!c || !(dt > 0.0) || !(gamma > 0.0) || !(T >= 0.0))
+    return gh_fail(c, GH_EINVAL, "ermak needs dt > 0, gamma > 0, T >= 0");
+  if (rng != GH_RNG_LCG && rng != GH_RNG_PHILOX) return gh_fail(c, GH_EINVAL, "bad rng");
+  c->integ = GH_INT_ERMAK;
+  c->dt = dt;
+  c->T = T;
+  c->gamma = gamma;
+  c->rng_mode = rng;
+  c->philox_seed = philox_seed;
+  return 0;
+}
+
+// Draw one step of the reference stream: per atom in caller order, per axis,
+// rang(r1, r2) (ermak_a, Integration.F90:990-996).
+static int lcg_step_normals(gh_ctx *c) {
+  const int n = c->n, ncap = c->ncap;
+  if (!c->gauss) {
+    GH_TRY(dalloc(c, &c->gauss, 6 * (size_t)ncap));
+    GH_HIP(c, hipHostMalloc((void **)&c->gauss_h, sizeof(double) * 6 * ncap, hipHostMallocDefault));
+  }
+  GH_HIP(c, hipStreamSynchronize(c->stream));  // the staging buffer is free again
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double r1, r2;
+      c->lcg.rang(&r1, &r2);
+      c->gauss_h[(size_t)(2 * d) * ncap + i] = r1;
+      c->gauss_h[(size_t)(2 * d + 1) * ncap + i] = r2;
+    }
+  GH_HIP(c, hipMemcpyAsync(c->gauss, c->gauss_h, sizeof(double) * 6 * ncap,
+                           hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
+  if (!c || nsteps < 0) return GH_EINVAL;
+  if (each > 0 && !energy) return gh_fail(c, GH_EINVAL, "energy buffer required when each > 0");
+  if (!(c->dt != 0.0)) return gh_fail(c, GH_ESTATE, "no integrator: call gh_integrator_*");
+  GH_HIP(c, hipSetDevice(c->device));
+  const double ui_ev = ghc::ui_ev();
+  if (c->timing && c->evpool.empty()) {
+    c->evpool.resize((size_t)EV_STEPS * EV_PER_STEP);
+    for (auto &e : c->evpool) GH_HIP(c, hipEventCreate(&e));
+  }
+  // dinamic: one force evaluation before the loop (Programs.f90:129)
+  GH_TRY(interact(c));
+  int iout = 0;
+  const bool ermak = c->integ == GH_INT_ERMAK;
+  const bool lcg = ermak && c->rng_mode == GH_RNG_LCG;
+  for (int ns = 1; ns <= nsteps; ++ns) {
+    c->step += 1;
+    if (lcg) GH_TRY(lcg_step_normals(c));
+    GH_TRY(rec(c, 0));
+    GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
+    GH_TRY(rec(c, 1));
+    GH_TRY(gh_launch_rebuild(c, false));
+    GH_TRY(rec(c, 2));
+    GH_TRY(gh_launch_forces(c, ev_at(c, 3)));
+    GH_TRY(rec(c, 4));
+    GH_TRY(ermak ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
+    GH_TRY(rec(c, 5));
+    if (c->timing) {
+      c->evused += 1;
+      if (c->evused == EV_STEPS) GH_TRY(timing_flush(c));
+    }
+    if (each > 0 && ns % each == 0) {
+      double ep, ek;
+      GH_TRY(gh_launch_energies(c, &ep, &ek));
+      const double f1 = ep * ui_ev, f2 = ek * ui_ev;
+      energy[3 * iout + 0] = f1;
+      energy[3 * iout + 1] = f2;
+      energy[3 * iout + 2] = f1 + f2;
+      ++iout;
+    }
+  }
+  GH_TRY(timing_flush(c));
+  GH_TRY(check_status(c));
+  return 0;
+}
+
+int gh_energies(gh_ctx *c, double *epot, double *ekin) {
+  if (!c) return GH_EINVAL;
+  if (c->first_force) GH_TRY(interact(c));
+  double ep, ek;
+  GH_TRY(gh_launch_energies(c, &ep, &ek));
+  if (epot) *epot = ep * ghc::ui_ev();
+  if (ekin) *ekin = ek * ghc::ui_ev();
+  return 0;
+}
+
+int gh_get_state(gh_ctx *c, double *x, double *v, double *F, double *e) {
+  if (!c || !c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms");
+  std::vector<int> orig;
+  GH_TRY(fetch_orig(c, orig));
+  if (x) {
+    std::vector<double4> h(c->n);
+    GH_HIP(c, hipMemcpyAsync(h.data(), c->pos, sizeof(double4) * c->n, hipMemcpyDeviceToHost,
+                             c->stream));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < c->n; ++s) {
+      x[3 * (size_t)orig[s] + 0] = h[s].x;
+      x[3 * (size_t)orig[s] + 1] = h[s].y;
+      x[3 * (size_t)orig[s] + 2] = h[s].z;
+    }
+  }
+  if (v) GH_TRY(fetch_soa3(c, c->vel, orig, v));
+  if (F) GH_TRY(fetch_soa3(c, c->frc, orig, F));
+  if (e) {
+    std::vector<double> h(c->n);
+    GH_HIP(c, hipMemcpyAsync(h.data(), c->epa, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                             c->stream));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < c->n; ++s) e[orig[s]] = h[s];
+  }
+  return 0;
+}
+
+int gh_get_stats(gh_ctx *c, gh_stats *s) {
+  if (!c || !s) return GH_EINVAL;
+  GhStatus h;
+  GH_TRY(read_status(c, &h));
+  s->nupd = h.nupd;
+  s->maxnn = h.maxnn;
+  s->cap = c->cap;
+  for (int d = 0; d < 3; ++d) s->ncells[d] = c->list_mode ? c->nc[d] : 0;
+  s->list_mode = c->list_mode;
+  s->steps = c->step;
+  return 0;
+}
+
+int gh_set_timing(gh_ctx *c, int enable) {
+  if (!c) return GH_EINVAL;
+  GH_TRY(timing_flush(c));
+  c->timing = enable != 0;
+  for (int k = 0; k < 4; ++k) {
+    c->t_ms[k] = 0;
+    c->t_cnt[k] = 0;
+  }
+  return 0;
+}
+
+int gh_get_timing(gh_ctx *c, double ms[4], int64_t count[4]) {
+  if (!c || !ms || !count) return GH_EINVAL;
+  GH_TRY(timing_flush(c));
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = c->t_ms[k];
+    count[k] = c->t_cnt[k];
+  }
+  return 0;
+}
+
+}  // extern "C"

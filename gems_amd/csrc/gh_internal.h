@@ -1,0 +1,179 @@
+// Internal declarations of the gfx950 GEMS hot-path library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gems_hip.h"
+
+#define GH_MAXT 8
+#define GH_JMASK_HOST 0x07FFFFFF
+
+// ---------------------------------------------------------------------------
+// GEMS units and constants.  The reference evaluates them in real(dp) from
+// the literals of src/Constants.F90:114,124-127,173-175; the element masses
+// are default-real literals widened to real(dp) (src/Elements.F90).
+// ---------------------------------------------------------------------------
+namespace ghc {
+inline double ui_ev() {
+  const double axps_mxs = 1.0e2, uma_kg = 1.6605402e-27, qe_si = 1.60219e-19;
+  const double joule_ev = 1.0 / qe_si;
+  return axps_mxs * axps_mxs * uma_kg * joule_ev;
+}
+inline double ev_ui() { return 1.0 / ui_ev(); }
+inline double kB_ui() { return 8.617385e-05 * ev_ui(); }
+}  // namespace ghc
+
+// ---------------------------------------------------------------------------
+// Host side of the `prng lcg` stream (src/Random.F90:145-250).
+// ---------------------------------------------------------------------------
+struct GhLcg {
+  int32_t idum = -123456789, ix = -1, iy = -1;
+  bool stored = false;
+  double g = 0.0;
+  void seed(int s);
+  double ranu();
+  double gasdev();
+  void rang(double *r1, double *r2);
+};
+
+// SMA-TB coefficients for one ordered type pair (TB.F90:106-170).
+struct GhTbPair {
+  double A, xi, p, q, r0, rci, rce;
+  double b1, b2, b3, r1, r2, r3;
+};
+void gh_tb_coeffs(GhTbPair &c);
+
+// Kernel-side parameter block (one per type pair), FP64 and FP32 forms.
+struct TbDev {
+  // FP64 form
+  double ca, eps2, p, q, r0, one_r0, rci, rce, rce2, b1, b2, b3, r1, r2, r3;
+  double p_r0, q2_r0;  // p/r0, 2q/r0
+  // FP32 form: exponents as base-2 affine functions of r
+  float rep_a, rep_b;  // A*exp(-p(r/r0-1))   = exp2(rep_a + rep_b*r)
+  float rho_a, rho_b;  // xi^2*exp(-2q(r/r0-1)) = exp2(rho_a + rho_b*r)
+  float frep, fband;   // 2p/r0 and q/r0 (for -dE/dr)
+  float rci_f, rce_f;
+  float b1f, b2f, b3f, r1f, r2f, r3f;
+};
+
+struct GhStatus {                     // device-resident counters
+  unsigned long long maxdisp2_bits;   // max |dx|^2 since the last build
+  int rebuild;                        // rebuild at this step
+  int overflow;                       // a row exceeded the capacity (sticky)
+  int maxnn;                          // longest row at the last build
+  int outside;                        // an atom left a non-periodic grid
+  long long nupd;                     // rebuilds so far
+  int force_rebuild;                  // host request
+  int pad;
+};
+
+struct gh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = GH_FP64;
+  std::string err;
+
+  // box
+  int boxed = 0;
+  double L[3] = {1e6, 1e6, 1e6};
+  int pbc[3] = {0, 0, 0};
+
+  // atoms
+  int n = 0, ncap = 0;
+  int nt = 1;
+  double mass_t[GH_MAXT] = {1, 1, 1, 1, 1, 1, 1, 1};
+  bool have_atoms = false;
+
+  // potential
+  int pot = GH_POT_NONE;
+  GhTbPair tb[GH_MAXT * GH_MAXT];
+  double lj_eps = 0, lj_sigma = 0, lj_rc = 0;
+  double rcut = 0;      // potential cut radius
+  double skin = 1.0;    // nb_dcut default (Neighbour.F90:237)
+
+  // integrator
+  int integ = GH_INT_VV;
+  double dt = 0.0;
+  double T = 0.0, gamma = 1.0;
+  int rng_mode = GH_RNG_LCG;
+  uint64_t philox_seed = 0;
+  GhLcg lcg;
+  bool first_force = true;   // forces not yet valid (interact not called)
+  long long step = 0;
+
+  // neighbour grid
+  int list_mode = 1;          // 0 = O(N^2), 1 = cells
+  int nc[3] = {1, 1, 1};      // cells per dim (interior)
+  int gofs[3] = {0, 0, 0};    // 1 for a non-periodic dim (one guard cell)
+  int gn[3] = {1, 1, 1};      // cells per dim including guard cells
+  double cell[3] = {1, 1, 1}; // cell edge
+  int ncell = 1;
+  int cap = 0;                // list row capacity
+  int cap_req = 0;
+
+  // device state (cell-sorted order)
+  double4 *pos = nullptr, *pos_t = nullptr;   // x,y,z,(type as double)
+  double *vel = nullptr, *vel_t = nullptr;    // [3][ncap] SoA
+  double *acc = nullptr, *acc_t = nullptr;    // [3][ncap]
+  double *frc = nullptr;                      // [3][ncap]
+  double *epa = nullptr;                      // [ncap] per-atom energy (ui)
+  double *rho = nullptr, *eband = nullptr;    // [ncap]
+  float *eband32 = nullptr;                   // [ncap]
+  int *type = nullptr, *type_t = nullptr;     // [ncap]
+  int *orig = nullptr, *orig_t = nullptr;     // caller index of sorted atom
+  double *xold = nullptr;                     // [3][ncap] positions at build
+  double *posv = nullptr;                     // Ermak random displacement [3][ncap]
+  double *velv = nullptr;                     // Ermak random velocity [3][ncap]
+  double *gauss = nullptr;                    // host-stream gaussians [6][ncap] (caller order)
+  double *gauss_h = nullptr;                  // pinned host staging
+  int *cellof = nullptr;                      // [ncap] cell of sorted atom
+  int *cellid = nullptr;                      // [ncap] cell of unsorted atom (rebuild scratch)
+  int *cell_count = nullptr, *cell_start = nullptr, *cell_cursor = nullptr;
+  int *perm = nullptr;                        // [ncap]
+  int *nn = nullptr;                          // [ncap]
+  int *list = nullptr;                        // [cap][ncap] ELL
+  TbDev *tbd = nullptr;                       // [MAXT*MAXT]
+  GhStatus *status = nullptr;
+  double *red = nullptr;                      // reduction scratch
+  int nred = 0;
+  double *red_h = nullptr;                    // pinned result
+  int cell_alloc = 0;
+
+  // timing: a pool of events, 8 per step, read back when the pool fills
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  int evused = 0;  // steps recorded in the pool
+  double t_ms[4] = {0, 0, 0, 0};
+  int64_t t_cnt[4] = {0, 0, 0, 0};
+
+  // host copies in the caller's order
+  std::vector<int> type_h;
+  bool ready = false;       // grid and list sized for the current setup
+  bool pos_dirty = true;    // host changed positions since the last build
+};
+
+// error helpers
+int gh_fail(gh_ctx *c, int code, const std::string &msg);
+#define GH_HIP(c, call)                                                    \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return gh_fail(c, GH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// launchers (gh_kernels.hip)
+int gh_launch_rebuild(gh_ctx *c, bool force);
+int gh_launch_forces(gh_ctx *c, hipEvent_t mid);
+int gh_launch_vv_a(gh_ctx *c);
+int gh_launch_vv_b(gh_ctx *c);
+int gh_launch_ermak_a(gh_ctx *c);
+int gh_launch_ermak_b(gh_ctx *c);
+int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui);
+int gh_count_neighbours(gh_ctx *c, int *maxnn);
+int gh_fill_int(gh_ctx *c, int *p, int v, int n);
+int gh_launch_wrap(gh_ctx *c);
+int gh_upload_tb(gh_ctx *c);

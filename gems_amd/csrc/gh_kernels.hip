@@ -1,0 +1,1196 @@
+// gfx950 kernels of the GEMS force-evaluation hot path.
+//
+// Data layout (HBM, cell-sorted order after each rebuild):
+//   pos    double4[ncap]   x, y, z, w unused             32 B/atom
+//   vel, acc, frc          3 x double[ncap] (SoA)         24 B/atom each
+//   epa, rho, eband        double[ncap]
+//   eband32                float[ncap] (mixed precision gathers)
+//   type, orig, cellof     int[ncap]
+//   list   uint32[cap][ld] ELL: row k of all atoms is contiguous, so a
+//          wavefront reading entry k of 64 consecutive atoms is coalesced.
+//          entry = j | (image code << 27); image code w = (wx+1)+3(wy+1)+9(wz+1)
+//          with the neighbour's periodic image at x_j + wd*L_d.
+// Wavefronts are 64 lanes; per-atom kernels use 256-thread workgroups.
+#include <math.h>
+
+#include "gh_internal.h"
+
+#define GH_BLOCK 256
+#define GH_JMASK 0x07FFFFFFu
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_imax(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block-wide sum of NV doubles; deterministic order.  Result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double part[GH_BLOCK / 64][NV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[wid][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < GH_BLOCK / 64; ++w) s += part[w][k];
+      v[k] = s;
+    }
+  }
+}
+
+__global__ void k_fill_int(int *__restrict__ p, int v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Rebuild decision (test_update, Neighbour.F90:1305-1346).  The reference
+// rebuilds when the two largest displacements since the last build sum past
+// the skin.  Here the displacement is the true (minimum-image) one, and the
+// test uses 2*max, which bounds d1+d2 from above.
+// ---------------------------------------------------------------------------
+__global__ void k_decide(GhStatus *st, double skin) {
+  double d2 = __longlong_as_double((long long)st->maxdisp2_bits);
+  int rb = st->force_rebuild || (2.0 * sqrt(d2) > skin);
+  st->rebuild = rb;
+  if (rb) {
+    st->nupd += 1;
+    st->force_rebuild = 0;
+    st->maxnn = 0;
+  }
+  st->maxdisp2_bits = 0ull;
+}
+
+struct GridArgs {
+  int nc[3], gn[3], gofs[3], pbc[3];
+  double L[3], one_cell[3];
+  int ncell;
+};
+
+__device__ __forceinline__ int cell_coord(double x, const GridArgs &g, int d,
+                                          int *outside) {
+  int c = (int)floor(x * g.one_cell[d]);
+  if (g.pbc[d]) {
+    c %= g.nc[d];
+    if (c < 0) c += g.nc[d];
+  } else {
+    c += g.gofs[d];
+    if (c < 0) { c = 0; *outside = 1; }
+    if (c >= g.gn[d]) { c = g.gn[d] - 1; *outside = 1; }
+  }
+  return c;
+}
+
+__global__ void k_zero_cells(const GhStatus *st, int *__restrict__ count, int ncell) {
+  if (!st->rebuild) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell) count[c] = 0;
+}
+
+__global__ void k_bin(GhStatus *st, const double4 *__restrict__ pos, int n, GridArgs g,
+                      int *__restrict__ cellid, int *__restrict__ count) {
+  if (!st->rebuild) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = pos[i];
+  int out = 0;
+  int cx = cell_coord(p.x, g, 0, &out);
+  int cy = cell_coord(p.y, g, 1, &out);
+  int cz = cell_coord(p.z, g, 2, &out);
+  if (out) st->outside = 1;
+  int c = cx + g.gn[0] * (cy + g.gn[1] * cz);
+  cellid[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of the cell counts in one 1024-thread workgroup.
+__global__ __launch_bounds__(1024) void k_scan(const GhStatus *st, const int *__restrict__ count,
+                                               int *__restrict__ start, int *__restrict__ cursor,
+                                               int ncell) {
+  if (!st->rebuild) return;
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (ncell + 1023) / 1024;
+  const int b = t * per, e = min(b + per, ncell);
+  int s = 0;
+  for (int c = b; c < e; ++c) s += count[c];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = (t >= o) ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;  // exclusive prefix of this thread's chunk
+  for (int c = b; c < e; ++c) {
+    start[c] = run;
+    cursor[c] = run;
+    run += count[c];
+  }
+  if (t == 1023) start[ncell] = part[1023];
+}
+
+__global__ void k_scatter(const GhStatus *st, const int *__restrict__ cellid, int n,
+                          int *__restrict__ cursor, int *__restrict__ perm) {
+  if (!st->rebuild) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int slot = atomicAdd(&cursor[cellid[i]], 1);
+  perm[slot] = i;
+}
+
+// Order the atoms of each cell by their caller index: the sorted layout is
+// then independent of the atomic scatter order (deterministic results).
+__global__ void k_sort_cells(const GhStatus *st, const int *__restrict__ start,
+                             const int *__restrict__ orig, int *__restrict__ perm,
+                             int ncell) {
+  if (!st->rebuild) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = start[c], e = start[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    int v = perm[k];
+    int key = orig[v];
+    int m = k - 1;
+    while (m >= b && orig[perm[m]] > key) {
+      perm[m + 1] = perm[m];
+      --m;
+    }
+    perm[m + 1] = v;
+  }
+}
+
+__global__ void k_gather(const GhStatus *st, const int *__restrict__ perm, int n, int ncap,
+                         const int *__restrict__ cellid, const double4 *__restrict__ pos,
+                         const double *__restrict__ vel, const double *__restrict__ acc,
+                         const int *__restrict__ type, const int *__restrict__ orig,
+                         double4 *__restrict__ pos_t, double *__restrict__ vel_t,
+                         double *__restrict__ acc_t, int *__restrict__ type_t,
+                         int *__restrict__ orig_t, int *__restrict__ cellof) {
+  if (!st->rebuild) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = perm[s];
+  pos_t[s] = pos[o];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel_t[d * ncap + s] = vel[d * ncap + o];
+    acc_t[d * ncap + s] = acc[d * ncap + o];
+  }
+  type_t[s] = type[o];
+  orig_t[s] = orig[o];
+  cellof[s] = cellid[o];
+}
+
+__global__ void k_copyback(const GhStatus *st, int n, int ncap, const double4 *__restrict__ pos_t,
+                           const double *__restrict__ vel_t, const double *__restrict__ acc_t,
+                           const int *__restrict__ type_t, const int *__restrict__ orig_t,
+                           double4 *__restrict__ pos, double *__restrict__ vel,
+                           double *__restrict__ acc, int *__restrict__ type,
+                           int *__restrict__ orig) {
+  if (!st->rebuild) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  pos[s] = pos_t[s];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel[d * ncap + s] = vel_t[d * ncap + s];
+    acc[d * ncap + s] = acc_t[d * ncap + s];
+  }
+  type[s] = type_t[s];
+  orig[s] = orig_t[s];
+}
+
+// |vd|^2 rounded exactly as the reference's dot_product (no contraction).
+__device__ __forceinline__ double dist2_rn(double x, double y, double z) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+// Cell-list build (linkedcell_self, Neighbour.F90:991-1087): full list at
+// radius rl, rd < rl^2 kept, i itself excluded.  vd = x_j - x_i, then the
+// periodic image shift, which equals the reference's vd - L*idnint(vd/L)
+// for every pair within rl (rl <= L/3 on periodic axes).
+__global__ __launch_bounds__(GH_BLOCK) void k_build_cells(
+    GhStatus *st, const double4 *__restrict__ pos, const int *__restrict__ cellof,
+    const int *__restrict__ start, int n, GridArgs g, double rl2, int cap, int ld,
+    int count_only, int *__restrict__ nn, unsigned *__restrict__ list,
+    double *__restrict__ xold, int ncap) {
+  if (!st->rebuild) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = 0;
+  if (s < n) {
+    const double4 pi = pos[s];
+    const int c = cellof[s];
+    const int cx = c % g.gn[0], cy = (c / g.gn[0]) % g.gn[1], cz = c / (g.gn[0] * g.gn[1]);
+    for (int oz = -1; oz <= 1; ++oz) {
+      int nz = cz + oz, wz = 0;
+      if (g.pbc[2]) {
+        if (nz < 0) { nz += g.nc[2]; wz = -1; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; wz = 1; }
+      } else if (nz < 0 || nz >= g.gn[2]) continue;
+      for (int oy = -1; oy <= 1; ++oy) {
+        int ny = cy + oy, wy = 0;
+        if (g.pbc[1]) {
+          if (ny < 0) { ny += g.nc[1]; wy = -1; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; wy = 1; }
+        } else if (ny < 0 || ny >= g.gn[1]) continue;
+        for (int ox = -1; ox <= 1; ++ox) {
+          int nx = cx + ox, wx = 0;
+          if (g.pbc[0]) {
+            if (nx < 0) { nx += g.nc[0]; wx = -1; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; wx = 1; }
+          } else if (nx < 0 || nx >= g.gn[0]) continue;
+          const int c2 = nx + g.gn[0] * (ny + g.gn[1] * nz);
+          const unsigned code = (unsigned)((wx + 1) + 3 * (wy + 1) + 9 * (wz + 1)) << 27;
+          const double shx = wx * g.L[0], shy = wy * g.L[1], shz = wz * g.L[2];
+          const int jb = start[c2], je = start[c2 + 1];
+          for (int j = jb; j < je; ++j) {
+            if (j == s) continue;
+            const double4 pj = pos[j];
+            double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+            if (wx) vx = vx + shx;
+            if (wy) vy = vy + shy;
+            if (wz) vz = vz + shz;
+            const double rd = dist2_rn(vx, vy, vz);
+            if (rd < rl2) {
+              if (!count_only && m < cap) list[(size_t)m * ld + s] = (unsigned)j | code;
+              ++m;
+            }
+          }
+        }
+      }
+    }
+    if (!count_only) {
+      nn[s] = min(m, cap);
+      xold[s] = pi.x;
+      xold[ncap + s] = pi.y;
+      xold[2 * ncap + s] = pi.z;
+    }
+  }
+  int wm = wave_imax(m);
+  if ((threadIdx.x & 63) == 0 && wm > 0) {
+    atomicMax(&st->maxnn, wm);
+    if (wm > cap) st->overflow = 1;
+  }
+}
+
+// O(N^2) build (verlet_self, Neighbour.F90:676-759): rd > rl^2 -> skip, so
+// equality is kept; minimum image by rounding as vdistance does.
+__global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
+    GhStatus *st, const double4 *__restrict__ pos, int n, GridArgs g, double rl2,
+    int cap, int ld, int count_only, int *__restrict__ nn, unsigned *__restrict__ list,
+    double *__restrict__ xold, int ncap) {
+  if (!st->rebuild) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = 0;
+  if (s < n) {
+    const double4 pi = pos[s];
+    const double ob0 = 1.0 / g.L[0], ob1 = 1.0 / g.L[1], ob2 = 1.0 / g.L[2];
+    for (int j = 0; j < n; ++j) {
+      if (j == s) continue;
+      const double4 pj = pos[j];
+      double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+      int wx = 0, wy = 0, wz = 0;
+      if (g.pbc[0]) { wx = -(int)round(vx * ob0); vx = vx + wx * g.L[0]; }
+      if (g.pbc[1]) { wy = -(int)round(vy * ob1); vy = vy + wy * g.L[1]; }
+      if (g.pbc[2]) { wz = -(int)round(vz * ob2); vz = vz + wz * g.L[2]; }
+      const double rd = dist2_rn(vx, vy, vz);
+      if (rd <= rl2) {
+        if (!count_only && m < cap) {
+          const unsigned code = (unsigned)((wx + 1) + 3 * (wy + 1) + 9 * (wz + 1)) << 27;
+          list[(size_t)m * ld + s] = (unsigned)j | code;
+        }
+        ++m;
+      }
+    }
+    if (!count_only) {
+      nn[s] = min(m, cap);
+      xold[s] = pi.x;
+      xold[ncap + s] = pi.y;
+      xold[2 * ncap + s] = pi.z;
+    }
+  }
+  int wm = wave_imax(m);
+  if ((threadIdx.x & 63) == 0 && wm > 0) {
+    atomicMax(&st->maxnn, wm);
+    if (wm > cap) st->overflow = 1;
+  }
+}
+
+// image shift table in LDS: shift[w] = (wx*L0, wy*L1, wz*L2)
+struct ShiftLds {
+  double x[27], y[27], z[27];
+};
+__device__ __forceinline__ void load_shift(ShiftLds &sh, double L0, double L1, double L2) {
+  if (threadIdx.x < 27) {
+    int w = threadIdx.x;
+    sh.x[w] = (w % 3 - 1) * L0;
+    sh.y[w] = ((w / 3) % 3 - 1) * L1;
+    sh.z[w] = (w / 9 - 1) * L2;
+  }
+}
+
+__device__ __forceinline__ void load_params(TbDev *sp, const TbDev *__restrict__ tbd, int nt) {
+  for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) {
+    int a = k / GH_MAXT, b = k % GH_MAXT;
+    if (a < nt && b < nt) sp[k] = tbd[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SMA-TB pass 1 (smatb_preinteraction, TB.F90:367-455):
+//   rho_i = sum_j [ r < rci ? xi^2 exp(-2q(r/r0-1)) : (b1 x^5 + b2 x^4 + b3 x^3)^2 ]
+//   with x = r - rce and r > rce skipped; eband_i = 1/sqrt(rho_i).
+// FP64 form.
+// ---------------------------------------------------------------------------
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_BLOCK) void k_tb_rho64(
+    const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
+    const unsigned *__restrict__ list, int ld, int n, const TbDev *__restrict__ tbd, int nt,
+    double L0, double L1, double L2, double *__restrict__ rho, double *__restrict__ eband,
+    float *__restrict__ eband32) {
+  __shared__ ShiftLds sh;
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  load_shift(sh, L0, L1, L2);
+  if (SINGLE) {
+    if (threadIdx.x == 0) sp[0] = tbd[0];
+  } else {
+    load_params(sp, tbd, nt);
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int ti = SINGLE ? 0 : type[i];
+  const int m = nn[i];
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = list[(size_t)k * ld + i];
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
+    const double dr = sqrt(dist2_rn(vx, vy, vz));
+    if (dr > P.rce) continue;
+    double banda;
+    if (dr < P.rci) {
+      banda = P.eps2 * exp(-2.0 * P.q * (dr * P.one_r0 - 1.0));
+    } else {
+      const double d = dr - P.rce;
+      const double d2 = d * d, d3 = d2 * d, d4 = d3 * d, d5 = d4 * d;
+      banda = P.b1 * d5 + P.b2 * d4 + P.b3 * d3;
+      banda = banda * banda;
+    }
+    acc += banda;
+  }
+  rho[i] = acc;
+  const double eb = 1.0 / sqrt(acc);
+  eband[i] = eb;
+  eband32[i] = (float)eb;
+}
+
+// SMA-TB pass 2 (smatb_interact, TB.F90:457-586), full list, newton off:
+//   r <= rci: repul = A e^{-p(r/r0-1)};
+//             factor = 2 repul p/r0 - xi^2/2 e^{-2q(r/r0-1)} (2q/r0)(eb_i+eb_j)
+//   rci < r <= rce: factor = P P' (eb_i+eb_j) - 2 R';  repul = R
+//   F_i -= factor * vd / r  (vd = x_j - x_i), e_i = sum repul - sqrt(rho_i).
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_BLOCK) void k_tb_force64(
+    const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
+    const unsigned *__restrict__ list, int ld, int n, int ncap, const TbDev *__restrict__ tbd,
+    int nt, double L0, double L1, double L2, const double *__restrict__ rho,
+    const double *__restrict__ eband, double ev_ui, double *__restrict__ frc,
+    double *__restrict__ epa) {
+  __shared__ ShiftLds sh;
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  load_shift(sh, L0, L1, L2);
+  if (SINGLE) {
+    if (threadIdx.x == 0) sp[0] = tbd[0];
+  } else {
+    load_params(sp, tbd, nt);
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int ti = SINGLE ? 0 : type[i];
+  const double ebi = eband[i];
+  const int m = nn[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = list[(size_t)k * ld + i];
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
+    const double dr = sqrt(dist2_rn(vx, vy, vz));
+    if (dr > P.rce) continue;
+    double factor, repul;
+    if (dr <= P.rci) {
+      const double t = dr * P.one_r0 - 1.0;
+      repul = P.ca * exp(-P.p * t);
+      factor = 2.0 * repul * P.p_r0 -
+               P.eps2 * 0.5 * exp(-2.0 * P.q * t) * P.q2_r0 * (ebi + eband[j]);
+    } else {
+      const double d = dr - P.rce;
+      const double d2 = d * d, d3 = d2 * d, d4 = d3 * d, d5 = d4 * d;
+      factor = (P.b1 * d5 + P.b2 * d4 + P.b3 * d3) *
+               (5.0 * P.b1 * d4 + 4.0 * P.b2 * d3 + 3.0 * P.b3 * d2) * (ebi + eband[j]);
+      factor = factor - 2.0 * (5.0 * P.r1 * d4 + 4.0 * P.r2 * d3 + 3.0 * P.r3 * d2);
+      repul = P.r1 * d5 + P.r2 * d4 + P.r3 * d3;
+    }
+    const double s = factor / dr;
+    fx -= s * vx;
+    fy -= s * vy;
+    fz -= s * vz;
+    ei += repul;
+  }
+  frc[i] = fx * ev_ui;
+  frc[ncap + i] = fy * ev_ui;
+  frc[2 * ncap + i] = fz * ev_ui;
+  epa[i] = (ei - sqrt(rho[i])) * ev_ui;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision SMA-TB: positions and the pair vector stay FP64 (the
+// difference x_j - x_i is exact to FP64 rounding); distance, exponentials and
+// the pair force factor are FP32; rho, energies and forces accumulate in FP64.
+// ---------------------------------------------------------------------------
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_BLOCK) void k_tb_rho_mx(
+    const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
+    const unsigned *__restrict__ list, int ld, int n, const TbDev *__restrict__ tbd, int nt,
+    double L0, double L1, double L2, double *__restrict__ rho, double *__restrict__ eband,
+    float *__restrict__ eband32) {
+  __shared__ ShiftLds sh;
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  load_shift(sh, L0, L1, L2);
+  if (SINGLE) {
+    if (threadIdx.x == 0) sp[0] = tbd[0];
+  } else {
+    load_params(sp, tbd, nt);
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int ti = SINGLE ? 0 : type[i];
+  const int m = nn[i];
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = list[(size_t)k * ld + i];
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
+    const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+    if (r2 > P.rce2) continue;
+    const float r = __fsqrt_rn((float)r2);
+    float phi;
+    if (r < P.rci_f) {
+      phi = exp2f(fmaf(P.rho_b, r, P.rho_a));
+    } else {
+      const float d = r - P.rce_f;
+      const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
+      phi = pp * pp;
+    }
+    acc += (double)phi;
+  }
+  rho[i] = acc;
+  const double eb = 1.0 / sqrt(acc);
+  eband[i] = eb;
+  eband32[i] = (float)eb;
+}
+
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
+    const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
+    const unsigned *__restrict__ list, int ld, int n, int ncap, const TbDev *__restrict__ tbd,
+    int nt, double L0, double L1, double L2, const double *__restrict__ rho,
+    const float *__restrict__ eband32, double ev_ui, double *__restrict__ frc,
+    double *__restrict__ epa) {
+  __shared__ ShiftLds sh;
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  load_shift(sh, L0, L1, L2);
+  if (SINGLE) {
+    if (threadIdx.x == 0) sp[0] = tbd[0];
+  } else {
+    load_params(sp, tbd, nt);
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int ti = SINGLE ? 0 : type[i];
+  const float ebi = eband32[i];
+  const int m = nn[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = list[(size_t)k * ld + i];
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
+    const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+    if (r2 > P.rce2) continue;
+    const float r2f = (float)r2;
+    const float r = __fsqrt_rn(r2f);
+    const float rinv = 1.0f / r;
+    const float ebs = ebi + eband32[j];
+    float factor, repul;
+    if (r <= P.rci_f) {
+      repul = exp2f(fmaf(P.rep_b, r, P.rep_a));
+      const float band = exp2f(fmaf(P.rho_b, r, P.rho_a));
+      factor = fmaf(repul, P.frep, -band * P.fband * ebs);
+    } else {
+      const float d = r - P.rce_f;
+      const float d2 = d * d, d3 = d2 * d;
+      const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
+      const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
+      const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
+      factor = fmaf(pp * dp, ebs, -2.0f * dr_);
+      repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
+    }
+    const double s = (double)(factor * rinv);
+    fx = fma(-s, vx, fx);
+    fy = fma(-s, vy, fy);
+    fz = fma(-s, vz, fz);
+    ei += (double)repul;
+  }
+  frc[i] = fx * ev_ui;
+  frc[ncap + i] = fy * ev_ui;
+  frc[2 * ncap + i] = fz * ev_ui;
+  epa[i] = (ei - sqrt(rho[i])) * ev_ui;
+}
+
+// ---------------------------------------------------------------------------
+// Lennard-Jones (lj_interact, Pairs.F90:207-279), full list, newton off:
+//   r^2 > rc^2 skipped; p = 4e(s6^2 - s6); f = e(48 s6 - 24) s6 / r;
+//   e_i += p/2; F_i -= f vd / r.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GH_BLOCK) void k_lj64(
+    const double4 *__restrict__ pos, const int *__restrict__ nn, const unsigned *__restrict__ list,
+    int ld, int n, int ncap, double eps, double sigma, double rc2, double L0, double L1,
+    double L2, double ev_ui, double *__restrict__ frc, double *__restrict__ epa) {
+  __shared__ ShiftLds sh;
+  load_shift(sh, L0, L1, L2);
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int m = nn[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = list[(size_t)k * ld + i];
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const double r2 = dist2_rn(vx, vy, vz);
+    if (r2 > rc2) continue;
+    const double dr = sqrt(r2);
+    const double t = sigma / dr;
+    const double t2 = t * t, t4 = t2 * t2, s6 = t4 * t2;
+    const double p = 4.0 * eps * (s6 * s6 - s6);
+    const double f = eps * (48.0 * s6 - 24.0) * s6 / dr;
+    ei += p;
+    const double s = f / dr;
+    fx -= s * vx;
+    fy -= s * vy;
+    fz -= s * vz;
+  }
+  frc[i] = fx * ev_ui;
+  frc[ncap + i] = fy * ev_ui;
+  frc[2 * ncap + i] = fz * ev_ui;
+  epa[i] = ei * 0.5 * ev_ui;
+}
+
+__global__ __launch_bounds__(GH_BLOCK) void k_lj_mx(
+    const double4 *__restrict__ pos, const int *__restrict__ nn, const unsigned *__restrict__ list,
+    int ld, int n, int ncap, float eps, float sigma2, double rc2, double L0, double L1,
+    double L2, double ev_ui, double *__restrict__ frc, double *__restrict__ epa) {
+  __shared__ ShiftLds sh;
+  load_shift(sh, L0, L1, L2);
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int m = nn[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = list[(size_t)k * ld + i];
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+    if (r2 > rc2) continue;
+    const float ir2 = 1.0f / (float)r2;
+    const float s2 = sigma2 * ir2;
+    const float s6 = s2 * s2 * s2;
+    const float p = 4.0f * eps * (s6 * s6 - s6);
+    const float fr2 = eps * fmaf(48.0f, s6, -24.0f) * s6 * ir2;  // f / r
+    ei += (double)p;
+    const double s = (double)fr2;
+    fx = fma(-s, vx, fx);
+    fy = fma(-s, vy, fy);
+    fz = fma(-s, vz, fz);
+  }
+  frc[i] = fx * ev_ui;
+  frc[ncap + i] = fy * ev_ui;
+  frc[2 * ncap + i] = fz * ev_ui;
+  epa[i] = ei * 0.5 * ev_ui;
+}
+
+// ---------------------------------------------------------------------------
+// Integrators.  Per-type masses come in the kernel argument block.
+// ---------------------------------------------------------------------------
+struct MassArgs {
+  double om[GH_MAXT];   // 1/m
+  double osm[GH_MAXT];  // sqrt(1/m)
+};
+
+struct WrapArgs {
+  int pbc[3];
+  double L[3];
+};
+
+// do_pbc_1 (Set_Properties.f90:444-470): one box length at most.
+__device__ __forceinline__ double wrap1(double x, int pbc, double L) {
+  if (!pbc) return x;
+  if (x >= L) return x - L;
+  if (x < 0.0) return x + L;
+  return x;
+}
+
+__device__ __forceinline__ double min_image(double d, int pbc, double L) {
+  if (!pbc) return d;
+  if (d > 0.5 * L) return d - L;
+  if (d < -0.5 * L) return d + L;
+  return d;
+}
+
+__device__ __forceinline__ void disp_reduce(double d2, GhStatus *st) {
+  double wm = wave_max(d2);
+  if ((threadIdx.x & 63) == 0 && wm > 0.0)
+    atomicMax(&st->maxdisp2_bits, (unsigned long long)__double_as_longlong(wm));
+}
+
+// velocity_verlet_a (Integration.F90:1150-1169) + do_pbc + displacement.
+__global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
+    int n, int ncap, double dt, MassArgs ma, WrapArgs wa, const int *__restrict__ type,
+    double4 *__restrict__ pos, double *__restrict__ vel, double *__restrict__ acc,
+    const double *__restrict__ frc, const double *__restrict__ xold, GhStatus *st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    const double acel_dv = dt * 0.5, acel_dx = acel_dv * dt;
+    const double om = ma.om[type[i]];
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double a = frc[d * ncap + i] * om;
+      const double v = vel[d * ncap + i];
+      x[d] = x[d] + dt * v + acel_dx * a;
+      vel[d * ncap + i] = v + acel_dv * a;
+      acc[d * ncap + i] = a;
+      x[d] = wrap1(x[d], wa.pbc[d], wa.L[d]);
+      const double dd = min_image(x[d] - xold[d * ncap + i], wa.pbc[d], wa.L[d]);
+      d2 += dd * dd;
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    pos[i] = p;
+  }
+  disp_reduce(d2, st);
+}
+
+// velocity_verlet_b (Integration.F90:1171-1186).
+__global__ __launch_bounds__(GH_BLOCK) void k_vv_b(int n, int ncap, double dt, MassArgs ma,
+                                                   const int *__restrict__ type,
+                                                   double *__restrict__ vel,
+                                                   double *__restrict__ acc,
+                                                   const double *__restrict__ frc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double acel_dv = dt * 0.5;
+  const double om = ma.om[type[i]];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double a = frc[d * ncap + i] * om;
+    acc[d * ncap + i] = a;
+    vel[d * ncap + i] = vel[d * ncap + i] + acel_dv * a;
+  }
+}
+
+// Philox4x32-10 (Salmon et al., SC'11): counter-based stream.
+__device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
+__device__ __forceinline__ double u01(unsigned a, unsigned b) {
+  // 53-bit uniform in (0,1)
+  const unsigned long long v = ((unsigned long long)a << 21) ^ (unsigned long long)b;
+  return ((double)(v & ((1ull << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Six standard normals for atom `id` at step `step` (Box-Muller).
+__device__ __forceinline__ void philox_normals(unsigned long long seed, long long step, int id,
+                                               double g[6]) {
+  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  double u[6];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const uint4 c = philox(make_uint4((unsigned)id, (unsigned)b, (unsigned)step,
+                                      (unsigned)((unsigned long long)step >> 32)), key);
+    u[2 * b] = u01(c.x, c.y);
+    u[2 * b + 1] = u01(c.z, c.w);
+  }
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const double r = sqrt(-2.0 * log(u[2 * b]));
+    double sn, cs;
+    sincospi(2.0 * u[2 * b + 1], &sn, &cs);
+    g[2 * b] = r * cs;
+    g[2 * b + 1] = r * sn;
+  }
+}
+
+struct ErmakArgs {
+  double c0, c1, c2, sdr, sdv, crv1, crv2, skt, dt;
+  int philox;
+  unsigned long long seed;
+  long long step;
+};
+
+__device__ __forceinline__ void ermak_gauss(const ErmakArgs &k, int id,
+                                            const double *__restrict__ gauss, int ncap,
+                                            double g[6]) {
+  if (k.philox) {
+    philox_normals(k.seed, k.step, id, g);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) g[q] = gauss[(size_t)q * ncap + id];
+  }
+}
+
+// ermak_a (Integration.F90:963-1011) + do_pbc + displacement.
+__global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
+    int n, int ncap, ErmakArgs k, MassArgs ma, WrapArgs wa, const int *__restrict__ type,
+    const int *__restrict__ orig, const double *__restrict__ gauss, double4 *__restrict__ pos,
+    const double *__restrict__ vel, const double *__restrict__ acc,
+    const double *__restrict__ xold, GhStatus *st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    double g[6];
+    ermak_gauss(k, orig[i], gauss, ncap, g);
+    const double f_lan = k.skt * ma.osm[type[i]];
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double pos_v = f_lan * k.sdr * g[2 * d];
+      x[d] = x[d] + k.c1 * vel[d * ncap + i] + k.c2 * k.dt * acc[d * ncap + i] + pos_v;
+      x[d] = wrap1(x[d], wa.pbc[d], wa.L[d]);
+      const double dd = min_image(x[d] - xold[d * ncap + i], wa.pbc[d], wa.L[d]);
+      d2 += dd * dd;
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    pos[i] = p;
+  }
+  disp_reduce(d2, st);
+}
+
+// ermak_b (Integration.F90:1013-1031); vel_v is re-derived from the same
+// normals that ermak_a used (same stream entry), not stored.
+__global__ __launch_bounds__(GH_BLOCK) void k_ermak_b(
+    int n, int ncap, ErmakArgs k, MassArgs ma, const int *__restrict__ type,
+    const int *__restrict__ orig, const double *__restrict__ gauss, double *__restrict__ vel,
+    double *__restrict__ acc, const double *__restrict__ frc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double g[6];
+  ermak_gauss(k, orig[i], gauss, ncap, g);
+  const int t = type[i];
+  const double f_lan = k.skt * ma.osm[t];
+  const double om = ma.om[t];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double vel_v = f_lan * k.sdv * (k.crv1 * g[2 * d] + k.crv2 * g[2 * d + 1]);
+    const double f = frc[d * ncap + i];
+    vel[d * ncap + i] = k.c0 * vel[d * ncap + i] + (k.c1 - k.c2) * acc[d * ncap + i] +
+                        k.c2 * f * om + vel_v;
+    acc[d * ncap + i] = f * om;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Energies (write_energy, Output.F90:824-831): deterministic two-stage sums.
+// stage A: [epot, M, Px, Py, Pz] partials; stage B: sum m |v - vcm|^2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GH_BLOCK) void k_red_a(int n, int ncap, MassArgs ma,
+                                                    const int *__restrict__ type,
+                                                    const double *__restrict__ epa,
+                                                    const double *__restrict__ vel,
+                                                    double *__restrict__ part) {
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double m = 1.0 / ma.om[type[i]];
+    v[0] += epa[i];
+    v[1] += m;
+    v[2] += m * vel[i];
+    v[3] += m * vel[ncap + i];
+    v[4] += m * vel[2 * ncap + i];
+  }
+  block_sum<5>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 5; ++k) part[blockIdx.x * 5 + k] = v[k];
+}
+
+__global__ __launch_bounds__(GH_BLOCK) void k_red_final(int nblk, int nv,
+                                                        const double *__restrict__ part,
+                                                        double *__restrict__ out) {
+  // one workgroup; nv <= 5
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+    for (int k = 0; k < nv; ++k) v[k] += part[b * nv + k];
+  block_sum<5>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nv; ++k) out[k] = v[k];
+}
+
+__global__ __launch_bounds__(GH_BLOCK) void k_red_b(int n, int ncap, MassArgs ma,
+                                                    const int *__restrict__ type,
+                                                    const double *__restrict__ vel,
+                                                    const double *__restrict__ sums,
+                                                    double *__restrict__ part) {
+  const double M = sums[1];
+  const double cx = sums[2] / M, cy = sums[3] / M, cz = sums[4] / M;
+  double v[1] = {0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double m = 1.0 / ma.om[type[i]];
+    const double wx = vel[i] - cx, wy = vel[ncap + i] - cy, wz = vel[2 * ncap + i] - cz;
+    v[0] += (wx * wx + wy * wy + wz * wz) * m;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Wraps positions into [0, L) on periodic axes (used when the box or the
+// atoms change; the timestep kernels apply do_pbc_1's single shift).
+__global__ void k_wrap(int n, WrapArgs wa, double4 *__restrict__ pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = pos[i];
+  double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (wa.pbc[d]) {
+      x[d] = x[d] - wa.L[d] * floor(x[d] / wa.L[d]);
+      if (x[d] >= wa.L[d]) x[d] -= wa.L[d];
+    }
+  p.x = x[0];
+  p.y = x[1];
+  p.z = x[2];
+  pos[i] = p;
+}
+
+inline int nblocks(int n) { return (n + GH_BLOCK - 1) / GH_BLOCK; }
+
+GridArgs make_grid(const gh_ctx *c) {
+  GridArgs g;
+  for (int d = 0; d < 3; ++d) {
+    g.nc[d] = c->nc[d];
+    g.gn[d] = c->gn[d];
+    g.gofs[d] = c->gofs[d];
+    g.pbc[d] = c->pbc[d];
+    g.L[d] = c->L[d];
+    g.one_cell[d] = 1.0 / c->cell[d];
+  }
+  g.ncell = c->ncell;
+  return g;
+}
+
+MassArgs make_mass(const gh_ctx *c) {
+  MassArgs m;
+  for (int t = 0; t < GH_MAXT; ++t) {
+    m.om[t] = 1.0 / c->mass_t[t];
+    m.osm[t] = sqrt(m.om[t]);
+  }
+  return m;
+}
+
+WrapArgs make_wrap(const gh_ctx *c) {
+  WrapArgs w;
+  for (int d = 0; d < 3; ++d) {
+    w.pbc[d] = c->boxed ? c->pbc[d] : 0;
+    w.L[d] = c->L[d];
+  }
+  return w;
+}
+
+ErmakArgs make_ermak(const gh_ctx *c) {
+  ErmakArgs k;
+  const double dt = c->dt, gama = c->gamma;
+  k.c0 = exp(-dt * gama);
+  k.c1 = (1.0 - k.c0) / gama;
+  k.c2 = (1.0 - k.c1 / dt) / gama;
+  k.sdr = sqrt(dt / gama * (2.0 - (3.0 - 4.0 * k.c0 + k.c0 * k.c0) / (dt * gama)));
+  k.sdv = sqrt(1.0 - k.c0 * k.c0);
+  k.crv1 = (1.0 - k.c0) * (1.0 - k.c0) / (gama * k.sdr * k.sdv);
+  k.crv2 = sqrt(1.0 - (k.crv1 * k.crv1));
+  k.skt = sqrt(ghc::kB_ui() * c->T);
+  k.dt = dt;
+  k.philox = c->rng_mode == GH_RNG_PHILOX;
+  k.seed = c->philox_seed;
+  k.step = c->step;
+  return k;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+// Neighbour rebuild.  force: request a rebuild regardless of displacement.
+// Every kernel of the rebuild reads the device flag set by k_decide, so the
+// sequence can be enqueued every step without a host round trip.
+int gh_launch_rebuild(gh_ctx *c, bool force) {
+  hipStream_t s = c->stream;
+  if (force) {
+    GhStatus h;
+    GH_HIP(c, hipMemcpyAsync(&h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipStreamSynchronize(s));
+    h.force_rebuild = 1;
+    GH_HIP(c, hipMemcpyAsync(c->status, &h, sizeof(GhStatus), hipMemcpyHostToDevice, s));
+    GH_HIP(c, hipStreamSynchronize(s));
+  }
+  k_decide<<<1, 1, 0, s>>>(c->status, c->skin);
+  const int n = c->n, nb = nblocks(n);
+  const GridArgs g = make_grid(c);
+  const double rl = c->rcut + c->skin;
+  const double rl2 = rl * rl;
+  const int ld = c->ncap;
+  if (c->list_mode == 1) {
+    k_zero_cells<<<nblocks(c->ncell), GH_BLOCK, 0, s>>>(c->status, c->cell_count, c->ncell);
+    k_bin<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, c->cellid, c->cell_count);
+    k_scan<<<1, 1024, 0, s>>>(c->status, c->cell_count, c->cell_start, c->cell_cursor,
+                              c->ncell);
+    k_scatter<<<nb, GH_BLOCK, 0, s>>>(c->status, c->cellid, n, c->cell_cursor, c->perm);
+    k_sort_cells<<<nblocks(c->ncell), GH_BLOCK, 0, s>>>(c->status, c->cell_start, c->orig,
+                                                        c->perm, c->ncell);
+    k_gather<<<nb, GH_BLOCK, 0, s>>>(c->status, c->perm, n, c->ncap, c->cellid, c->pos,
+                                     c->vel, c->acc, c->type, c->orig, c->pos_t, c->vel_t,
+                                     c->acc_t, c->type_t, c->orig_t, c->cellof);
+    k_copyback<<<nb, GH_BLOCK, 0, s>>>(c->status, n, c->ncap, c->pos_t, c->vel_t, c->acc_t,
+                                       c->type_t, c->orig_t, c->pos, c->vel, c->acc, c->type,
+                                       c->orig);
+    k_build_cells<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, c->cellof, c->cell_start, n, g,
+                                          rl2, c->cap, ld, 0, c->nn, (unsigned *)c->list,
+                                          c->xold, c->ncap);
+  } else {
+    k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
+                                       (unsigned *)c->list, c->xold, c->ncap);
+  }
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// Count-only pass used once to size the list rows (host-synchronous).
+int gh_count_neighbours(gh_ctx *c, int *maxnn) {
+  hipStream_t s = c->stream;
+  const int n = c->n, nb = nblocks(n);
+  const GridArgs g = make_grid(c);
+  const double rl = c->rcut + c->skin;
+  const double rl2 = rl * rl;
+  if (c->list_mode == 1)
+    k_build_cells<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, c->cellof, c->cell_start, n, g,
+                                          rl2, 0, c->ncap, 1, c->nn, (unsigned *)c->list,
+                                          c->xold, c->ncap);
+  else
+    k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, 0, c->ncap, 1, c->nn,
+                                       (unsigned *)c->list, c->xold, c->ncap);
+  GH_HIP(c, hipGetLastError());
+  GhStatus h;
+  GH_HIP(c, hipMemcpyAsync(&h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, s));
+  GH_HIP(c, hipStreamSynchronize(s));
+  *maxnn = h.maxnn;
+  return 0;
+}
+
+int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
+  hipStream_t s = c->stream;
+  const int n = c->n, nb = nblocks(n);
+  const int ld = c->ncap;
+  const double ev_ui = ghc::ev_ui();
+  const double L0 = c->L[0], L1 = c->L[1], L2 = c->L[2];
+  const unsigned *list = (const unsigned *)c->list;
+  const bool single = c->nt == 1;
+  const bool mixed = c->precision == GH_MIXED;
+  if (c->pot == GH_POT_SMATB) {
+    if (mixed) {
+      if (single)
+        k_tb_rho_mx<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+                                                  c->nt, L0, L1, L2, c->rho, c->eband,
+                                                  c->eband32);
+      else
+        k_tb_rho_mx<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+                                                   c->nt, L0, L1, L2, c->rho, c->eband,
+                                                   c->eband32);
+    } else {
+      if (single)
+        k_tb_rho64<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+                                                 c->nt, L0, L1, L2, c->rho, c->eband,
+                                                 c->eband32);
+      else
+        k_tb_rho64<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+                                                  c->nt, L0, L1, L2, c->rho, c->eband,
+                                                  c->eband32);
+    }
+    if (mid) GH_HIP(c, hipEventRecord(mid, s));
+    if (mixed) {
+      if (single)
+        k_tb_force_mx<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+                                                    c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
+                                                    c->eband32, ev_ui, c->frc, c->epa);
+      else
+        k_tb_force_mx<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+                                                     c->ncap, c->tbd, c->nt, L0, L1, L2,
+                                                     c->rho, c->eband32, ev_ui, c->frc,
+                                                     c->epa);
+    } else {
+      if (single)
+        k_tb_force64<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+                                                   c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
+                                                   c->eband, ev_ui, c->frc, c->epa);
+      else
+        k_tb_force64<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+                                                    c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
+                                                    c->eband, ev_ui, c->frc, c->epa);
+    }
+  } else if (c->pot == GH_POT_LJ) {
+    if (mid) GH_HIP(c, hipEventRecord(mid, s));
+    const double rc2 = c->lj_rc * c->lj_rc;
+    if (mixed)
+      k_lj_mx<<<nb, GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, (float)c->lj_eps,
+                                      (float)(c->lj_sigma * c->lj_sigma), rc2, L0, L1, L2,
+                                      ev_ui, c->frc, c->epa);
+    else
+      k_lj64<<<nb, GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
+                                     c->lj_sigma, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
+  } else {
+    return gh_fail(c, GH_ESTATE, "no potential set");
+  }
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int gh_launch_vv_a(gh_ctx *c) {
+  k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c),
+                                                    make_wrap(c), c->type, c->pos, c->vel,
+                                                    c->acc, c->frc, c->xold, c->status);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int gh_launch_vv_b(gh_ctx *c) {
+  k_vv_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c), c->type,
+                                                    c->vel, c->acc, c->frc);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int gh_launch_ermak_a(gh_ctx *c) {
+  k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
+      c->n, c->ncap, make_ermak(c), make_mass(c), make_wrap(c), c->type, c->orig, c->gauss,
+      c->pos, c->vel, c->acc, c->xold, c->status);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int gh_launch_ermak_b(gh_ctx *c) {
+  k_ermak_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, make_ermak(c),
+                                                       make_mass(c), c->type, c->orig,
+                                                       c->gauss, c->vel, c->acc, c->frc);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// epot and ekin (ui) of the current state; host-synchronous.
+int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui) {
+  hipStream_t s = c->stream;
+  const int nblk = std::min(c->nred, std::max(1, nblocks(c->n)));
+  const MassArgs ma = make_mass(c);
+  double *part = c->red;                 // [nblk*5]
+  double *sums = c->red + 5 * c->nred;   // [5]
+  double *kin = sums + 8;                // [1]
+  k_red_a<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->epa, c->vel, part);
+  k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 5, part, sums);
+  k_red_b<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->vel, sums, part);
+  k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 1, part, kin);
+  GH_HIP(c, hipGetLastError());
+  GH_HIP(c, hipMemcpyAsync(c->red_h, sums, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
+  GH_HIP(c, hipStreamSynchronize(s));
+  *epot_ui = c->red_h[0];
+  *ekin_ui = c->red_h[8] * 0.5;
+  return 0;
+}
+
+int gh_launch_wrap(gh_ctx *c) {
+  k_wrap<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, make_wrap(c), c->pos);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// Fill helper for host code (no hipMemset in code that may be captured).
+int gh_fill_int(gh_ctx *c, int *p, int v, int n) {
+  if (n <= 0) return 0;
+  k_fill_int<<<nblocks(n), GH_BLOCK, 0, c->stream>>>(p, v, n);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}

@@ -1,0 +1,218 @@
+"""ctypes binding of the gfx950 library (include/gems_hip.h).
+
+The library is loaded from gems_amd/lib/libgems_hip.so, built in-tree by
+``python gems_amd/build.py`` (or ``__graft_entry__.build()``).  There is no
+fallback: if the library is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgems_hip.so")
+
+GH_FP64, GH_MIXED = 0, 1
+GH_RNG_LCG, GH_RNG_PHILOX = 0, 1
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
+
+# every symbol declared in include/gems_hip.h
+SYMBOLS = [
+    "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box",
+    "gh_set_atoms", "gh_set_masses", "gh_set_velocities", "gh_set_smatb", "gh_set_lj",
+    "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
+    "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
+    "gh_get_stats", "gh_set_timing", "gh_get_timing",
+]
+
+
+class GhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [("nupd", C.c_int64), ("maxnn", C.c_int32), ("cap", C.c_int32),
+                ("ncells", C.c_int32 * 3), ("list_mode", C.c_int32), ("steps", C.c_int64)]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def load():
+    """Loads the in-tree library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("gems_amd: %s is missing; build it with "
+                          "`python gems_amd/build.py`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.gh_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+    L.gh_destroy.argtypes = [vp]
+    L.gh_destroy.restype = None
+    L.gh_last_error.argtypes = [vp]
+    L.gh_last_error.restype = C.c_char_p
+    L.gh_set_precision.argtypes = [vp, C.c_int]
+    L.gh_set_box.argtypes = [vp, _dp, _ip]
+    L.gh_set_atoms.argtypes = [vp, C.c_int, _dp, _ip]
+    L.gh_set_masses.argtypes = [vp, C.c_int, _dp]
+    L.gh_set_velocities.argtypes = [vp, _dp]
+    L.gh_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
+    L.gh_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+    L.gh_set_skin.argtypes = [vp, C.c_double]
+    L.gh_compute.argtypes = [vp, _dp, _dp, _dp]
+    L.gh_neighbors.argtypes = [vp, _ip, _ip, C.c_int]
+    L.gh_prng_lcg.argtypes = [vp, C.c_int]
+    L.gh_set_gdist.argtypes = [vp, C.c_double]
+    L.gh_integrator_vv.argtypes = [vp, C.c_double]
+    L.gh_integrator_ermak.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_uint64]
+    L.gh_run.argtypes = [vp, C.c_int, C.c_int, _dp]
+    L.gh_energies.argtypes = [vp, _dp, _dp]
+    L.gh_get_state.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.gh_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.gh_set_timing.argtypes = [vp, C.c_int]
+    L.gh_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_int64)]
+    _lib = L
+    return L
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """One system on one device (a gh_ctx)."""
+
+    def __init__(self, device=0, precision=GH_FP64):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.gh_create(C.byref(h), device, precision)
+        if rc != 0:
+            raise GhError(rc, "gh_create failed (no HIP device %d?)" % device)
+        self.h = h
+        self.n = 0
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise GhError(rc, self.L.gh_last_error(self.h).decode())
+
+    def close(self):
+        if self.h is not None:
+            self.L.gh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- setup ---
+    def set_precision(self, p):
+        self._ck(self.L.gh_set_precision(self.h, p))
+
+    def set_box(self, L=None, pbc=(1, 1, 1)):
+        if L is None:
+            self._ck(self.L.gh_set_box(self.h, None, None))
+        else:
+            Ls = np.ascontiguousarray(L, np.float64)
+            pb = np.ascontiguousarray(pbc, np.int32)
+            self._ck(self.L.gh_set_box(self.h, _p(Ls), _p(pb, _ip)))
+
+    def set_atoms(self, x, types=None):
+        x = np.ascontiguousarray(x, np.float64)
+        self.n = x.shape[0]
+        t = None if types is None else np.ascontiguousarray(types, np.int32)
+        self._ck(self.L.gh_set_atoms(self.h, self.n, _p(x), _p(t, _ip)))
+
+    def set_masses(self, masses):
+        m = np.ascontiguousarray(np.atleast_1d(masses), np.float64)
+        self._ck(self.L.gh_set_masses(self.h, len(m), _p(m)))
+
+    def set_velocities(self, v=None):
+        vv = None if v is None else np.ascontiguousarray(v, np.float64)
+        self._ck(self.L.gh_set_velocities(self.h, _p(vv)))
+
+    def set_smatb(self, nt, A, xi, p, q, r0, rci, rce):
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (nt, nt)))
+                for a in (A, xi, p, q, r0, rci, rce)]
+        self._ck(self.L.gh_set_smatb(self.h, nt, *[_p(a) for a in arrs]))
+
+    def set_lj(self, eps, sigma, rc):
+        self._ck(self.L.gh_set_lj(self.h, eps, sigma, rc))
+
+    def set_skin(self, skin):
+        self._ck(self.L.gh_set_skin(self.h, skin))
+
+    # --- evaluation ---
+    def compute(self):
+        F = np.zeros((self.n, 3))
+        e = np.zeros(self.n)
+        et = C.c_double(0.0)
+        self._ck(self.L.gh_compute(self.h, _p(F), _p(e), C.byref(et)))
+        return F, e, et.value
+
+    def neighbors(self, cap=256):
+        nn = np.zeros(self.n, np.int32)
+        idx = np.zeros((self.n, cap), np.int32)
+        self._ck(self.L.gh_neighbors(self.h, _p(nn, _ip), _p(idx, _ip), cap))
+        return nn, idx
+
+    def prng_lcg(self, seed):
+        self._ck(self.L.gh_prng_lcg(self.h, seed))
+
+    def set_gdist(self, T):
+        self._ck(self.L.gh_set_gdist(self.h, T))
+
+    def integrator_vv(self, dt):
+        self._ck(self.L.gh_integrator_vv(self.h, dt))
+
+    def integrator_ermak(self, dt, T, gamma, rng=GH_RNG_LCG, seed=0):
+        self._ck(self.L.gh_integrator_ermak(self.h, dt, T, gamma, rng, seed))
+
+    def run(self, nsteps, each=0):
+        nout = nsteps // each if each > 0 else 0
+        en = np.zeros((max(nout, 1), 3))
+        self._ck(self.L.gh_run(self.h, nsteps, each, _p(en) if each > 0 else None))
+        return en[:nout]
+
+    def energies(self):
+        ep, ek = C.c_double(), C.c_double()
+        self._ck(self.L.gh_energies(self.h, C.byref(ep), C.byref(ek)))
+        return ep.value, ek.value
+
+    def state(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        F = np.zeros((self.n, 3))
+        e = np.zeros(self.n)
+        self._ck(self.L.gh_get_state(self.h, _p(x), _p(v), _p(F), _p(e)))
+        return x, v, F, e
+
+    def stats(self):
+        s = Stats()
+        self._ck(self.L.gh_get_stats(self.h, C.byref(s)))
+        return dict(nupd=s.nupd, maxnn=s.maxnn, cap=s.cap, ncells=tuple(s.ncells),
+                    list_mode=s.list_mode, steps=s.steps)
+
+    def set_timing(self, on=True):
+        self._ck(self.L.gh_set_timing(self.h, int(on)))
+
+    def timing(self):
+        ms = np.zeros(4)
+        cnt = (C.c_int64 * 4)()
+        self._ck(self.L.gh_get_timing(self.h, _p(ms), cnt))
+        names = ["rho", "force", "integrate", "rebuild"]
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}

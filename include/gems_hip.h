@@ -1,0 +1,160 @@
+/*
+ * gems_hip.h -- C ABI of the MI355X (gfx950) force-evaluation library for the
+ * GEMS molecular simulator.
+ *
+ * The library replaces GEMS's hot path: neighbour build (src/Neighbour.F90),
+ * the Lennard-Jones pair force (src/Pairs.F90), the SMA-TB/Gupta two-pass
+ * force (src/TB.F90) and the velocity-Verlet / Ermak updates
+ * (src/Integration.F90).  Every entry point names the reference routine it
+ * stands in for.
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions cross the boundary.
+ *  - Return 0 on success, a negative GH_E* code on failure; gh_last_error()
+ *    then holds a message.  (GEMS's own errors STOP via werr, Errors.f90:43-59;
+ *    a Fortran caller maps a non-zero return to werr(gh_last_error(ctx)).)
+ *  - Units are GEMS internal units: Angstrom, amu, ps, energy "ui"
+ *    (amu*A^2/ps^2).  Per-atom arrays are in the caller's atom order with the
+ *    reference's per-atom layout x[3*i+d].  The library copies in and out and
+ *    never keeps host pointers.
+ *  - One context = one system on one device, driven by one host thread, with
+ *    one HIP stream of its own (default priority).
+ */
+#ifndef GEMS_HIP_H
+#define GEMS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gh_ctx gh_ctx;
+
+/* error codes */
+#define GH_OK 0
+#define GH_EINVAL -1    /* bad argument                                   */
+#define GH_EHIP -2      /* HIP runtime error                              */
+#define GH_ESTATE -3    /* call out of order (e.g. no atoms / potential)  */
+#define GH_EOVERFLOW -4 /* neighbour list capacity exceeded               */
+#define GH_ERANGE -5    /* an atom left the cell grid of a non-periodic box */
+
+/* precision of the pair arithmetic */
+#define GH_FP64 0   /* plain FP64 everywhere                                */
+#define GH_MIXED 1  /* FP32 pair terms; FP64 positions, sums, integrator    */
+
+/* potentials */
+#define GH_POT_NONE 0
+#define GH_POT_SMATB 1
+#define GH_POT_LJ 2
+
+/* integrators */
+#define GH_INT_VV 0     /* velocity Verlet   (Integration.F90:1150-1186) */
+#define GH_INT_ERMAK 1  /* Ermak / Langevin  (Integration.F90:909-1031)  */
+
+/* random streams for Ermak */
+#define GH_RNG_LCG 0    /* the reference "prng lcg" stream, drawn on the host
+                           in atom order (Random.F90:145-250)              */
+#define GH_RNG_PHILOX 1 /* counter-based Philox4x32-10 on the device       */
+
+/* Context lifetime.  No reference counterpart: GEMS keeps global state
+ * (src/Program_Types.F90:37-102); a context holds that state for one system.
+ * device: HIP ordinal.  precision: GH_FP64 or GH_MIXED. */
+int gh_create(gh_ctx **ctx, int device, int precision);
+void gh_destroy(gh_ctx *ctx);
+const char *gh_last_error(const gh_ctx *ctx);
+int gh_set_precision(gh_ctx *ctx, int precision);
+
+/* Box.  Replaces `box size` + `set pbc` (box_setvars, Program_Types.F90:126-141;
+ * set_pbc; do_pbc_1, Set_Properties.f90:444-470).  L == NULL: no box (free
+ * system; the reference then uses the O(N^2) verlet_self list). */
+int gh_set_box(gh_ctx *ctx, const double *L, const int *pbc);
+
+/* Atoms.  Replaces the atom creation commands and the intergroup atom table
+ * (intergroup%at, Neighbour.F90:114; adda, :359-392).  x: 3n positions (A);
+ * type: n internal types 0..nt-1 (TB.F90:194-215 numbering). */
+int gh_set_atoms(gh_ctx *ctx, int n, const double *x, const int *type);
+/* Per-type masses in amu (atom%mass, Atoms.F90:44; set mass). */
+int gh_set_masses(gh_ctx *ctx, int nt, const double *mass);
+/* Velocities, 3n (A/ps); NULL sets them to zero. */
+int gh_set_velocities(gh_ctx *ctx, const double *v);
+
+/* SMA-TB parameters for every type pair, row-major nt*nt arrays (eV, A).
+ * Replaces smatb_set / smatb_readprm (TB.F90:106-170, 291-365); the quintic
+ * tail coefficients are derived the same way.  Sets rcut = max(rce)
+ * (TB.F90:283). */
+int gh_set_smatb(gh_ctx *ctx, int nt, const double *A, const double *xi,
+                 const double *p, const double *q, const double *r0,
+                 const double *rci, const double *rce);
+
+/* Lennard-Jones 12-6, unshifted: `interact ... under lj e s rc`
+ * (lj_set, Pairs.F90:198-205).  eps in eV, sigma and rc in A. */
+int gh_set_lj(gh_ctx *ctx, double eps, double sigma, double rc);
+
+/* Verlet skin `nb_dcut` (Neighbour.F90:237), A. */
+int gh_set_skin(gh_ctx *ctx, double skin);
+
+/* One force evaluation == interact() (Interaction.f90:135-241): skin test
+ * and neighbour rebuild (test_update/update, Neighbour.F90:1242-1346), then
+ * smatb_interact (TB.F90:457-586) or lj_interact (Pairs.F90:207-279).
+ * F: 3n forces (ui/A), e_atom: n per-atom energies (ui), e_tot: sum (ui).
+ * Any output pointer may be NULL. */
+int gh_compute(gh_ctx *ctx, double *F, double *e_atom, double *e_tot);
+
+/* The current full neighbour list (ig%nn, ig%list; Neighbour.F90:120-121)
+ * at radius rcut+skin, in the caller's atom indices, each row sorted
+ * ascending.  nn: n counts; idx: n*cap entries, row i at idx[i*cap].
+ * Returns GH_EOVERFLOW if a row is longer than cap. */
+int gh_neighbors(gh_ctx *ctx, int *nn, int *idx, int cap);
+
+/* Random stream: `prng lcg` + `prng seed s` (Random.F90:137-172). */
+int gh_prng_lcg(gh_ctx *ctx, int seed);
+/* `set tempgdist T` on all atoms (set_gdist + set_scal_vel,
+ * Set_Properties.f90:301-377), drawing from the lcg stream. */
+int gh_set_gdist(gh_ctx *ctx, double T);
+
+/* Integrators: `evolve v_verlet` and `evolve ermak T gamma`
+ * (integrate_cli, Integration.F90:118-230; set_ermak :909-961).
+ * dt in ps.  rng: GH_RNG_LCG (reproduces GEMS's stream) or GH_RNG_PHILOX
+ * (device counter-based; philox_seed keys it). */
+int gh_integrator_vv(gh_ctx *ctx, double dt);
+int gh_integrator_ermak(gh_ctx *ctx, double dt, double T, double gamma,
+                        int rng, uint64_t philox_seed);
+
+/* `dinamica nsteps` (dinamic, Programs.f90:114-167).  Every `each` steps
+ * (each > 0) one row [epot, ekin, epot+ekin] in eV is written to energy
+ * (write_energy, Output.F90:824-831; ekin about the centre of mass,
+ * Inq_Properties.F90:399-422).  energy may be NULL when each <= 0. */
+int gh_run(gh_ctx *ctx, int nsteps, int each, double *energy);
+
+/* Energies of the current state in eV (write_energy). */
+int gh_energies(gh_ctx *ctx, double *epot, double *ekin);
+
+/* Copy the state back: 3n positions, 3n velocities, 3n forces (ui/A), n
+ * per-atom energies (ui).  Any pointer may be NULL. */
+int gh_get_state(gh_ctx *ctx, double *x, double *v, double *F, double *e);
+
+/* Counters: nupd = neighbour rebuilds (nupd_vlist, Neighbour.F90:238),
+ * maxnn = longest list row at the last build, cap = row capacity. */
+typedef struct {
+  int64_t nupd;
+  int32_t maxnn;
+  int32_t cap;
+  int32_t ncells[3];
+  int32_t list_mode; /* 0 = O(N^2) (verlet_self), 1 = cells (linkedcell_self) */
+  int64_t steps;
+} gh_stats;
+int gh_get_stats(gh_ctx *ctx, gh_stats *s);
+
+/* Kernel timing with HIP events on the context's stream.  While enabled,
+ * the library records events around the force kernels; gh_get_timing
+ * returns accumulated milliseconds and launch counts for: [0] density pass
+ * (SMA-TB pass 1), [1] force pass (SMA-TB pass 2 or LJ), [2] integrator,
+ * [3] neighbour rebuild. */
+int gh_set_timing(gh_ctx *ctx, int enable);
+int gh_get_timing(gh_ctx *ctx, double ms[4], int64_t count[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+"""Shared helpers of the test-suite: golden fixtures and parameter tables."""
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+# src/Constants.F90 (evaluated in real(dp)); eV_kcm is a default-real literal
+UI_EV = 1.0e2 * 1.0e2 * 1.6605402e-27 * (1.0 / 1.60219e-19)
+EV_UI = 1.0 / UI_EV
+KB_EV = 8.617385e-05
+KB_UI = KB_EV * EV_UI
+MASS_AU = float(np.float32(196.97))
+MASS_AG = float(np.float32(107.87))
+MASS_CU = float(np.float32(63.55))
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+
+
+def meta():
+    with open(os.path.join(GOLDEN, "meta.json")) as f:
+        return json.load(f)
+
+
+def au_params():
+    m = meta()["au_prm"]
+    return [(0, 0, m["A"], m["xi"], m["p"], m["q"], m["r0"], m["rci"], m["rce"])]
+
+
+def agcu_params(first_symbol):
+    """Rows with internal types numbered by first appearance (TB.F90:194-215)."""
+    m = meta()
+    tid = {"Ag": 0, "Cu": 1} if first_symbol == "Ag" else {"Cu": 0, "Ag": 1}
+    rows = []
+    for r in m["agcu_prm"]:
+        rows.append((tid[r["e1"]], tid[r["e2"]], r["A"], r["xi"], r["p"], r["q"], r["r0"],
+                     r["rci"], r["rce"]))
+    return rows
+
+
+def agcu_masses(first_symbol):
+    return [MASS_AG, MASS_CU] if first_symbol == "Ag" else [MASS_CU, MASS_AG]
+
+
+def matrices(rows, nt):
+    """(i,j,A,xi,p,q,r0,rci,rce) rows -> symmetric nt x nt arrays."""
+    out = [np.zeros((nt, nt)) for _ in range(7)]
+    for r in rows:
+        i, j = r[0], r[1]
+        for k in range(7):
+            out[k][i, j] = r[2 + k]
+            out[k][j, i] = r[2 + k]
+    return out
+
+
+def fcc(nx, a, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0, 0.5, 0.5], [0.5, 0, 0.5]])
+    g = np.array([[i, j, k] for k in range(nz) for j in range(ny) for i in range(nx)], float)
+    return ((g[:, None, :] + base[None, :, :]).reshape(-1, 3)) * a
+
+
+def force_err(F, Fref):
+    """max |dF| / rms |F| (per component)."""
+    rms = np.sqrt(np.mean(Fref ** 2))
+    return np.max(np.abs(F - Fref)) / rms
