@@ -170,7 +170,7 @@ def main():
             "precision": "mixed",
         },
         "roofline": {
-            "kernel": "k_tb_force_mx (SMA-TB pass 2, mixed)",
+            "kernel": "k_tb_force_tiled<mixed> (SMA-TB pass 2)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -180,7 +180,7 @@ def main():
             "bytes_per_atom": BYTES_PASS2,
             "avg_launch_us": t_force * 1e6,
         },
-        "pass1": {"kernel": "k_tb_rho_mx", "avg_launch_us": t_rho * 1e6,
+        "pass1": {"kernel": "k_tb_rho_tiled<mixed>", "avg_launch_us": t_rho * 1e6,
                   "achieved_GBs": BYTES_PASS1 * n / t_rho / 1e9},
         "fp64": {"value": value64, "steps": k64, "ms_per_step": dt_64 / k64 * 1e3,
                  "force_pass_us": tm_64["force"][0] / tm_64["force"][1] * 1e3,

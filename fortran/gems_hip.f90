@@ -1,0 +1,184 @@
+! gems_hip.f90 -- ISO_C_BINDING interface to the MI355X force-evaluation
+! library (include/gems_hip.h).  A GEMS build uses it from Fortran exactly as
+! the C header describes: plain values and contiguous real(c_double) /
+! integer(c_int) arrays in GEMS internal units (Angstrom, amu, ps, ui).
+!
+! Entry points and the GEMS routines they replace:
+!   gh_compute          interact()            src/Interaction.f90:135-241
+!   gh_set_smatb        smatb_set/readprm     src/TB.F90:106-170, 291-365
+!   gh_set_lj           lj_set                src/Pairs.F90:198-205
+!   gh_set_skin         nb_dcut               src/Neighbour.F90:237
+!   gh_set_box          box_setvars/set_pbc   src/Program_Types.F90:126-141
+!   gh_neighbors        ig%nn, ig%list        src/Neighbour.F90:120-121
+!   gh_integrator_vv    evolve v_verlet       src/Integration.F90:1150-1186
+!   gh_integrator_ermak evolve ermak T g      src/Integration.F90:909-1031
+!   gh_run              dinamica n            src/Programs.f90:114-167
+!   gh_prng_lcg         prng lcg / seed       src/Random.F90:137-172
+!   gh_set_gdist        set tempgdist T       src/Set_Properties.f90:301-377
+module gems_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: GH_FP64 = 0, GH_MIXED = 1
+  integer(c_int), parameter :: GH_RNG_LCG = 0, GH_RNG_PHILOX = 1
+
+  interface
+    integer(c_int) function gh_create(ctx, device, precision) bind(C, name='gh_create')
+      import :: c_ptr, c_int
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: device, precision
+    end function
+
+    subroutine gh_destroy(ctx) bind(C, name='gh_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+
+    type(c_ptr) function gh_last_error_c(ctx) bind(C, name='gh_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function gh_set_precision(ctx, precision) bind(C, name='gh_set_precision')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+    end function
+
+    integer(c_int) function gh_set_box(ctx, L, pbc) bind(C, name='gh_set_box')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: L(3)
+      integer(c_int), intent(in) :: pbc(3)
+    end function
+
+    integer(c_int) function gh_set_box_free(ctx, L, pbc) bind(C, name='gh_set_box')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, L, pbc     ! pass c_null_ptr: no box
+    end function
+
+    integer(c_int) function gh_set_atoms(ctx, n, x, type) bind(C, name='gh_set_atoms')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: x(3, n)
+      integer(c_int), intent(in) :: type(n)
+    end function
+
+    integer(c_int) function gh_set_masses(ctx, nt, mass) bind(C, name='gh_set_masses')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: mass(nt)
+    end function
+
+    integer(c_int) function gh_set_velocities(ctx, v) bind(C, name='gh_set_velocities')
+      import :: c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: v(*)
+    end function
+
+    integer(c_int) function gh_set_smatb(ctx, nt, A, xi, p, q, r0, rci, rce) &
+        bind(C, name='gh_set_smatb')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: A(nt, nt), xi(nt, nt), p(nt, nt), q(nt, nt), &
+                                    r0(nt, nt), rci(nt, nt), rce(nt, nt)
+    end function
+
+    integer(c_int) function gh_set_lj(ctx, eps, sigma, rc) bind(C, name='gh_set_lj')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), value :: eps, sigma, rc
+    end function
+
+    integer(c_int) function gh_set_skin(ctx, skin) bind(C, name='gh_set_skin')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), value :: skin
+    end function
+
+    integer(c_int) function gh_compute(ctx, F, e_atom, e_tot) bind(C, name='gh_compute')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: F(*), e_atom(*), e_tot
+    end function
+
+    integer(c_int) function gh_neighbors(ctx, nn, idx, cap) bind(C, name='gh_neighbors')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: nn(*), idx(*)
+      integer(c_int), value :: cap
+    end function
+
+    integer(c_int) function gh_prng_lcg(ctx, seed) bind(C, name='gh_prng_lcg')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: seed
+    end function
+
+    integer(c_int) function gh_set_gdist(ctx, T) bind(C, name='gh_set_gdist')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), value :: T
+    end function
+
+    integer(c_int) function gh_integrator_vv(ctx, dt) bind(C, name='gh_integrator_vv')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+    end function
+
+    integer(c_int) function gh_integrator_ermak(ctx, dt, T, gamma, rng, seed) &
+        bind(C, name='gh_integrator_ermak')
+      import :: c_ptr, c_double, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt, T, gamma
+      integer(c_int), value :: rng
+      integer(c_int64_t), value :: seed
+    end function
+
+    integer(c_int) function gh_run(ctx, nsteps, each, energy) bind(C, name='gh_run')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps, each
+      real(c_double), intent(out) :: energy(*)
+    end function
+
+    integer(c_int) function gh_energies(ctx, epot, ekin) bind(C, name='gh_energies')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: epot, ekin
+    end function
+
+    integer(c_int) function gh_get_state(ctx, x, v, F, e) bind(C, name='gh_get_state')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: x(*), v(*), F(*), e(*)
+    end function
+  end interface
+
+contains
+
+  ! gh_last_error as a Fortran string (for werr).
+  function gh_last_error(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: n
+    interface
+      integer(c_size_t) function c_strlen(str) bind(C, name='strlen')
+        import :: c_ptr, c_size_t
+        type(c_ptr), value :: str
+      end function
+    end interface
+    p = gh_last_error_c(ctx)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function
+
+end module gems_hip

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libgems_hip.so")
-SOURCES = ["gh_kernels.hip", "gh_api.hip"]
+SOURCES = ["gh_kernels.hip", "gh_tile.hip", "gh_batch.hip", "gh_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

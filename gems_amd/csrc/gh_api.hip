@@ -143,11 +143,15 @@ int gh_upload_tb(gh_ctx *c) {
       d.r3 = t.r3;
       d.p_r0 = t.p / t.r0;
       d.q2_r0 = 2.0 * t.q / t.r0;
-      // A e^{-p(r/r0-1)} = 2^{log2(A) + p log2e - (p/r0) log2e r}
-      d.rep_a = (float)(log2(t.A) + t.p * log2e);
-      d.rep_b = (float)(-t.p / t.r0 * log2e);
-      d.rho_a = (float)(log2(t.xi * t.xi) + 2.0 * t.q * log2e);
-      d.rho_b = (float)(-2.0 * t.q / t.r0 * log2e);
+      // A e^{-p(r/r0-1)} = A 2^{-(p/r0) log2e (r - r0)}
+      //                 = [A 2^{b (r0f - r0)}] 2^{b (r - r0f)},  b = -(p/r0) log2e
+      d.r0f = (float)t.r0;
+      const double r0f = (double)d.r0f;
+      const double bp = -t.p / t.r0 * log2e, bq = -2.0 * t.q / t.r0 * log2e;
+      d.rep_b = (float)bp;
+      d.rep_a = (float)(t.A * exp2((double)d.rep_b * (r0f - t.r0)));
+      d.rho_b = (float)bq;
+      d.rho_a = (float)(t.xi * t.xi * exp2((double)d.rho_b * (r0f - t.r0)));
       d.frep = (float)(2.0 * t.p / t.r0);
       d.fband = (float)(t.q / t.r0);  // xi^2/2 * 2q/r0 = xi^2 q/r0 (xi^2 inside band)
       d.rci_f = (float)t.rci;
@@ -173,6 +177,62 @@ int gh_upload_tb(gh_ctx *c) {
 // visit distinct cells; otherwise the O(N^2) list is used.  Non-periodic axes
 // get one guard cell on each side so atoms that drift out of the box within
 // a cell length are still binned exactly.
+static const int GH_LDS_BYTES = 65536;
+
+// LDS bytes a workgroup of the tiled force pass needs per halo slot, and the
+// static LDS it uses besides (halo tables, parameter table).
+static int slot_bytes(const gh_ctx *c) { return 24 + 8 + (c->nt > 1 ? 4 : 0); }
+static int static_lds(const gh_ctx *c) { return 8192 + (c->nt > 1 ? 12800 : 256); }
+static int max_slots(const gh_ctx *c) {
+  return (GH_LDS_BYTES - static_lds(c)) / slot_bytes(c);
+}
+
+// Brick edges (cells) for the tiled passes: the largest brick whose expected
+// halo fits the LDS slot budget with a 30% margin; ties go to the smaller
+// halo.  bd_req (gh_set_tile) overrides.
+static void choose_bricks(gh_ctx *c, int limit) {
+  TileGeom &t = c->tg;
+  double cells = 1.0;
+  for (int d = 0; d < 3; ++d) cells *= t.nc[d];
+  const double apc = c->n / cells;  // atoms per interior cell
+  int best[3] = {1, 1, 1};
+  long bestc = 0, besth = 1L << 40;
+  for (int bz = 1; bz <= 4; ++bz)
+    for (int by = 1; by <= 4; ++by)
+      for (int bx = 1; bx <= 4; ++bx) {
+        const int b[3] = {bx, by, bz};
+        bool ok = true;
+        for (int d = 0; d < 3; ++d)
+          if (b[d] > t.gn[d]) ok = false;
+        if (!ok) continue;
+        const long halo = (long)(bx + 2) * (by + 2) * (bz + 2);
+        if (apc * halo * 1.3 > limit && !(bx == 1 && by == 1 && bz == 1)) continue;
+        const long nc = (long)bx * by * bz;
+        if (nc > bestc || (nc == bestc && halo < besth)) {
+          bestc = nc;
+          besth = halo;
+          best[0] = bx;
+          best[1] = by;
+          best[2] = bz;
+        }
+      }
+  for (int d = 0; d < 3; ++d) {
+    t.bd[d] = c->bd_req[d] > 0 ? std::min(c->bd_req[d], std::min(4, t.gn[d])) : best[d];
+  }
+}
+
+static void finish_tiles(gh_ctx *c) {
+  TileGeom &t = c->tg;
+  t.cpb = t.bd[0] * t.bd[1] * t.bd[2];
+  t.nbrick = 1;
+  for (int d = 0; d < 3; ++d) {
+    t.nbr[d] = (t.gn[d] + t.bd[d] - 1) / t.bd[d];
+    t.nbrick *= t.nbr[d];
+  }
+  t.nkey = t.nbrick * t.cpb;
+  c->ncell = t.nkey;
+}
+
 static void setup_grid(gh_ctx *c) {
   const double rl = c->rcut + c->skin;
   c->list_mode = 0;
@@ -194,14 +254,31 @@ static void setup_grid(gh_ctx *c) {
   }
   if (all_small || pbc_small) return;
   c->list_mode = 1;
-  c->ncell = 1;
+  TileGeom &t = c->tg;
   for (int d = 0; d < 3; ++d) {
     c->nc[d] = nc[d];
     c->cell[d] = c->L[d] / nc[d];
     c->gofs[d] = c->pbc[d] ? 0 : 1;
     c->gn[d] = nc[d] + 2 * c->gofs[d];
-    c->ncell *= c->gn[d];
+    t.nc[d] = c->nc[d];
+    t.gn[d] = c->gn[d];
+    t.gofs[d] = c->gofs[d];
+    t.pbc[d] = c->pbc[d];
+    t.L[d] = c->L[d];
+    t.one_cell[d] = 1.0 / c->cell[d];
   }
+  choose_bricks(c, max_slots(c));
+  finish_tiles(c);
+}
+
+static int alloc_cells(gh_ctx *c) {
+  if (c->ncell > c->cell_alloc) {
+    GH_TRY(dalloc(c, &c->cell_count, (size_t)c->ncell));
+    GH_TRY(dalloc(c, &c->cell_start, (size_t)c->ncell + 1));
+    GH_TRY(dalloc(c, &c->cell_cursor, (size_t)c->ncell));
+    c->cell_alloc = c->ncell;
+  }
+  return 0;
 }
 
 static int ensure_ready(gh_ctx *c) {
@@ -210,33 +287,75 @@ static int ensure_ready(gh_ctx *c) {
   if (c->ready) return 0;
   setup_grid(c);
   if (c->boxed) GH_TRY(gh_launch_wrap(c));
-  if (c->list_mode == 1 && c->ncell > c->cell_alloc) {
-    GH_TRY(dalloc(c, &c->cell_count, (size_t)c->ncell));
-    GH_TRY(dalloc(c, &c->cell_start, (size_t)c->ncell + 1));
-    GH_TRY(dalloc(c, &c->cell_cursor, (size_t)c->ncell));
-    c->cell_alloc = c->ncell;
-  }
-  // sizing pass: a rebuild with zero row capacity counts the rows
   GhStatus h;
   GH_TRY(read_status(c, &h));
   const long long nupd0 = h.nupd;
-  c->cap = 0;
-  GH_TRY(gh_launch_rebuild(c, true));
-  GH_TRY(read_status(c, &h));
-  int maxnn = h.maxnn;
+  int maxnn = 0;
+  if (c->list_mode == 1) {
+    // sizing: bin, count rows, measure the largest halo; shrink the bricks
+    // until the halo fits the LDS budget
+    for (int attempt = 0;; ++attempt) {
+      GH_TRY(alloc_cells(c));
+      c->tg.slot_cap = (GH_LDS_BYTES - 8192) / 24;  // count pass: positions only
+      c->cap = 0;
+      GH_TRY(gh_launch_rebuild(c, true));
+      int max_halo = 0, max_brick = 0;
+      GH_TRY(gh_tile_occupancy(c, &max_halo, &max_brick));
+      GH_TRY(read_status(c, &h));
+      int slot_cap = max_halo + std::max(32, max_halo / 8);
+      slot_cap = roundup(slot_cap, 32);
+      if (slot_cap <= max_slots(c) && !h.tile_overflow) {
+        c->tg.slot_cap = slot_cap;
+        maxnn = h.maxnn;
+        break;
+      }
+      int *bd = c->tg.bd;
+      int dmax = 0;
+      for (int d = 1; d < 3; ++d)
+        if (bd[d] > bd[dmax]) dmax = d;
+      if (bd[dmax] == 1 || attempt > 12)
+        return gh_fail(c, GH_EINVAL,
+                       "atom density too high for the LDS halo of one cell (halo of " +
+                           std::to_string(max_halo) + " atoms)");
+      bd[dmax] -= 1;
+      c->bd_req[0] = c->bd_req[1] = c->bd_req[2] = 0;
+      finish_tiles(c);
+      h.tile_overflow = 0;
+      GH_TRY(write_status(c, &h));
+    }
+  } else {
+    c->cap = 0;
+    GH_TRY(gh_launch_rebuild(c, true));
+    GH_TRY(read_status(c, &h));
+    maxnn = h.maxnn;
+  }
   int cap = maxnn + std::max(16, maxnn / 4);
   if (c->cap_req > cap) cap = c->cap_req;
   cap = roundup(std::max(cap, 8), 8);
   if ((size_t)cap * c->ncap > (size_t)1 << 31)
     return gh_fail(c, GH_EINVAL, "neighbour list too large");
-  GH_TRY(dalloc(c, &c->list, (size_t)cap * c->ncap));
+  if (c->list_mode == 1) {
+    if (c->list) {
+      GH_HIP(c, hipFree(c->list));
+      c->list = nullptr;
+    }
+    GH_TRY(dalloc(c, &c->list16, (size_t)cap * c->ncap));
+  } else {
+    if (c->list16) {
+      GH_HIP(c, hipFree(c->list16));
+      c->list16 = nullptr;
+    }
+    GH_TRY(dalloc(c, &c->list, (size_t)cap * c->ncap));
+  }
   c->cap = cap;
   h.overflow = 0;
+  h.tile_overflow = 0;
   h.nupd = nupd0;
   GH_TRY(write_status(c, &h));
   GH_TRY(gh_launch_rebuild(c, true));
   GH_TRY(read_status(c, &h));
   if (h.overflow) return gh_fail(c, GH_EOVERFLOW, "neighbour list overflow at sizing");
+  if (h.tile_overflow) return gh_fail(c, GH_EOVERFLOW, "LDS halo overflow at sizing");
   c->ready = true;
   c->pos_dirty = false;
   return 0;
@@ -252,6 +371,12 @@ static int check_status(gh_ctx *c) {
                    "neighbour list row capacity exceeded (" + std::to_string(h.maxnn) + " > " +
                        std::to_string(c->cap) + "); results of this call are invalid; the "
                        "next call re-sizes the list");
+  }
+  if (h.tile_overflow) {
+    c->ready = false;
+    return gh_fail(c, GH_EOVERFLOW,
+                   "a brick's halo outgrew the LDS slots sized at setup; results of this call "
+                   "are invalid; the next call re-sizes the tiles");
   }
   if (h.outside)
     return gh_fail(c, GH_ERANGE, "an atom left the cell grid of a non-periodic box axis");
@@ -346,7 +471,7 @@ void gh_destroy(gh_ctx *c) {
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->status,
-                 c->red};
+                 c->red, c->dispblk, c->list16};
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->gauss_h) (void)hipHostFree(c->gauss_h);
@@ -415,6 +540,9 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
     GH_TRY(dalloc(c, &c->cellid, ncap));
     GH_TRY(dalloc(c, &c->perm, ncap));
     GH_TRY(dalloc(c, &c->nn, ncap));
+    c->ndispblk = (ncap + 255) / 256;
+    GH_TRY(dalloc(c, &c->dispblk, (size_t)c->ndispblk));
+    GH_HIP(c, hipMemset(c->dispblk, 0, sizeof(double) * c->ndispblk));
     c->nred = std::min(1024, std::max(1, (ncap + 255) / 256));
     GH_TRY(dalloc(c, &c->red, (size_t)c->nred * 5 + 16));
     if (c->gauss) {
@@ -610,9 +738,22 @@ int gh_neighbors(gh_ctx *c, int *nn_out, int *idx, int cap) {
   GH_TRY(fetch_orig(c, orig));
   std::vector<int> nn(c->n);
   std::vector<unsigned> lst((size_t)c->cap * c->ncap);
+  if (c->list_mode == 1) {
+    int *tmp = nullptr;
+    GH_HIP(c, hipMalloc((void **)&tmp, sizeof(int) * (size_t)c->cap * c->ncap));
+    int rc2 = gh_tiled_to_global(c, tmp);
+    if (rc2 == 0) {
+      GH_HIP(c, hipMemcpyAsync(lst.data(), tmp, sizeof(int) * lst.size(), hipMemcpyDeviceToHost,
+                               c->stream));
+      GH_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    GH_HIP(c, hipFree(tmp));
+    if (rc2) return rc2;
+  } else {
+    GH_HIP(c, hipMemcpyAsync(lst.data(), c->list, sizeof(unsigned) * lst.size(),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
   GH_HIP(c, hipMemcpyAsync(nn.data(), c->nn, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
-  GH_HIP(c, hipMemcpyAsync(lst.data(), c->list, sizeof(unsigned) * lst.size(),
-                           hipMemcpyDeviceToHost, c->stream));
   GH_HIP(c, hipStreamSynchronize(c->stream));
   int rc = 0;
   std::vector<int> row;
@@ -797,9 +938,15 @@ int gh_get_state(gh_ctx *c, double *x, double *v, double *F, double *e) {
                              c->stream));
     GH_HIP(c, hipStreamSynchronize(c->stream));
     for (int s = 0; s < c->n; ++s) {
-      x[3 * (size_t)orig[s] + 0] = h[s].x;
-      x[3 * (size_t)orig[s] + 1] = h[s].y;
-      x[3 * (size_t)orig[s] + 2] = h[s].z;
+      double v[3] = {h[s].x, h[s].y, h[s].z};
+      // positions run continuously between rebuilds (at most half a skin
+      // outside the box); report them wrapped as do_pbc_1 does
+      for (int d = 0; d < 3; ++d)
+        if (c->boxed && c->pbc[d]) {
+          if (v[d] >= c->L[d]) v[d] -= c->L[d];
+          else if (v[d] < 0.0) v[d] += c->L[d];
+        }
+      for (int d = 0; d < 3; ++d) x[3 * (size_t)orig[s] + d] = v[d];
     }
   }
   if (v) GH_TRY(fetch_soa3(c, c->vel, orig, v));

@@ -1,0 +1,901 @@
+// Batched free clusters: many independent unboxed systems advanced in one
+// launch, one workgroup per system, largest system first.
+//
+// GEMS has no batching (one system per process, replica exchange over MPI
+// ranks, src/ReplicaExchange.f90:27-151); per system the work is the same hot
+// path: the O(N^2) verlet_self list (Neighbour.F90:676-759, rd <= rl^2 kept),
+// the two SMA-TB passes (TB.F90:367-586) and the Ermak update
+// (Integration.F90:963-1031).  A system's atoms are contiguous; all state is
+// FP64 in HBM (SoA with stride ld), neighbour lists are ELL rows of 16-bit
+// indices local to the system.  Within a workgroup the steps of a system are
+// separated by workgroup barriers only, so nsteps need one launch.
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "gh_device.h"
+
+#define GH_BBLOCK 256
+
+struct gh_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = GH_FP64;
+  std::string err;
+  int nsys = 0, ntot = 0, ld = 0;
+  int nt = 1;
+  double mass_t[GH_MAXT] = {1, 1, 1, 1, 1, 1, 1, 1};
+  GhTbPair tb[GH_MAXT * GH_MAXT];
+  bool have_pot = false;
+  double rcut = 0.0, skin = 1.0;
+  int cap = 0;
+  long long step = 0;
+  bool lists_valid = false;
+  bool forces_valid = false;
+  std::vector<int> off_h, order_h;
+  std::vector<double> T_h, epot_h;
+  // device
+  int *off = nullptr, *order = nullptr, *type = nullptr, *nn = nullptr, *flags = nullptr;
+  double4 *pos = nullptr;
+  double *vel = nullptr, *acc = nullptr, *frc = nullptr, *e = nullptr, *rho = nullptr,
+         *eband = nullptr, *xold = nullptr, *T = nullptr, *out = nullptr, *scale = nullptr;
+  float *eband32 = nullptr;
+  unsigned short *list = nullptr;
+  TbDev *tbd = nullptr;
+};
+
+namespace {
+using namespace ghd;
+
+struct BatchArgs {
+  const int *off, *order, *type;
+  double4 *pos;
+  double *vel, *acc, *frc, *e, *rho, *eband, *xold;
+  float *eband32;
+  int *nn;
+  unsigned short *list;
+  int cap, ld;
+  const TbDev *tbd;
+  int nt;
+  double om[GH_MAXT], osm[GH_MAXT];
+  double rl2;
+  double ev_ui;
+  const double *T;
+  double *out;  // per system: [0] epot ui, [1] ekin ui, [2] rebuilds, [3] fmax eV/A,
+                // [4] iterations, [5] converged
+  int *flags;   // [0] list overflow
+};
+
+#define OUTW 6
+
+struct BlockRed {
+  double v[GH_BBLOCK / 64][8];  // up to 8 values per reduction
+  int flag;
+};
+
+// Block-wide sums of NV values; the result is returned to every thread.
+template <int NV>
+__device__ __forceinline__ void bsum(BlockRed &R, double (&v)[NV]) {
+  static_assert(NV <= 8, "BlockRed holds 8 values per wave");
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) R.v[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < GH_BBLOCK / 64; ++w) s += R.v[w][k];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double bmax(BlockRed &R, double v) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) R.v[threadIdx.x >> 6][0] = v;
+  __syncthreads();
+  double m = R.v[0][0];
+  for (int w = 1; w < GH_BBLOCK / 64; ++w) m = fmax(m, R.v[w][0]);
+  __syncthreads();
+  return m;
+}
+
+// verlet_self list of one system (all pairs, rd <= rl^2, i excluded).
+__device__ void sys_build(const BatchArgs &B, int a0, int a1) {
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double4 pi = B.pos[i];
+    int m = 0;
+    for (int j = a0; j < a1; ++j) {
+      if (j == i) continue;
+      const double4 pj = B.pos[j];
+      if (dist2_rn(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) <= B.rl2) {
+        if (m < B.cap) B.list[(size_t)m * B.ld + i] = (unsigned short)(j - a0);
+        ++m;
+      }
+    }
+    if (m > B.cap) B.flags[0] = 1;
+    B.nn[i] = min(m, B.cap);
+    B.xold[i] = pi.x;
+    B.xold[B.ld + i] = pi.y;
+    B.xold[2 * B.ld + i] = pi.z;
+  }
+  __syncthreads();
+}
+
+// Both SMA-TB passes of one system; ends with a barrier.
+template <bool MIXED, bool SINGLE>
+__device__ void sys_forces(const BatchArgs &B, const TbDev *sp, int a0, int a1) {
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double4 pi = B.pos[i];
+    const int ti = SINGLE ? 0 : B.type[i];
+    const int m = B.nn[i];
+    double acc = 0.0;
+    for (int k = 0; k < m; ++k) {
+      const int j = a0 + B.list[(size_t)k * B.ld + i];
+      const double4 pj = B.pos[j];
+      const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + B.type[j]];
+      const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+      if (MIXED) {
+        const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+        if (r2 > P.rce2) continue;
+        const float r2f = (float)r2;
+        acc += (double)tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
+      } else {
+        const double dr = sqrt(dist2_rn(vx, vy, vz));
+        if (dr > P.rce) continue;
+        acc += tb_band64(P, dr);
+      }
+    }
+    B.rho[i] = acc;
+    const double eb = 1.0 / sqrt(acc);
+    B.eband[i] = eb;
+    B.eband32[i] = (float)eb;
+  }
+  __syncthreads();
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double4 pi = B.pos[i];
+    const int ti = SINGLE ? 0 : B.type[i];
+    const int m = B.nn[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+    const double ebi = B.eband[i];
+    const float ebi32 = B.eband32[i];
+    for (int k = 0; k < m; ++k) {
+      const int j = a0 + B.list[(size_t)k * B.ld + i];
+      const double4 pj = B.pos[j];
+      const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + B.type[j]];
+      const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+      if (MIXED) {
+        const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+        if (r2 > P.rce2) continue;
+        const float r2f = (float)r2;
+        const float rinv = __builtin_amdgcn_rsqf(r2f);
+        float factor, repul;
+        tb_pair32(P, r2f * rinv, ebi32 + B.eband32[j], factor, repul);
+        const double sf = (double)(factor * rinv);
+        fx = fma(-sf, vx, fx);
+        fy = fma(-sf, vy, fy);
+        fz = fma(-sf, vz, fz);
+        ei += (double)repul;
+      } else {
+        const double dr = sqrt(dist2_rn(vx, vy, vz));
+        if (dr > P.rce) continue;
+        double factor, repul;
+        tb_pair64(P, dr, ebi + B.eband[j], factor, repul);
+        const double sf = factor / dr;
+        fx -= sf * vx;
+        fy -= sf * vy;
+        fz -= sf * vz;
+        ei += repul;
+      }
+    }
+    B.frc[i] = fx * B.ev_ui;
+    B.frc[B.ld + i] = fy * B.ev_ui;
+    B.frc[2 * B.ld + i] = fz * B.ev_ui;
+    B.e[i] = (ei - sqrt(B.rho[i])) * B.ev_ui;
+  }
+  __syncthreads();
+}
+
+template <bool SINGLE>
+__device__ __forceinline__ void load_sp(TbDev *sp, const BatchArgs &B) {
+  if (SINGLE) {
+    if (threadIdx.x == 0) sp[0] = B.tbd[0];
+  } else {
+    load_params(sp, B.tbd, B.nt);
+  }
+  __syncthreads();
+}
+
+// epot (ui) and ekin about the centre of mass (ui) of one system.
+__device__ void sys_energies(const BatchArgs &B, BlockRed &R, int a0, int a1, double &epot,
+                             double &ekin) {
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double m = 1.0 / B.om[B.type[i]];
+    s[0] += B.e[i];
+    s[1] += m;
+    s[2] += m * B.vel[i];
+    s[3] += m * B.vel[B.ld + i];
+    s[4] += m * B.vel[2 * B.ld + i];
+  }
+  bsum<5>(R, s);
+  const double cx = s[2] / s[1], cy = s[3] / s[1], cz = s[4] / s[1];
+  double k[1] = {0.0};
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double m = 1.0 / B.om[B.type[i]];
+    const double wx = B.vel[i] - cx, wy = B.vel[B.ld + i] - cy, wz = B.vel[2 * B.ld + i] - cz;
+    k[0] += (wx * wx + wy * wy + wz * wz) * m;
+  }
+  bsum<1>(R, k);
+  epot = s[0];
+  ekin = 0.5 * k[0];
+}
+
+template <bool MIXED, bool SINGLE>
+__global__ __launch_bounds__(GH_BBLOCK) void k_batch_compute(BatchArgs B) {
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  __shared__ BlockRed R;
+  load_sp<SINGLE>(sp, B);
+  const int s = B.order[blockIdx.x];
+  const int a0 = B.off[s], a1 = B.off[s + 1];
+  sys_build(B, a0, a1);
+  sys_forces<MIXED, SINGLE>(B, sp, a0, a1);
+  double ep, ek;
+  sys_energies(B, R, a0, a1, ep, ek);
+  if (threadIdx.x == 0) {
+    B.out[s * OUTW + 0] = ep;
+    B.out[s * OUTW + 1] = ek;
+  }
+}
+
+struct ErmakC {
+  double c0, c1, c2, sdr, sdv, crv1, crv2, dt, kB_ui;
+};
+
+// nsteps Ermak steps of every system (ermak_a / ermak_b with the Philox
+// stream keyed by (seed, step, atom)); lists rebuilt per system when
+// 2 max|x - x_old| exceeds the skin.
+template <bool MIXED, bool SINGLE>
+__global__ __launch_bounds__(GH_BBLOCK) void k_batch_langevin(BatchArgs B, int nsteps,
+                                                              long long step0, ErmakC k,
+                                                              unsigned long long seed,
+                                                              double skin) {
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  __shared__ BlockRed R;
+  load_sp<SINGLE>(sp, B);
+  const int s = B.order[blockIdx.x];
+  const int a0 = B.off[s], a1 = B.off[s + 1];
+  const double skt = sqrt(k.kB_ui * B.T[s]);
+  int nreb = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    const long long step = step0 + st;
+    double d2 = 0.0;
+    for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+      double g[6];
+      philox_normals(seed, step, i, g);
+      const double f_lan = skt * B.osm[B.type[i]];
+      double4 p = B.pos[i];
+      double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double pos_v = f_lan * k.sdr * g[2 * d];
+        x[d] = x[d] + k.c1 * B.vel[d * B.ld + i] + k.c2 * k.dt * B.acc[d * B.ld + i] + pos_v;
+        const double dd = x[d] - B.xold[d * B.ld + i];
+        d2 += dd * dd;
+      }
+      p.x = x[0];
+      p.y = x[1];
+      p.z = x[2];
+      B.pos[i] = p;
+    }
+    const double dmax2 = bmax(R, d2);
+    if (2.0 * sqrt(dmax2) > skin) {
+      sys_build(B, a0, a1);
+      ++nreb;
+    }
+    sys_forces<MIXED, SINGLE>(B, sp, a0, a1);
+    for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+      double g[6];
+      philox_normals(seed, step, i, g);
+      const int t = B.type[i];
+      const double f_lan = skt * B.osm[t];
+      const double om = B.om[t];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double vel_v = f_lan * k.sdv * (k.crv1 * g[2 * d] + k.crv2 * g[2 * d + 1]);
+        const double f = B.frc[d * B.ld + i];
+        B.vel[d * B.ld + i] = k.c0 * B.vel[d * B.ld + i] + (k.c1 - k.c2) * B.acc[d * B.ld + i] +
+                              k.c2 * f * om + vel_v;
+        B.acc[d * B.ld + i] = f * om;
+      }
+    }
+    __syncthreads();
+  }
+  double ep, ek;
+  sys_energies(B, R, a0, a1, ep, ek);
+  if (threadIdx.x == 0) {
+    B.out[s * OUTW + 0] = ep;
+    B.out[s * OUTW + 1] = ek;
+    B.out[s * OUTW + 2] += nreb;
+  }
+}
+
+struct FireC {
+  double dt_max, alpha0, fswitch, ftol, ui_ev;
+  int max_iter, mixed;
+};
+
+template <bool SINGLE>
+__device__ __forceinline__ void fire_forces(const BatchArgs &B, const TbDev *sp, int a0, int a1,
+                                            bool mixed) {
+  if (mixed) sys_forces<true, SINGLE>(B, sp, a0, a1);
+  else sys_forces<false, SINGLE>(B, sp, a0, a1);
+}
+
+__device__ double sys_fmax(const BatchArgs &B, BlockRed &R, int a0, int a1) {
+  double m = 0.0;
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double fx = B.frc[i], fy = B.frc[B.ld + i], fz = B.frc[2 * B.ld + i];
+    m = fmax(m, fx * fx + fy * fy + fz * fz);
+  }
+  return sqrt(bmax(R, m));
+}
+
+// FIRE (Bitzek et al. 2006) per system: velocity-Verlet MD step, then
+// P = F.v; v <- (1-a) v + a |v| F/|F|; P > 0 for more than N_min steps grows
+// dt (x1.1, capped at dt_max) and shrinks a (x0.99); P <= 0 halves dt, zeroes
+// v and resets a.  Mixed precision switches to FP64 pair terms once
+// max|F| < fswitch; convergence is max|F| < ftol in FP64.
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_BBLOCK) void k_batch_fire(BatchArgs B, FireC fc, double skin) {
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  __shared__ BlockRed R;
+  load_sp<SINGLE>(sp, B);
+  const int s = B.order[blockIdx.x];
+  const int a0 = B.off[s], a1 = B.off[s + 1];
+  bool mixed = fc.mixed != 0;
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x)
+    for (int d = 0; d < 3; ++d) B.vel[d * B.ld + i] = 0.0;
+  sys_build(B, a0, a1);
+  fire_forces<SINGLE>(B, sp, a0, a1, mixed);
+  double dt = fc.dt_max * 0.1, alpha = fc.alpha0;
+  int npos = 0, it = 0, conv = 0;
+  double fmax = 0.0;
+  for (it = 0; it < fc.max_iter; ++it) {
+    fmax = sys_fmax(B, R, a0, a1) * fc.ui_ev;  // eV/A
+    if (mixed && fmax < fc.fswitch) {
+      mixed = false;
+      fire_forces<SINGLE>(B, sp, a0, a1, mixed);
+      fmax = sys_fmax(B, R, a0, a1) * fc.ui_ev;
+    }
+    if (!mixed && fmax < fc.ftol) {
+      conv = 1;
+      break;
+    }
+    // velocity Verlet step
+    double d2 = 0.0;
+    for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+      const double om = B.om[B.type[i]];
+      double4 p = B.pos[i];
+      double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double v = B.vel[d * B.ld + i] + 0.5 * dt * B.frc[d * B.ld + i] * om;
+        B.vel[d * B.ld + i] = v;
+        x[d] += dt * v;
+        const double dd = x[d] - B.xold[d * B.ld + i];
+        d2 += dd * dd;
+      }
+      p.x = x[0];
+      p.y = x[1];
+      p.z = x[2];
+      B.pos[i] = p;
+    }
+    const double dmax2 = bmax(R, d2);
+    if (2.0 * sqrt(dmax2) > skin) sys_build(B, a0, a1);
+    fire_forces<SINGLE>(B, sp, a0, a1, mixed);
+    double pv[3] = {0.0, 0.0, 0.0};  // F.v, |v|^2, |F|^2
+    for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+      const double om = B.om[B.type[i]];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double f = B.frc[d * B.ld + i];
+        const double v = B.vel[d * B.ld + i] + 0.5 * dt * f * om;
+        B.vel[d * B.ld + i] = v;
+        pv[0] += f * v;
+        pv[1] += v * v;
+        pv[2] += f * f;
+      }
+    }
+    bsum<3>(R, pv);
+    const double mixv = pv[2] > 0.0 ? alpha * sqrt(pv[1]) / sqrt(pv[2]) : 0.0;
+    const bool up = pv[0] > 0.0;
+    for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double v = B.vel[d * B.ld + i];
+        B.vel[d * B.ld + i] = up ? (1.0 - alpha) * v + mixv * B.frc[d * B.ld + i] : 0.0;
+      }
+    if (up) {
+      ++npos;
+      if (npos > 5) {
+        dt = fmin(dt * 1.1, fc.dt_max);
+        alpha *= 0.99;
+      }
+    } else {
+      npos = 0;
+      dt *= 0.5;
+      alpha = fc.alpha0;
+    }
+    __syncthreads();
+  }
+  double ep, ek;
+  sys_energies(B, R, a0, a1, ep, ek);
+  if (threadIdx.x == 0) {
+    B.out[s * OUTW + 0] = ep;
+    B.out[s * OUTW + 3] = fmax;
+    B.out[s * OUTW + 4] = it;
+    B.out[s * OUTW + 5] = conv;
+  }
+}
+
+__global__ void k_batch_scale(int nsys, const int *__restrict__ off,
+                              const double *__restrict__ scale, double *__restrict__ vel,
+                              int ld) {
+  const int s = blockIdx.x;
+  if (s >= nsys) return;
+  const double f = scale[s];
+  for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x)
+    for (int d = 0; d < 3; ++d) vel[d * ld + i] *= f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int bfail(gh_batch *b, int code, const std::string &m) {
+  if (b) b->err = m;
+  return code;
+}
+#define BH(b, call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return bfail(b, GH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+#define BT(x)                 \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+template <typename T>
+static int balloc(gh_batch *b, T **p, size_t n) {
+  if (*p) {
+    BH(b, hipFree(*p));
+    *p = nullptr;
+  }
+  if (n) BH(b, hipMalloc((void **)p, sizeof(T) * n));
+  return 0;
+}
+
+static BatchArgs bargs(const gh_batch *b) {
+  BatchArgs A;
+  A.off = b->off;
+  A.order = b->order;
+  A.type = b->type;
+  A.pos = b->pos;
+  A.vel = b->vel;
+  A.acc = b->acc;
+  A.frc = b->frc;
+  A.e = b->e;
+  A.rho = b->rho;
+  A.eband = b->eband;
+  A.xold = b->xold;
+  A.eband32 = b->eband32;
+  A.nn = b->nn;
+  A.list = b->list;
+  A.cap = b->cap;
+  A.ld = b->ld;
+  A.tbd = b->tbd;
+  A.nt = b->nt;
+  for (int t = 0; t < GH_MAXT; ++t) {
+    A.om[t] = 1.0 / b->mass_t[t];
+    A.osm[t] = sqrt(A.om[t]);
+  }
+  const double rl = b->rcut + b->skin;
+  A.rl2 = rl * rl;
+  A.ev_ui = ghc::ev_ui();
+  A.T = b->T;
+  A.out = b->out;
+  A.flags = b->flags;
+  return A;
+}
+
+static int check_flags(gh_batch *b) {
+  int f = 0;
+  BH(b, hipMemcpyAsync(&f, b->flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BH(b, hipStreamSynchronize(b->stream));
+  if (f) {
+    b->lists_valid = false;
+    return bfail(b, GH_EOVERFLOW, "a cluster's neighbour row exceeded the capacity; "
+                                  "results of this call are invalid; call again after "
+                                  "gh_batch_set_skin or with fewer atoms per system");
+  }
+  return 0;
+}
+
+static int fetch_out(gh_batch *b, std::vector<double> &o) {
+  o.resize((size_t)b->nsys * OUTW);
+  BH(b, hipMemcpyAsync(o.data(), b->out, sizeof(double) * o.size(), hipMemcpyDeviceToHost,
+                       b->stream));
+  BH(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// capacity of a list row: every other atom of the largest system, capped by
+// the densest packing within rcut+skin (fcc metals: < 90 neighbours)
+static int batch_ready(gh_batch *b) {
+  if (!b->nsys) return bfail(b, GH_ESTATE, "no systems");
+  if (!b->have_pot) return bfail(b, GH_ESTATE, "no potential");
+  if (b->cap == 0) {
+    int nmax = 0;
+    for (int s = 0; s < b->nsys; ++s) nmax = std::max(nmax, b->off_h[s + 1] - b->off_h[s]);
+    const double rl = b->rcut + b->skin;
+    // close packing density of the smallest pair distance (0.8 r0 at most)
+    double rmin = 1e30;
+    for (int i = 0; i < b->nt; ++i)
+      for (int j = 0; j < b->nt; ++j) rmin = std::min(rmin, 0.8 * b->tb[i * GH_MAXT + j].r0);
+    const double dens = 1.4142135623730951 / (rmin * rmin * rmin);
+    int est = (int)(4.18879 * rl * rl * rl * dens) + 16;
+    b->cap = std::max(8, std::min(nmax, est));
+    b->cap = (b->cap + 7) / 8 * 8;
+    BT(balloc(b, &b->list, (size_t)b->cap * b->ld));
+  }
+  return 0;
+}
+
+extern "C" {
+
+int gh_batch_create(gh_batch **out, int device, int precision) {
+  if (!out) return GH_EINVAL;
+  *out = nullptr;
+  if (precision != GH_FP64 && precision != GH_MIXED) return GH_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return GH_EHIP;
+  gh_batch *b = new gh_batch();
+  b->device = device;
+  b->precision = precision;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void **)&b->flags, sizeof(int) * 4) != hipSuccess ||
+      hipMemset(b->flags, 0, sizeof(int) * 4) != hipSuccess) {
+    delete b;
+    return GH_EHIP;
+  }
+  *out = b;
+  return 0;
+}
+
+void gh_batch_destroy(gh_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  void *p[] = {b->off, b->order, b->type, b->nn, b->flags, b->pos, b->vel, b->acc, b->frc,
+               b->e, b->rho, b->eband, b->xold, b->T, b->out, b->scale, b->eband32, b->list,
+               b->tbd};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *gh_batch_last_error(const gh_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int gh_batch_set_precision(gh_batch *b, int precision) {
+  if (!b) return GH_EINVAL;
+  if (precision != GH_FP64 && precision != GH_MIXED) return bfail(b, GH_EINVAL, "precision");
+  b->precision = precision;
+  b->forces_valid = false;
+  return 0;
+}
+
+int gh_batch_set_systems(gh_batch *b, int nsys, const int *off, const double *x,
+                         const int *type) {
+  if (!b || nsys <= 0 || !off || !x) return bfail(b, GH_EINVAL, "gh_batch_set_systems");
+  if (off[0] != 0) return bfail(b, GH_EINVAL, "off[0] must be 0");
+  for (int s = 0; s < nsys; ++s) {
+    const int n = off[s + 1] - off[s];
+    if (n < 1 || n > 65535) return bfail(b, GH_EINVAL, "systems need 1..65535 atoms");
+  }
+  BH(b, hipSetDevice(b->device));
+  b->nsys = nsys;
+  b->ntot = off[nsys];
+  b->ld = (b->ntot + 255) / 256 * 256;
+  const int ld = b->ld;
+  b->off_h.assign(off, off + nsys + 1);
+  b->order_h.resize(nsys);
+  std::iota(b->order_h.begin(), b->order_h.end(), 0);
+  std::stable_sort(b->order_h.begin(), b->order_h.end(), [&](int a, int c) {
+    return off[a + 1] - off[a] > off[c + 1] - off[c];
+  });
+  b->T_h.assign(nsys, 0.0);
+  BT(balloc(b, &b->off, nsys + 1));
+  BT(balloc(b, &b->order, nsys));
+  BT(balloc(b, &b->type, ld));
+  BT(balloc(b, &b->nn, ld));
+  BT(balloc(b, &b->pos, ld));
+  BT(balloc(b, &b->vel, 3 * (size_t)ld));
+  BT(balloc(b, &b->acc, 3 * (size_t)ld));
+  BT(balloc(b, &b->frc, 3 * (size_t)ld));
+  BT(balloc(b, &b->e, ld));
+  BT(balloc(b, &b->rho, ld));
+  BT(balloc(b, &b->eband, ld));
+  BT(balloc(b, &b->eband32, ld));
+  BT(balloc(b, &b->xold, 3 * (size_t)ld));
+  BT(balloc(b, &b->T, nsys));
+  BT(balloc(b, &b->out, (size_t)nsys * OUTW));
+  BT(balloc(b, &b->scale, nsys));
+  b->cap = 0;
+  if (b->list) {
+    BH(b, hipFree(b->list));
+    b->list = nullptr;
+  }
+  std::vector<double4> hp(ld, make_double4(0, 0, 0, 0));
+  std::vector<int> ht(ld, 0);
+  for (int i = 0; i < b->ntot; ++i) {
+    hp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+    ht[i] = type ? type[i] : 0;
+    if (ht[i] < 0 || ht[i] >= GH_MAXT) return bfail(b, GH_EINVAL, "type out of range");
+  }
+  BH(b, hipMemcpy(b->pos, hp.data(), sizeof(double4) * ld, hipMemcpyHostToDevice));
+  BH(b, hipMemcpy(b->type, ht.data(), sizeof(int) * ld, hipMemcpyHostToDevice));
+  BH(b, hipMemcpy(b->off, off, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice));
+  BH(b, hipMemcpy(b->order, b->order_h.data(), sizeof(int) * nsys, hipMemcpyHostToDevice));
+  BH(b, hipMemset(b->vel, 0, sizeof(double) * 3 * ld));
+  BH(b, hipMemset(b->acc, 0, sizeof(double) * 3 * ld));
+  BH(b, hipMemset(b->frc, 0, sizeof(double) * 3 * ld));
+  BH(b, hipMemset(b->out, 0, sizeof(double) * nsys * OUTW));
+  BH(b, hipMemset(b->T, 0, sizeof(double) * nsys));
+  b->lists_valid = false;
+  b->forces_valid = false;
+  return 0;
+}
+
+int gh_batch_set_smatb(gh_batch *b, int nt, const double *A, const double *xi, const double *p,
+                       const double *q, const double *r0, const double *rci, const double *rce) {
+  if (!b || nt <= 0 || nt > GH_MAXT) return bfail(b, GH_EINVAL, "gh_batch_set_smatb");
+  // reuse the single-system parameter upload through a scratch context
+  gh_ctx tmp;
+  tmp.device = b->device;
+  int rc = gh_set_smatb(&tmp, nt, A, xi, p, q, r0, rci, rce);
+  if (rc) return bfail(b, rc, tmp.err);
+  for (int k = 0; k < GH_MAXT * GH_MAXT; ++k) b->tb[k] = tmp.tb[k];
+  if (b->tbd) BH(b, hipFree(b->tbd));
+  b->tbd = tmp.tbd;  // ownership moves to the batch
+  tmp.tbd = nullptr;
+  b->nt = nt;
+  b->rcut = tmp.rcut;
+  b->have_pot = true;
+  b->cap = 0;
+  b->lists_valid = false;
+  b->forces_valid = false;
+  return 0;
+}
+
+int gh_batch_set_masses(gh_batch *b, int nt, const double *mass) {
+  if (!b || nt <= 0 || nt > GH_MAXT || !mass) return bfail(b, GH_EINVAL, "gh_batch_set_masses");
+  for (int t = 0; t < nt; ++t) {
+    if (!(mass[t] > 0)) return bfail(b, GH_EINVAL, "masses must be positive");
+    b->mass_t[t] = mass[t];
+  }
+  return 0;
+}
+
+int gh_batch_set_skin(gh_batch *b, double skin) {
+  if (!b || !(skin >= 0)) return bfail(b, GH_EINVAL, "skin must be >= 0");
+  b->skin = skin;
+  b->cap = 0;
+  b->lists_valid = false;
+  return 0;
+}
+
+int gh_batch_set_velocities(gh_batch *b, const double *v) {
+  if (!b || !b->nsys) return bfail(b, GH_ESTATE, "no systems");
+  std::vector<double> h(3 * (size_t)b->ld, 0.0);
+  if (v)
+    for (int i = 0; i < b->ntot; ++i)
+      for (int d = 0; d < 3; ++d) h[(size_t)d * b->ld + i] = v[3 * (size_t)i + d];
+  BH(b, hipMemcpy(b->vel, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int gh_batch_set_temperatures(gh_batch *b, const double *T) {
+  if (!b || !b->nsys || !T) return bfail(b, GH_ESTATE, "no systems");
+  b->T_h.assign(T, T + b->nsys);
+  BH(b, hipMemcpy(b->T, T, sizeof(double) * b->nsys, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int gh_batch_compute(gh_batch *b, double *F, double *epot) {
+  if (!b) return GH_EINVAL;
+  BH(b, hipSetDevice(b->device));
+  BT(batch_ready(b));
+  const BatchArgs A = bargs(b);
+  const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
+  if (mixed) {
+    if (single) k_batch_compute<true, true><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
+    else k_batch_compute<true, false><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
+  } else {
+    if (single) k_batch_compute<false, true><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
+    else k_batch_compute<false, false><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
+  }
+  BH(b, hipGetLastError());
+  BT(check_flags(b));
+  b->lists_valid = true;
+  b->forces_valid = true;
+  std::vector<double> o;
+  BT(fetch_out(b, o));
+  const double ui_ev = ghc::ui_ev();
+  b->epot_h.resize(b->nsys);
+  for (int s = 0; s < b->nsys; ++s) b->epot_h[s] = o[s * OUTW] * ui_ev;
+  if (epot)
+    for (int s = 0; s < b->nsys; ++s) epot[s] = b->epot_h[s];
+  if (F) {
+    std::vector<double> h(3 * (size_t)b->ld);
+    BH(b, hipMemcpy(h.data(), b->frc, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ntot; ++i)
+      for (int d = 0; d < 3; ++d) F[3 * (size_t)i + d] = h[(size_t)d * b->ld + i];
+  }
+  return 0;
+}
+
+int gh_batch_langevin(gh_batch *b, int nsteps, double dt, double gamma, uint64_t seed,
+                      double *epot, double *ekin) {
+  if (!b || nsteps < 0 || !(dt > 0) || !(gamma > 0))
+    return bfail(b, GH_EINVAL, "langevin needs nsteps >= 0, dt > 0, gamma > 0");
+  if (!b->forces_valid) BT(gh_batch_compute(b, nullptr, nullptr));
+  ErmakC k;
+  k.c0 = exp(-dt * gamma);
+  k.c1 = (1.0 - k.c0) / gamma;
+  k.c2 = (1.0 - k.c1 / dt) / gamma;
+  k.sdr = sqrt(dt / gamma * (2.0 - (3.0 - 4.0 * k.c0 + k.c0 * k.c0) / (dt * gamma)));
+  k.sdv = sqrt(1.0 - k.c0 * k.c0);
+  k.crv1 = (1.0 - k.c0) * (1.0 - k.c0) / (gamma * k.sdr * k.sdv);
+  k.crv2 = sqrt(1.0 - (k.crv1 * k.crv1));
+  k.dt = dt;
+  k.kB_ui = ghc::kB_ui();
+  const BatchArgs A = bargs(b);
+  const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
+#define LAUNCH(MX, SG)                                                                 \
+  k_batch_langevin<MX, SG><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A, nsteps, b->step + 1, k, \
+                                                                 seed, b->skin)
+  if (mixed) {
+    if (single) LAUNCH(true, true); else LAUNCH(true, false);
+  } else {
+    if (single) LAUNCH(false, true); else LAUNCH(false, false);
+  }
+#undef LAUNCH
+  BH(b, hipGetLastError());
+  b->step += nsteps;
+  BT(check_flags(b));
+  std::vector<double> o;
+  BT(fetch_out(b, o));
+  const double ui_ev = ghc::ui_ev();
+  b->epot_h.resize(b->nsys);
+  for (int s = 0; s < b->nsys; ++s) {
+    b->epot_h[s] = o[s * OUTW] * ui_ev;
+    if (epot) epot[s] = b->epot_h[s];
+    if (ekin) ekin[s] = o[s * OUTW + 1] * ui_ev;
+  }
+  return 0;
+}
+
+// splitmix64: host stream for the Metropolis draws of the exchanges
+static double smix_u01(uint64_t &st) {
+  uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+int gh_batch_exchange(gh_batch *b, int nladder, int R, int *sys, int parity, uint64_t seed,
+                      int *naccept) {
+  if (!b || nladder <= 0 || R < 2 || !sys) return bfail(b, GH_EINVAL, "gh_batch_exchange");
+  if ((int)b->epot_h.size() != b->nsys) return bfail(b, GH_ESTATE, "no energies yet");
+  const double kB = ghc::kB_ui(), ev_ui = ghc::ev_ui();
+  std::vector<double> scale(b->nsys, 1.0);
+  uint64_t st = seed;
+  int acc = 0;
+  for (int l = 0; l < nladder; ++l)
+    for (int r = parity & 1; r + 1 < R; r += 2) {
+      const int lo = sys[l * R + r], hi = sys[l * R + r + 1];
+      // ReplicaExchange.f90:86-110: deltab = beta - beta_partner,
+      // deltau = (E_partner - E) deltab; swap if deltau <= 0 or u < e^{-deltau}
+      const double beta = 1.0 / (kB * b->T_h[lo]), betap = 1.0 / (kB * b->T_h[hi]);
+      const double du = (b->epot_h[hi] - b->epot_h[lo]) * ev_ui * (beta - betap);
+      bool swap = true;
+      if (du > 0) swap = smix_u01(st) < exp(-du);
+      if (!swap) continue;
+      ++acc;
+      // the configurations trade temperatures: velocities scale by sqrt(T_new/T_old)
+      std::swap(b->T_h[lo], b->T_h[hi]);
+      scale[lo] = sqrt(b->T_h[lo] / b->T_h[hi]);
+      scale[hi] = sqrt(b->T_h[hi] / b->T_h[lo]);
+      std::swap(sys[l * R + r], sys[l * R + r + 1]);
+    }
+  BH(b, hipMemcpyAsync(b->T, b->T_h.data(), sizeof(double) * b->nsys, hipMemcpyHostToDevice,
+                       b->stream));
+  BH(b, hipMemcpyAsync(b->scale, scale.data(), sizeof(double) * b->nsys, hipMemcpyHostToDevice,
+                       b->stream));
+  k_batch_scale<<<b->nsys, GH_BBLOCK, 0, b->stream>>>(b->nsys, b->off, b->scale, b->vel, b->ld);
+  BH(b, hipGetLastError());
+  BH(b, hipStreamSynchronize(b->stream));
+  if (naccept) *naccept = acc;
+  return 0;
+}
+
+int gh_batch_fire(gh_batch *b, int max_iter, double dt_max, double alpha0, double fswitch,
+                  double ftol, int *iters, int *converged, double *epot, double *fmax) {
+  if (!b || max_iter < 0 || !(dt_max > 0)) return bfail(b, GH_EINVAL, "gh_batch_fire");
+  BH(b, hipSetDevice(b->device));
+  BT(batch_ready(b));
+  FireC fc;
+  fc.dt_max = dt_max;
+  fc.alpha0 = alpha0;
+  fc.fswitch = fswitch;
+  fc.ftol = ftol;
+  fc.ui_ev = ghc::ui_ev();
+  fc.max_iter = max_iter;
+  fc.mixed = b->precision == GH_MIXED;
+  const BatchArgs A = bargs(b);
+  if (b->nt == 1) k_batch_fire<true><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A, fc, b->skin);
+  else k_batch_fire<false><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A, fc, b->skin);
+  BH(b, hipGetLastError());
+  BT(check_flags(b));
+  b->lists_valid = true;
+  b->forces_valid = false;  // the last forces may be FP64 while precision is mixed
+  std::vector<double> o;
+  BT(fetch_out(b, o));
+  const double ui_ev = ghc::ui_ev();
+  b->epot_h.resize(b->nsys);
+  for (int s = 0; s < b->nsys; ++s) {
+    b->epot_h[s] = o[s * OUTW] * ui_ev;
+    if (epot) epot[s] = b->epot_h[s];
+    if (fmax) fmax[s] = o[s * OUTW + 3];
+    if (iters) iters[s] = (int)o[s * OUTW + 4];
+    if (converged) converged[s] = (int)o[s * OUTW + 5];
+  }
+  return 0;
+}
+
+int gh_batch_get_state(gh_batch *b, double *x, double *v) {
+  if (!b || !b->nsys) return bfail(b, GH_ESTATE, "no systems");
+  if (x) {
+    std::vector<double4> h(b->ld);
+    BH(b, hipMemcpy(h.data(), b->pos, sizeof(double4) * b->ld, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ntot; ++i) {
+      x[3 * (size_t)i] = h[i].x;
+      x[3 * (size_t)i + 1] = h[i].y;
+      x[3 * (size_t)i + 2] = h[i].z;
+    }
+  }
+  if (v) {
+    std::vector<double> h(3 * (size_t)b->ld);
+    BH(b, hipMemcpy(h.data(), b->vel, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ntot; ++i)
+      for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = h[(size_t)d * b->ld + i];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -52,13 +52,34 @@ struct TbDev {
   // FP64 form
   double ca, eps2, p, q, r0, one_r0, rci, rce, rce2, b1, b2, b3, r1, r2, r3;
   double p_r0, q2_r0;  // p/r0, 2q/r0
-  // FP32 form: exponents as base-2 affine functions of r
-  float rep_a, rep_b;  // A*exp(-p(r/r0-1))   = exp2(rep_a + rep_b*r)
-  float rho_a, rho_b;  // xi^2*exp(-2q(r/r0-1)) = exp2(rho_a + rho_b*r)
+  // FP32 form: A e^{-p(r/r0-1)} = rep_a * 2^{rep_b (r - r0f)} with r0f the
+  // float nearest r0 (r - r0f is exact near r0) and rep_a compensating the
+  // rounding of r0 in FP64, so no constant of the exponent carries a bias.
+  float r0f;
+  float rep_a, rep_b;
+  float rho_a, rho_b;  // xi^2 e^{-2q(r/r0-1)} = rho_a * 2^{rho_b (r - r0f)}
   float frep, fband;   // 2p/r0 and q/r0 (for -dE/dr)
   float rci_f, rce_f;
   float b1f, b2f, b3f, r1f, r2f, r3f;
 };
+
+// Cell grid and its grouping into bricks.  Atoms are sorted by key =
+// brick * cpb + cell-within-brick, so each brick's atoms are contiguous and
+// one workgroup stages the brick's halo (the bricks' cells grown by one cell)
+// into LDS.  Grid axes: nc interior cells; non-periodic axes add one guard
+// cell on each side (gofs = 1, gn = nc + 2).
+struct TileGeom {
+  int nc[3], gn[3], gofs[3], pbc[3];
+  int bd[3];      // brick edge in cells
+  int nbr[3];     // bricks per axis (ceil(gn / bd))
+  int cpb;        // cells per brick
+  int nbrick;     // bricks
+  int nkey;       // nbrick * cpb
+  int slot_cap;   // LDS slots per workgroup (halo atoms)
+  double L[3];
+  double one_cell[3];
+};
+#define GH_MAXHALO 216  // (4 + 2)^3 halo cells at most (brick edge <= 4)
 
 struct GhStatus {                     // device-resident counters
   unsigned long long maxdisp2_bits;   // max |dx|^2 since the last build
@@ -68,6 +89,9 @@ struct GhStatus {                     // device-resident counters
   int outside;                        // an atom left a non-periodic grid
   long long nupd;                     // rebuilds so far
   int force_rebuild;                  // host request
+  int tile_overflow;                  // a brick's halo exceeded slot_cap (sticky)
+  int max_halo;                       // largest halo (atoms) at the last build
+  int max_brick;                      // largest brick (atoms) at the last build
   int pad;
 };
 
@@ -112,6 +136,8 @@ struct gh_ctx {
   int gn[3] = {1, 1, 1};      // cells per dim including guard cells
   double cell[3] = {1, 1, 1}; // cell edge
   int ncell = 1;
+  TileGeom tg;                // brick grouping (cell mode)
+  int bd_req[3] = {0, 0, 0};  // requested brick edge (0 = automatic)
   int cap = 0;                // list row capacity
   int cap_req = 0;
 
@@ -135,9 +161,12 @@ struct gh_ctx {
   int *cell_count = nullptr, *cell_start = nullptr, *cell_cursor = nullptr;
   int *perm = nullptr;                        // [ncap]
   int *nn = nullptr;                          // [ncap]
-  int *list = nullptr;                        // [cap][ncap] ELL
+  int *list = nullptr;                        // [cap][ncap] ELL (O(N^2) mode)
+  unsigned short *list16 = nullptr;           // [cap][ncap] ELL of LDS slots (cell mode)
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
   GhStatus *status = nullptr;
+  double *dispblk = nullptr;                  // per-workgroup max displacement^2
+  int ndispblk = 0;
   double *red = nullptr;                      // reduction scratch
   int nred = 0;
   double *red_h = nullptr;                    // pinned result
@@ -173,7 +202,10 @@ int gh_launch_vv_b(gh_ctx *c);
 int gh_launch_ermak_a(gh_ctx *c);
 int gh_launch_ermak_b(gh_ctx *c);
 int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui);
-int gh_count_neighbours(gh_ctx *c, int *maxnn);
+int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only);
+int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid);
 int gh_fill_int(gh_ctx *c, int *p, int v, int n);
 int gh_launch_wrap(gh_ctx *c);
+int gh_tile_occupancy(gh_ctx *c, int *max_halo, int *max_brick);
+int gh_tiled_to_global(gh_ctx *c, int *dst);
 int gh_upload_tb(gh_ctx *c);

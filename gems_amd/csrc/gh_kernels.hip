@@ -13,25 +13,11 @@
 // Wavefronts are 64 lanes; per-atom kernels use 256-thread workgroups.
 #include <math.h>
 
-#include "gh_internal.h"
+#include "gh_device.h"
 
-#define GH_BLOCK 256
-#define GH_JMASK 0x07FFFFFFu
 
 namespace {
-
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_imax(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
-}
+using namespace ghd;
 
 // Block-wide sum of NV doubles; deterministic order.  Result valid in thread 0.
 template <int NV>
@@ -65,16 +51,29 @@ __global__ void k_fill_int(int *__restrict__ p, int v, int n) {
 // the skin.  Here the displacement is the true (minimum-image) one, and the
 // test uses 2*max, which bounds d1+d2 from above.
 // ---------------------------------------------------------------------------
-__global__ void k_decide(GhStatus *st, double skin) {
-  double d2 = __longlong_as_double((long long)st->maxdisp2_bits);
-  int rb = st->force_rebuild || (2.0 * sqrt(d2) > skin);
-  st->rebuild = rb;
-  if (rb) {
-    st->nupd += 1;
-    st->force_rebuild = 0;
-    st->maxnn = 0;
+__global__ __launch_bounds__(1024) void k_decide(GhStatus *st, double *__restrict__ dispblk,
+                                                 int nblk, double skin) {
+  __shared__ double part[16];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    m = fmax(m, dispblk[b]);
+    dispblk[b] = 0.0;  // consumed: the next integrator step writes fresh maxima
   }
-  st->maxdisp2_bits = 0ull;
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, part[w]);
+    const double d2 = fmax(m, __longlong_as_double((long long)st->maxdisp2_bits));
+    const int rb = st->force_rebuild || (2.0 * sqrt(d2) > skin);
+    st->rebuild = rb;
+    if (rb) {
+      st->nupd += 1;
+      st->force_rebuild = 0;
+      st->maxnn = 0;
+    }
+    st->maxdisp2_bits = 0ull;
+  }
 }
 
 struct GridArgs {
@@ -83,19 +82,6 @@ struct GridArgs {
   int ncell;
 };
 
-__device__ __forceinline__ int cell_coord(double x, const GridArgs &g, int d,
-                                          int *outside) {
-  int c = (int)floor(x * g.one_cell[d]);
-  if (g.pbc[d]) {
-    c %= g.nc[d];
-    if (c < 0) c += g.nc[d];
-  } else {
-    c += g.gofs[d];
-    if (c < 0) { c = 0; *outside = 1; }
-    if (c >= g.gn[d]) { c = g.gn[d] - 1; *outside = 1; }
-  }
-  return c;
-}
 
 __global__ void k_zero_cells(const GhStatus *st, int *__restrict__ count, int ncell) {
   if (!st->rebuild) return;
@@ -103,20 +89,36 @@ __global__ void k_zero_cells(const GhStatus *st, int *__restrict__ count, int nc
   if (c < ncell) count[c] = 0;
 }
 
-__global__ void k_bin(GhStatus *st, const double4 *__restrict__ pos, int n, GridArgs g,
+__device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, int *outside) {
+  int c = (int)floor(x * t.one_cell[d]);
+  if (t.pbc[d]) {
+    c %= t.nc[d];
+    if (c < 0) c += t.nc[d];
+  } else {
+    c += t.gofs[d];
+    if (c < 0) { c = 0; *outside = 1; }
+    if (c >= t.gn[d]) { c = t.gn[d] - 1; *outside = 1; }
+  }
+  return c;
+}
+
+// Sort key of each atom: brick * cpb + cell within the brick.
+__global__ void k_bin(GhStatus *st, const double4 *__restrict__ pos, int n, TileGeom t,
                       int *__restrict__ cellid, int *__restrict__ count) {
   if (!st->rebuild) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double4 p = pos[i];
   int out = 0;
-  int cx = cell_coord(p.x, g, 0, &out);
-  int cy = cell_coord(p.y, g, 1, &out);
-  int cz = cell_coord(p.z, g, 2, &out);
+  const int cx = tile_coord(p.x, t, 0, &out);
+  const int cy = tile_coord(p.y, t, 1, &out);
+  const int cz = tile_coord(p.z, t, 2, &out);
   if (out) st->outside = 1;
-  int c = cx + g.gn[0] * (cy + g.gn[1] * cz);
-  cellid[i] = c;
-  atomicAdd(&count[c], 1);
+  const int bx = cx / t.bd[0], by = cy / t.bd[1], bz = cz / t.bd[2];
+  const int lx = cx - bx * t.bd[0], ly = cy - by * t.bd[1], lz = cz - bz * t.bd[2];
+  const int key = ((bz * t.nbr[1] + by) * t.nbr[0] + bx) * t.cpb + (lz * t.bd[1] + ly) * t.bd[0] + lx;
+  cellid[i] = key;
+  atomicAdd(&count[key], 1);
 }
 
 // Exclusive scan of the cell counts in one 1024-thread workgroup.
@@ -218,76 +220,6 @@ __global__ void k_copyback(const GhStatus *st, int n, int ncap, const double4 *_
   orig[s] = orig_t[s];
 }
 
-// |vd|^2 rounded exactly as the reference's dot_product (no contraction).
-__device__ __forceinline__ double dist2_rn(double x, double y, double z) {
-  return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
-}
-
-// Cell-list build (linkedcell_self, Neighbour.F90:991-1087): full list at
-// radius rl, rd < rl^2 kept, i itself excluded.  vd = x_j - x_i, then the
-// periodic image shift, which equals the reference's vd - L*idnint(vd/L)
-// for every pair within rl (rl <= L/3 on periodic axes).
-__global__ __launch_bounds__(GH_BLOCK) void k_build_cells(
-    GhStatus *st, const double4 *__restrict__ pos, const int *__restrict__ cellof,
-    const int *__restrict__ start, int n, GridArgs g, double rl2, int cap, int ld,
-    int count_only, int *__restrict__ nn, unsigned *__restrict__ list,
-    double *__restrict__ xold, int ncap) {
-  if (!st->rebuild) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  int m = 0;
-  if (s < n) {
-    const double4 pi = pos[s];
-    const int c = cellof[s];
-    const int cx = c % g.gn[0], cy = (c / g.gn[0]) % g.gn[1], cz = c / (g.gn[0] * g.gn[1]);
-    for (int oz = -1; oz <= 1; ++oz) {
-      int nz = cz + oz, wz = 0;
-      if (g.pbc[2]) {
-        if (nz < 0) { nz += g.nc[2]; wz = -1; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; wz = 1; }
-      } else if (nz < 0 || nz >= g.gn[2]) continue;
-      for (int oy = -1; oy <= 1; ++oy) {
-        int ny = cy + oy, wy = 0;
-        if (g.pbc[1]) {
-          if (ny < 0) { ny += g.nc[1]; wy = -1; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; wy = 1; }
-        } else if (ny < 0 || ny >= g.gn[1]) continue;
-        for (int ox = -1; ox <= 1; ++ox) {
-          int nx = cx + ox, wx = 0;
-          if (g.pbc[0]) {
-            if (nx < 0) { nx += g.nc[0]; wx = -1; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; wx = 1; }
-          } else if (nx < 0 || nx >= g.gn[0]) continue;
-          const int c2 = nx + g.gn[0] * (ny + g.gn[1] * nz);
-          const unsigned code = (unsigned)((wx + 1) + 3 * (wy + 1) + 9 * (wz + 1)) << 27;
-          const double shx = wx * g.L[0], shy = wy * g.L[1], shz = wz * g.L[2];
-          const int jb = start[c2], je = start[c2 + 1];
-          for (int j = jb; j < je; ++j) {
-            if (j == s) continue;
-            const double4 pj = pos[j];
-            double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
-            if (wx) vx = vx + shx;
-            if (wy) vy = vy + shy;
-            if (wz) vz = vz + shz;
-            const double rd = dist2_rn(vx, vy, vz);
-            if (rd < rl2) {
-              if (!count_only && m < cap) list[(size_t)m * ld + s] = (unsigned)j | code;
-              ++m;
-            }
-          }
-        }
-      }
-    }
-    if (!count_only) {
-      nn[s] = min(m, cap);
-      xold[s] = pi.x;
-      xold[ncap + s] = pi.y;
-      xold[2 * ncap + s] = pi.z;
-    }
-  }
-  int wm = wave_imax(m);
-  if ((threadIdx.x & 63) == 0 && wm > 0) {
-    atomicMax(&st->maxnn, wm);
-    if (wm > cap) st->overflow = 1;
-  }
-}
-
 // O(N^2) build (verlet_self, Neighbour.F90:676-759): rd > rl^2 -> skip, so
 // equality is kept; minimum image by rounding as vdistance does.
 __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
@@ -344,13 +276,6 @@ __device__ __forceinline__ void load_shift(ShiftLds &sh, double L0, double L1, d
   }
 }
 
-__device__ __forceinline__ void load_params(TbDev *sp, const TbDev *__restrict__ tbd, int nt) {
-  for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) {
-    int a = k / GH_MAXT, b = k % GH_MAXT;
-    if (a < nt && b < nt) sp[k] = tbd[k];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // SMA-TB pass 1 (smatb_preinteraction, TB.F90:367-455):
 //   rho_i = sum_j [ r < rci ? xi^2 exp(-2q(r/r0-1)) : (b1 x^5 + b2 x^4 + b3 x^3)^2 ]
@@ -372,14 +297,14 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_rho64(
     load_params(sp, tbd, nt);
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
   const int ti = SINGLE ? 0 : type[i];
   const int m = nn[i];
   double acc = 0.0;
   for (int k = 0; k < m; ++k) {
-    const unsigned e = list[(size_t)k * ld + i];
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
     const int j = (int)(e & GH_JMASK);
     const int w = (int)(e >> 27);
     const double4 pj = pos[j];
@@ -427,7 +352,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force64(
     load_params(sp, tbd, nt);
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
   const int ti = SINGLE ? 0 : type[i];
@@ -435,7 +360,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force64(
   const int m = nn[i];
   double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
   for (int k = 0; k < m; ++k) {
-    const unsigned e = list[(size_t)k * ld + i];
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
     const int j = (int)(e & GH_JMASK);
     const int w = (int)(e >> 27);
     const double4 pj = pos[j];
@@ -491,14 +416,14 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_rho_mx(
     load_params(sp, tbd, nt);
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
   const int ti = SINGLE ? 0 : type[i];
   const int m = nn[i];
   double acc = 0.0;
   for (int k = 0; k < m; ++k) {
-    const unsigned e = list[(size_t)k * ld + i];
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
     const int j = (int)(e & GH_JMASK);
     const int w = (int)(e >> 27);
     const double4 pj = pos[j];
@@ -508,10 +433,11 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_rho_mx(
     const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
     const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
     if (r2 > P.rce2) continue;
-    const float r = __fsqrt_rn((float)r2);
+    const float r2f = (float)r2;
+    const float r = r2f * __builtin_amdgcn_rsqf(r2f);
     float phi;
     if (r < P.rci_f) {
-      phi = exp2f(fmaf(P.rho_b, r, P.rho_a));
+      phi = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * (r - P.r0f));
     } else {
       const float d = r - P.rce_f;
       const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
@@ -541,7 +467,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
     load_params(sp, tbd, nt);
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
   const int ti = SINGLE ? 0 : type[i];
@@ -549,7 +475,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
   const int m = nn[i];
   double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
   for (int k = 0; k < m; ++k) {
-    const unsigned e = list[(size_t)k * ld + i];
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
     const int j = (int)(e & GH_JMASK);
     const int w = (int)(e >> 27);
     const double4 pj = pos[j];
@@ -560,13 +486,14 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
     const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
     if (r2 > P.rce2) continue;
     const float r2f = (float)r2;
-    const float r = __fsqrt_rn(r2f);
-    const float rinv = 1.0f / r;
+    const float rinv = __builtin_amdgcn_rsqf(r2f);
+    const float r = r2f * rinv;
     const float ebs = ebi + eband32[j];
     float factor, repul;
     if (r <= P.rci_f) {
-      repul = exp2f(fmaf(P.rep_b, r, P.rep_a));
-      const float band = exp2f(fmaf(P.rho_b, r, P.rho_a));
+      const float s0 = r - P.r0f;
+      repul = P.rep_a * __builtin_amdgcn_exp2f(P.rep_b * s0);
+      const float band = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * s0);
       factor = fmaf(repul, P.frep, -band * P.fband * ebs);
     } else {
       const float d = r - P.rce_f;
@@ -601,13 +528,13 @@ __global__ __launch_bounds__(GH_BLOCK) void k_lj64(
   __shared__ ShiftLds sh;
   load_shift(sh, L0, L1, L2);
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
   const int m = nn[i];
   double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
   for (int k = 0; k < m; ++k) {
-    const unsigned e = list[(size_t)k * ld + i];
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
     const int j = (int)(e & GH_JMASK);
     const int w = (int)(e >> 27);
     const double4 pj = pos[j];
@@ -640,13 +567,13 @@ __global__ __launch_bounds__(GH_BLOCK) void k_lj_mx(
   __shared__ ShiftLds sh;
   load_shift(sh, L0, L1, L2);
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
   const int m = nn[i];
   double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
   for (int k = 0; k < m; ++k) {
-    const unsigned e = list[(size_t)k * ld + i];
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
     const int j = (int)(e & GH_JMASK);
     const int w = (int)(e >> 27);
     const double4 pj = pos[j];
@@ -655,7 +582,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_lj_mx(
     const double vz = (pj.z - pi.z) + sh.z[w];
     const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
     if (r2 > rc2) continue;
-    const float ir2 = 1.0f / (float)r2;
+    const float ir2 = __builtin_amdgcn_rcpf((float)r2);
     const float s2 = sigma2 * ir2;
     const float s6 = s2 * s2 * s2;
     const float p = 4.0f * eps * (s6 * s6 - s6);
@@ -685,32 +612,29 @@ struct WrapArgs {
   double L[3];
 };
 
-// do_pbc_1 (Set_Properties.f90:444-470): one box length at most.
-__device__ __forceinline__ double wrap1(double x, int pbc, double L) {
-  if (!pbc) return x;
-  if (x >= L) return x - L;
-  if (x < 0.0) return x + L;
-  return x;
-}
-
-__device__ __forceinline__ double min_image(double d, int pbc, double L) {
-  if (!pbc) return d;
-  if (d > 0.5 * L) return d - L;
-  if (d < -0.5 * L) return d + L;
-  return d;
-}
-
-__device__ __forceinline__ void disp_reduce(double d2, GhStatus *st) {
-  double wm = wave_max(d2);
-  if ((threadIdx.x & 63) == 0 && wm > 0.0)
-    atomicMax(&st->maxdisp2_bits, (unsigned long long)__double_as_longlong(wm));
+// Positions are not wrapped between neighbour rebuilds: the list stores each
+// pair's periodic image, which stays valid while atoms move continuously.
+// They are wrapped into the box at every rebuild (k_wrap_rebuild) and when
+// copied out, which reproduces do_pbc_1 (Set_Properties.f90:444-470).
+// Per-workgroup maximum of |x - x_old|^2, one value per workgroup (the
+// rebuild decision reduces them; no single-address atomics).
+__device__ __forceinline__ void disp_reduce(double d2, double *__restrict__ dispblk) {
+  __shared__ double part[GH_BLOCK / 64];
+  d2 = wave_max(d2);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GH_BLOCK / 64; ++w) d2 = fmax(d2, part[w]);
+    dispblk[blockIdx.x] = d2;
+  }
 }
 
 // velocity_verlet_a (Integration.F90:1150-1169) + do_pbc + displacement.
 __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     int n, int ncap, double dt, MassArgs ma, WrapArgs wa, const int *__restrict__ type,
     double4 *__restrict__ pos, double *__restrict__ vel, double *__restrict__ acc,
-    const double *__restrict__ frc, const double *__restrict__ xold, GhStatus *st) {
+    const double *__restrict__ frc, const double *__restrict__ xold,
+    double *__restrict__ dispblk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -725,8 +649,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
       x[d] = x[d] + dt * v + acel_dx * a;
       vel[d * ncap + i] = v + acel_dv * a;
       acc[d * ncap + i] = a;
-      x[d] = wrap1(x[d], wa.pbc[d], wa.L[d]);
-      const double dd = min_image(x[d] - xold[d * ncap + i], wa.pbc[d], wa.L[d]);
+      const double dd = x[d] - xold[d * ncap + i];
       d2 += dd * dd;
     }
     p.x = x[0];
@@ -734,7 +657,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, st);
+  disp_reduce(d2, dispblk);
 }
 
 // velocity_verlet_b (Integration.F90:1171-1186).
@@ -752,49 +675,6 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_b(int n, int ncap, double dt, M
     const double a = frc[d * ncap + i] * om;
     acc[d * ncap + i] = a;
     vel[d * ncap + i] = vel[d * ncap + i] + acel_dv * a;
-  }
-}
-
-// Philox4x32-10 (Salmon et al., SC'11): counter-based stream.
-__device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
-    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
-    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
-    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
-    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-    k.x += 0x9E3779B9u;
-    k.y += 0xBB67AE85u;
-  }
-  return c;
-}
-
-__device__ __forceinline__ double u01(unsigned a, unsigned b) {
-  // 53-bit uniform in (0,1)
-  const unsigned long long v = ((unsigned long long)a << 21) ^ (unsigned long long)b;
-  return ((double)(v & ((1ull << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-// Six standard normals for atom `id` at step `step` (Box-Muller).
-__device__ __forceinline__ void philox_normals(unsigned long long seed, long long step, int id,
-                                               double g[6]) {
-  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  double u[6];
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const uint4 c = philox(make_uint4((unsigned)id, (unsigned)b, (unsigned)step,
-                                      (unsigned)((unsigned long long)step >> 32)), key);
-    u[2 * b] = u01(c.x, c.y);
-    u[2 * b + 1] = u01(c.z, c.w);
-  }
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const double r = sqrt(-2.0 * log(u[2 * b]));
-    double sn, cs;
-    sincospi(2.0 * u[2 * b + 1], &sn, &cs);
-    g[2 * b] = r * cs;
-    g[2 * b + 1] = r * sn;
   }
 }
 
@@ -821,7 +701,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     int n, int ncap, ErmakArgs k, MassArgs ma, WrapArgs wa, const int *__restrict__ type,
     const int *__restrict__ orig, const double *__restrict__ gauss, double4 *__restrict__ pos,
     const double *__restrict__ vel, const double *__restrict__ acc,
-    const double *__restrict__ xold, GhStatus *st) {
+    const double *__restrict__ xold, double *__restrict__ dispblk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -834,8 +714,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     for (int d = 0; d < 3; ++d) {
       const double pos_v = f_lan * k.sdr * g[2 * d];
       x[d] = x[d] + k.c1 * vel[d * ncap + i] + k.c2 * k.dt * acc[d * ncap + i] + pos_v;
-      x[d] = wrap1(x[d], wa.pbc[d], wa.L[d]);
-      const double dd = min_image(x[d] - xold[d * ncap + i], wa.pbc[d], wa.L[d]);
+      const double dd = x[d] - xold[d * ncap + i];
       d2 += dd * dd;
     }
     p.x = x[0];
@@ -843,7 +722,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, st);
+  disp_reduce(d2, dispblk);
 }
 
 // ermak_b (Integration.F90:1013-1031); vel_v is re-derived from the same
@@ -940,6 +819,25 @@ __global__ void k_wrap(int n, WrapArgs wa, double4 *__restrict__ pos) {
   pos[i] = p;
 }
 
+__global__ void k_wrap_rebuild(const GhStatus *st, int n, WrapArgs wa,
+                               double4 *__restrict__ pos) {
+  if (!st->rebuild) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = pos[i];
+  double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (wa.pbc[d]) {
+      x[d] = x[d] - wa.L[d] * floor(x[d] / wa.L[d]);
+      if (x[d] >= wa.L[d]) x[d] -= wa.L[d];
+    }
+  p.x = x[0];
+  p.y = x[1];
+  p.z = x[2];
+  pos[i] = p;
+}
+
 inline int nblocks(int n) { return (n + GH_BLOCK - 1) / GH_BLOCK; }
 
 GridArgs make_grid(const gh_ctx *c) {
@@ -1000,6 +898,8 @@ ErmakArgs make_ermak(const gh_ctx *c) {
 // Neighbour rebuild.  force: request a rebuild regardless of displacement.
 // Every kernel of the rebuild reads the device flag set by k_decide, so the
 // sequence can be enqueued every step without a host round trip.
+using namespace ghd;
+
 int gh_launch_rebuild(gh_ctx *c, bool force) {
   hipStream_t s = c->stream;
   if (force) {
@@ -1010,29 +910,32 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
     GH_HIP(c, hipMemcpyAsync(c->status, &h, sizeof(GhStatus), hipMemcpyHostToDevice, s));
     GH_HIP(c, hipStreamSynchronize(s));
   }
-  k_decide<<<1, 1, 0, s>>>(c->status, c->skin);
+  // dispblk holds the last integrator step's per-workgroup maxima (zero after
+  // a rebuild or at setup)
+  k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
   const int n = c->n, nb = nblocks(n);
+  if (c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
+    k_wrap_rebuild<<<nb, GH_BLOCK, 0, s>>>(c->status, n, make_wrap(c), c->pos);
   const GridArgs g = make_grid(c);
   const double rl = c->rcut + c->skin;
   const double rl2 = rl * rl;
   const int ld = c->ncap;
   if (c->list_mode == 1) {
-    k_zero_cells<<<nblocks(c->ncell), GH_BLOCK, 0, s>>>(c->status, c->cell_count, c->ncell);
-    k_bin<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, c->cellid, c->cell_count);
-    k_scan<<<1, 1024, 0, s>>>(c->status, c->cell_count, c->cell_start, c->cell_cursor,
-                              c->ncell);
+    const int nkey = c->tg.nkey;
+    k_zero_cells<<<nblocks(nkey), GH_BLOCK, 0, s>>>(c->status, c->cell_count, nkey);
+    k_bin<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, c->tg, c->cellid, c->cell_count);
+    k_scan<<<1, 1024, 0, s>>>(c->status, c->cell_count, c->cell_start, c->cell_cursor, nkey);
     k_scatter<<<nb, GH_BLOCK, 0, s>>>(c->status, c->cellid, n, c->cell_cursor, c->perm);
-    k_sort_cells<<<nblocks(c->ncell), GH_BLOCK, 0, s>>>(c->status, c->cell_start, c->orig,
-                                                        c->perm, c->ncell);
+    k_sort_cells<<<nblocks(nkey), GH_BLOCK, 0, s>>>(c->status, c->cell_start, c->orig, c->perm,
+                                                    nkey);
     k_gather<<<nb, GH_BLOCK, 0, s>>>(c->status, c->perm, n, c->ncap, c->cellid, c->pos,
                                      c->vel, c->acc, c->type, c->orig, c->pos_t, c->vel_t,
                                      c->acc_t, c->type_t, c->orig_t, c->cellof);
     k_copyback<<<nb, GH_BLOCK, 0, s>>>(c->status, n, c->ncap, c->pos_t, c->vel_t, c->acc_t,
                                        c->type_t, c->orig_t, c->pos, c->vel, c->acc, c->type,
                                        c->orig);
-    k_build_cells<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, c->cellof, c->cell_start, n, g,
-                                          rl2, c->cap, ld, 0, c->nn, (unsigned *)c->list,
-                                          c->xold, c->ncap);
+    GH_HIP(c, hipGetLastError());
+    return gh_launch_build_tiled(c, c->cap, 0);
   } else {
     k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
                                        (unsigned *)c->list, c->xold, c->ncap);
@@ -1041,29 +944,8 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   return 0;
 }
 
-// Count-only pass used once to size the list rows (host-synchronous).
-int gh_count_neighbours(gh_ctx *c, int *maxnn) {
-  hipStream_t s = c->stream;
-  const int n = c->n, nb = nblocks(n);
-  const GridArgs g = make_grid(c);
-  const double rl = c->rcut + c->skin;
-  const double rl2 = rl * rl;
-  if (c->list_mode == 1)
-    k_build_cells<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, c->cellof, c->cell_start, n, g,
-                                          rl2, 0, c->ncap, 1, c->nn, (unsigned *)c->list,
-                                          c->xold, c->ncap);
-  else
-    k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, 0, c->ncap, 1, c->nn,
-                                       (unsigned *)c->list, c->xold, c->ncap);
-  GH_HIP(c, hipGetLastError());
-  GhStatus h;
-  GH_HIP(c, hipMemcpyAsync(&h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, s));
-  GH_HIP(c, hipStreamSynchronize(s));
-  *maxnn = h.maxnn;
-  return 0;
-}
-
 int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
+  if (c->list_mode == 1) return gh_launch_forces_tiled(c, mid);
   hipStream_t s = c->stream;
   const int n = c->n, nb = nblocks(n);
   const int ld = c->ncap;
@@ -1075,41 +957,41 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
   if (c->pot == GH_POT_SMATB) {
     if (mixed) {
       if (single)
-        k_tb_rho_mx<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        k_tb_rho_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                   c->nt, L0, L1, L2, c->rho, c->eband,
                                                   c->eband32);
       else
-        k_tb_rho_mx<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        k_tb_rho_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                    c->nt, L0, L1, L2, c->rho, c->eband,
                                                    c->eband32);
     } else {
       if (single)
-        k_tb_rho64<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        k_tb_rho64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                  c->nt, L0, L1, L2, c->rho, c->eband,
                                                  c->eband32);
       else
-        k_tb_rho64<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        k_tb_rho64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                   c->nt, L0, L1, L2, c->rho, c->eband,
                                                   c->eband32);
     }
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     if (mixed) {
       if (single)
-        k_tb_force_mx<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        k_tb_force_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                     c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
                                                     c->eband32, ev_ui, c->frc, c->epa);
       else
-        k_tb_force_mx<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        k_tb_force_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                      c->ncap, c->tbd, c->nt, L0, L1, L2,
                                                      c->rho, c->eband32, ev_ui, c->frc,
                                                      c->epa);
     } else {
       if (single)
-        k_tb_force64<true><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        k_tb_force64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                    c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
                                                    c->eband, ev_ui, c->frc, c->epa);
       else
-        k_tb_force64<false><<<nb, GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        k_tb_force64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                     c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
                                                     c->eband, ev_ui, c->frc, c->epa);
     }
@@ -1117,11 +999,11 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
     if (mixed)
-      k_lj_mx<<<nb, GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, (float)c->lj_eps,
+      k_lj_mx<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, (float)c->lj_eps,
                                       (float)(c->lj_sigma * c->lj_sigma), rc2, L0, L1, L2,
                                       ev_ui, c->frc, c->epa);
     else
-      k_lj64<<<nb, GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
+      k_lj64<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
                                      c->lj_sigma, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");
@@ -1133,7 +1015,7 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
 int gh_launch_vv_a(gh_ctx *c) {
   k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c),
                                                     make_wrap(c), c->type, c->pos, c->vel,
-                                                    c->acc, c->frc, c->xold, c->status);
+                                                    c->acc, c->frc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1148,7 +1030,7 @@ int gh_launch_vv_b(gh_ctx *c) {
 int gh_launch_ermak_a(gh_ctx *c) {
   k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
       c->n, c->ncap, make_ermak(c), make_mass(c), make_wrap(c), c->type, c->orig, c->gauss,
-      c->pos, c->vel, c->acc, c->xold, c->status);
+      c->pos, c->vel, c->acc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
   return 0;
 }

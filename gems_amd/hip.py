@@ -23,6 +23,10 @@ SYMBOLS = [
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
     "gh_get_stats", "gh_set_timing", "gh_get_timing",
+    "gh_batch_create", "gh_batch_destroy", "gh_batch_last_error", "gh_batch_set_precision",
+    "gh_batch_set_systems", "gh_batch_set_smatb", "gh_batch_set_masses", "gh_batch_set_skin",
+    "gh_batch_set_velocities", "gh_batch_set_temperatures", "gh_batch_compute",
+    "gh_batch_langevin", "gh_batch_exchange", "gh_batch_fire", "gh_batch_get_state",
 ]
 
 
@@ -78,6 +82,24 @@ def load():
     L.gh_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.gh_set_timing.argtypes = [vp, C.c_int]
     L.gh_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_int64)]
+    L.gh_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+    L.gh_batch_destroy.argtypes = [vp]
+    L.gh_batch_destroy.restype = None
+    L.gh_batch_last_error.argtypes = [vp]
+    L.gh_batch_last_error.restype = C.c_char_p
+    L.gh_batch_set_precision.argtypes = [vp, C.c_int]
+    L.gh_batch_set_systems.argtypes = [vp, C.c_int, _ip, _dp, _ip]
+    L.gh_batch_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
+    L.gh_batch_set_masses.argtypes = [vp, C.c_int, _dp]
+    L.gh_batch_set_skin.argtypes = [vp, C.c_double]
+    L.gh_batch_set_velocities.argtypes = [vp, _dp]
+    L.gh_batch_set_temperatures.argtypes = [vp, _dp]
+    L.gh_batch_compute.argtypes = [vp, _dp, _dp]
+    L.gh_batch_langevin.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_uint64, _dp, _dp]
+    L.gh_batch_exchange.argtypes = [vp, C.c_int, C.c_int, _ip, C.c_int, C.c_uint64, _ip]
+    L.gh_batch_fire.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                _ip, _ip, _dp, _dp]
+    L.gh_batch_get_state.argtypes = [vp, _dp, _dp]
     _lib = L
     return L
 
@@ -216,3 +238,107 @@ class Context:
         self._ck(self.L.gh_get_timing(self.h, _p(ms), cnt))
         names = ["rho", "force", "integrate", "rebuild"]
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+
+
+class Batch:
+    """Many independent free clusters on one device (a gh_batch)."""
+
+    def __init__(self, device=0, precision=GH_FP64):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.gh_batch_create(C.byref(h), device, precision)
+        if rc != 0:
+            raise GhError(rc, "gh_batch_create failed (no HIP device %d?)" % device)
+        self.h = h
+        self.nsys = 0
+        self.ntot = 0
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise GhError(rc, self.L.gh_batch_last_error(self.h).decode())
+
+    def close(self):
+        if self.h is not None:
+            self.L.gh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_precision(self, p):
+        self._ck(self.L.gh_batch_set_precision(self.h, p))
+
+    def set_systems(self, off, x, types=None):
+        off = np.ascontiguousarray(off, np.int32)
+        x = np.ascontiguousarray(x, np.float64)
+        self.nsys = len(off) - 1
+        self.ntot = int(off[-1])
+        self.off = off
+        t = None if types is None else np.ascontiguousarray(types, np.int32)
+        self._ck(self.L.gh_batch_set_systems(self.h, self.nsys, _p(off, _ip), _p(x),
+                                             _p(t, _ip)))
+
+    def set_smatb(self, nt, A, xi, p, q, r0, rci, rce):
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (nt, nt)))
+                for a in (A, xi, p, q, r0, rci, rce)]
+        self._ck(self.L.gh_batch_set_smatb(self.h, nt, *[_p(a) for a in arrs]))
+
+    def set_masses(self, masses):
+        m = np.ascontiguousarray(np.atleast_1d(masses), np.float64)
+        self._ck(self.L.gh_batch_set_masses(self.h, len(m), _p(m)))
+
+    def set_skin(self, skin):
+        self._ck(self.L.gh_batch_set_skin(self.h, skin))
+
+    def set_velocities(self, v=None):
+        vv = None if v is None else np.ascontiguousarray(v, np.float64)
+        self._ck(self.L.gh_batch_set_velocities(self.h, _p(vv)))
+
+    def set_temperatures(self, T):
+        T = np.ascontiguousarray(T, np.float64)
+        self._ck(self.L.gh_batch_set_temperatures(self.h, _p(T)))
+
+    def compute(self):
+        F = np.zeros((self.ntot, 3))
+        ep = np.zeros(self.nsys)
+        self._ck(self.L.gh_batch_compute(self.h, _p(F), _p(ep)))
+        return F, ep
+
+    def langevin(self, nsteps, dt, gamma, seed):
+        ep = np.zeros(self.nsys)
+        ek = np.zeros(self.nsys)
+        self._ck(self.L.gh_batch_langevin(self.h, nsteps, dt, gamma, seed, _p(ep), _p(ek)))
+        return ep, ek
+
+    def exchange(self, ladders, parity, seed):
+        """ladders: (nladder, R) int array of system ids in rung order (updated)."""
+        lad = np.ascontiguousarray(ladders, np.int32)
+        nacc = C.c_int(0)
+        self._ck(self.L.gh_batch_exchange(self.h, lad.shape[0], lad.shape[1], _p(lad, _ip),
+                                          parity, seed, C.byref(nacc)))
+        ladders[...] = lad
+        return nacc.value
+
+    def fire(self, max_iter=20000, dt_max=0.005, alpha0=0.1, fswitch=1e-2, ftol=1e-4):
+        it = np.zeros(self.nsys, np.int32)
+        cv = np.zeros(self.nsys, np.int32)
+        ep = np.zeros(self.nsys)
+        fm = np.zeros(self.nsys)
+        self._ck(self.L.gh_batch_fire(self.h, max_iter, dt_max, alpha0, fswitch, ftol,
+                                      _p(it, _ip), _p(cv, _ip), _p(ep), _p(fm)))
+        return dict(iters=it, converged=cv, epot=ep, fmax=fm)
+
+    def state(self):
+        x = np.zeros((self.ntot, 3))
+        v = np.zeros((self.ntot, 3))
+        self._ck(self.L.gh_batch_get_state(self.h, _p(x), _p(v)))
+        return x, v

@@ -154,6 +154,61 @@ int gh_get_stats(gh_ctx *ctx, gh_stats *s);
 int gh_set_timing(gh_ctx *ctx, int enable);
 int gh_get_timing(gh_ctx *ctx, double ms[4], int64_t count[4]);
 
+/* ------------------------------------------------------------------------
+ * Batched free clusters.  GEMS runs one system per process
+ * (src/Program_Types.F90:89-102) and replica exchange as one replica per MPI
+ * rank (src/ReplicaExchange.f90:27-151); a batch holds many independent
+ * unboxed systems on one device and advances all of them in one launch, one
+ * workgroup per system (largest first).  Neighbour lists are the O(N^2)
+ * verlet_self lists of each system (Neighbour.F90:676-759); forces are the
+ * SMA-TB passes of TB.F90:367-586; Langevin is ermak_a/ermak_b
+ * (Integration.F90:963-1031) with the Philox stream.
+ * ------------------------------------------------------------------------ */
+typedef struct gh_batch gh_batch;
+int gh_batch_create(gh_batch **b, int device, int precision);
+void gh_batch_destroy(gh_batch *b);
+const char *gh_batch_last_error(const gh_batch *b);
+int gh_batch_set_precision(gh_batch *b, int precision);
+/* nsys systems; atoms of system s are off[s] .. off[s+1]-1 of x (3 per atom,
+ * A) and type (internal types 0..nt-1).  Velocities are set to zero. */
+int gh_batch_set_systems(gh_batch *b, int nsys, const int *off, const double *x,
+                         const int *type);
+int gh_batch_set_smatb(gh_batch *b, int nt, const double *A, const double *xi,
+                       const double *p, const double *q, const double *r0,
+                       const double *rci, const double *rce);
+int gh_batch_set_masses(gh_batch *b, int nt, const double *mass);
+int gh_batch_set_skin(gh_batch *b, double skin);
+int gh_batch_set_velocities(gh_batch *b, const double *v);
+/* Per-system temperatures (K) used by gh_batch_langevin. */
+int gh_batch_set_temperatures(gh_batch *b, const double *T);
+/* Forces (ui/A, 3 per atom) and per-system potential energies (eV) of the
+ * current positions: one interact() per system.  Either pointer may be NULL. */
+int gh_batch_compute(gh_batch *b, double *F, double *epot);
+/* nsteps Ermak steps (dt ps, gamma 1/ps) of every system in one launch.
+ * epot, ekin (eV, per system, after the last step; ekin about the CM) may be
+ * NULL. */
+int gh_batch_langevin(gh_batch *b, int nsteps, double dt, double gamma, uint64_t seed,
+                      double *epot, double *ekin);
+/* One replica-exchange sweep over ladders of systems (the pairing and
+ * Metropolis rule of src/ReplicaExchange.f90:63-138): rung r of ladder l is
+ * system sys[l*R + r] at temperature T[sys]; pairs (r, r+1) with
+ * r % 2 == parity exchange temperatures with probability min(1, exp(-du)),
+ * du = (E_upper - E_lower)(beta_lower - beta_upper), energies from the last
+ * gh_batch_langevin, and the velocities of both are rescaled by
+ * sqrt(T_new / T_old).  The rung order is updated in sys.  Returns the number
+ * of accepted swaps in *naccept. */
+int gh_batch_exchange(gh_batch *b, int nladder, int R, int *sys, int parity, uint64_t seed,
+                      int *naccept);
+/* FIRE relaxation (Bitzek et al., PRL 97, 170201 (2006)) of every system, one
+ * workgroup per system: dt starts at dt_max/10, N_min = 5, f_inc = 1.1,
+ * f_dec = 0.5, f_alpha = 0.99.  With precision GH_MIXED the pair terms are
+ * FP32 until max|F| < fswitch (eV/A), then FP64 to max|F| < ftol.  Outputs
+ * per system: iterations, converged flag, final epot (eV), final max|F|. */
+int gh_batch_fire(gh_batch *b, int max_iter, double dt_max, double alpha0, double fswitch,
+                  double ftol, int *iters, int *converged, double *epot, double *fmax);
+/* Copy back positions and velocities (3 per atom). */
+int gh_batch_get_state(gh_batch *b, double *x, double *v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -716,3 +716,88 @@ int or_md_run(or_md *md) {
   free(one_sqrt_mass);
   return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* FIRE (Bitzek et al. 2006), see gems_oracle.h                          */
+/* ------------------------------------------------------------------ */
+static double fire_forces(const double *x, const int *type, int n, const or_box *b,
+                          const or_smatb *s, double rl, int *maxnb, int **nn, int **list,
+                          double *f, double *e) {
+  for (;;) {
+    int need = or_nb_build(x, n, b, rl, *maxnb, *nn, *list, NULL, NULL, 1);
+    if (need == 0) break;
+    *maxnb = need + 16;
+    free(*list);
+    *list = (int *)malloc(sizeof(int) * (size_t)n * *maxnb);
+  }
+  double ep = 0.0;
+  or_smatb_force(x, type, n, b, s, *nn, *list, *maxnb, f, e, NULL, NULL, &ep, 1);
+  return ep;
+}
+
+int or_fire(double *x, const int *type, int n, const or_smatb *s, const double *mass,
+            double skin, int max_iter, double dt_max, double alpha0, double ftol,
+            double *epot, double *fmax, int *converged) {
+  double ui_ev = c_ui_ev();
+  or_box b;
+  or_box_set(&b, 0, NULL, NULL);
+  double rl = or_smatb_rcut(s) + skin;
+  int maxnb = 128;
+  int *nn = (int *)malloc(sizeof(int) * n);
+  int *list = (int *)malloc(sizeof(int) * (size_t)n * maxnb);
+  double *f = (double *)malloc(sizeof(double) * 3 * n);
+  double *e = (double *)malloc(sizeof(double) * n);
+  double *v = (double *)calloc(3 * (size_t)n, sizeof(double));
+  double ep = fire_forces(x, type, n, &b, s, rl, &maxnb, &nn, &list, f, e);
+  double dt = dt_max * 0.1, alpha = alpha0, fm = 0.0;
+  int npos = 0, it, conv = 0;
+  for (it = 0; it < max_iter; ++it) {
+    fm = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double q = f[3 * i] * f[3 * i] + f[3 * i + 1] * f[3 * i + 1] + f[3 * i + 2] * f[3 * i + 2];
+      if (q > fm) fm = q;
+    }
+    fm = sqrt(fm) * ui_ev;
+    if (fm < ftol) {
+      conv = 1;
+      break;
+    }
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        v[3 * i + d] += 0.5 * dt * f[3 * i + d] / mass[i];
+        x[3 * i + d] += dt * v[3 * i + d];
+      }
+    ep = fire_forces(x, type, n, &b, s, rl, &maxnb, &nn, &list, f, e);
+    double P = 0.0, vv = 0.0, ff = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        v[3 * i + d] += 0.5 * dt * f[3 * i + d] / mass[i];
+        P += f[3 * i + d] * v[3 * i + d];
+        vv += v[3 * i + d] * v[3 * i + d];
+        ff += f[3 * i + d] * f[3 * i + d];
+      }
+    double mixv = ff > 0.0 ? alpha * sqrt(vv) / sqrt(ff) : 0.0;
+    int up = P > 0.0;
+    for (int k = 0; k < 3 * n; ++k) v[k] = up ? (1.0 - alpha) * v[k] + mixv * f[k] : 0.0;
+    if (up) {
+      ++npos;
+      if (npos > 5) {
+        dt = fmin(dt * 1.1, dt_max);
+        alpha *= 0.99;
+      }
+    } else {
+      npos = 0;
+      dt *= 0.5;
+      alpha = alpha0;
+    }
+  }
+  *epot = ep * ui_ev;
+  *fmax = fm;
+  *converged = conv;
+  free(nn);
+  free(list);
+  free(f);
+  free(e);
+  free(v);
+  return it;
+}

@@ -158,6 +158,17 @@ int or_md_run(or_md *md);
 /* One interact() call: test_update/update + forces.  Returns like above. */
 int or_md_interact(or_md *md);
 
+/* ---- FIRE minimisation (Bitzek et al., PRL 97, 170201 (2006)) ----
+ * Not in the reference build (src/FIRE.f90 does not compile); restated from
+ * the paper for the batched relaxations: velocity-Verlet step, then
+ * P = F.v, v <- (1-a) v + a |v| F/|F|; P > 0 for more than 5 steps:
+ * dt <- min(1.1 dt, dt_max), a <- 0.99 a; P <= 0: dt <- dt/2, v <- 0,
+ * a <- alpha0.  Start: v = 0, dt = dt_max/10.  Free system, SMA-TB, FP64.
+ * Returns iterations; x is updated; *fmax in eV/A, *epot in eV. */
+int or_fire(double *x, const int *type, int n, const or_smatb *s, const double *mass,
+            double skin, int max_iter, double dt_max, double alpha0, double ftol,
+            double *epot, double *fmax, int *converged);
+
 #ifdef __cplusplus
 }
 #endif

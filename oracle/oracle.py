@@ -111,6 +111,9 @@ def _setup(L):
     L.or_md_run.argtypes = [C.POINTER(MD)]
     L.or_md_interact.restype = C.c_int
     L.or_md_interact.argtypes = [C.POINTER(MD)]
+    L.or_fire.restype = C.c_int
+    L.or_fire.argtypes = [_dp, _ip, C.c_int, C.POINTER(Smatb), _dp, C.c_double, C.c_int,
+                          C.c_double, C.c_double, C.c_double, _dp, _dp, _ip]
 
 
 def _p(a, t=_dp):
@@ -336,3 +339,14 @@ class MDRun:
     @property
     def nupd(self):
         return self.md.nupd
+
+
+def fire(x, types, s, masses, skin=1.0, max_iter=20000, dt_max=0.005, alpha0=0.1, ftol=1e-4):
+    """FP64 FIRE relaxation of one free system (see gems_oracle.h)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    ep, fm, cv = C.c_double(), C.c_double(), C.c_int()
+    it = lib().or_fire(_p(x), _p(t, _ip), len(t), C.byref(s), _p(m), skin, max_iter, dt_max,
+                       alpha0, ftol, C.byref(ep), C.byref(fm), C.byref(cv))
+    return dict(x=x, iters=it, epot=ep.value, fmax=fm.value, converged=bool(cv.value))

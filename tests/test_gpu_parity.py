@@ -265,7 +265,7 @@ def test_full_size_mixed_vs_fp64_and_momentum():
     d = x - c0
     d -= L * np.round(d / L)
     near = np.where(np.all(np.abs(d) < 14.0, axis=1))[0]
-    nn, lst, _ = O.neighbours_brute(x[near], box, O.smatb_rcut(s) + 1.0, strict=True)
+    nn, lst = O.neighbours_brute(x[near], box, O.smatb_rcut(s) + 1.0, strict=True)
     Fo, eo, _, _ = O.smatb_force(x[near], np.zeros(len(near), np.int32), box, s, nn, lst)
     core = np.where(np.all(np.abs(d[near]) < 2.0, axis=1))[0]  # complete shells only
     assert len(core) > 0
