@@ -100,6 +100,71 @@ def cpu_baseline(x, L, skin, budget_s=20.0):
                        "%d rebuilds)" % (steps, len(x), cores, md.nupd - 1))
 
 
+def batch_re(hip, rounds, swap_every=500):
+    """Replica-exchange Langevin of 512 Ag-Cu clusters (64 geometries x 8
+    temperatures, 300..900 K geometric), friction 1/ps, dt 2 fs, swaps every
+    `swap_every` steps; one persistent launch per swap interval."""
+    from gems_amd import clusters as CL
+    off, x, t, T, lad = CL.agcu_re_batch()
+    b = hip.Batch(0, hip.GH_MIXED)
+    b.set_systems(off, x, t)
+    b.set_smatb(2, *CL.agcu_params())
+    b.set_masses([CL.MASS_AG, CL.MASS_CU])
+    b.set_temperatures(T)
+    b.langevin(swap_every, 0.002, 1.0, 1)  # warm-up interval (untimed)
+    lad = lad.copy()
+    acc = tries = 0
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        b.langevin(swap_every, 0.002, 1.0, 2 + r)
+        acc += b.exchange(lad, r & 1, 1000 + r)
+        tries += lad.shape[0] * ((lad.shape[1] - (r & 1)) // 2)
+    dt = time.perf_counter() - t0
+    b.close()
+    nsys, ntot = len(off) - 1, int(off[-1])
+    steps = rounds * swap_every
+    return {"systems": nsys, "atoms": ntot, "steps": steps, "seconds": dt,
+            "cluster_steps_per_s": nsys * steps / dt, "atom_steps_per_s": ntot * steps / dt,
+            "swap_acceptance": acc / max(tries, 1),
+            "workload": "512 Ag-Cu clusters (38..2000 atoms log-uniform, fcc spheres, 50:50), "
+                        "8-rung ladder 300..900 K, Langevin (Ermak) gamma 1/ps dt 2 fs, "
+                        "exchange every %d steps, mixed precision" % swap_every}
+
+
+def batch_fire(hip, nclusters=4096):
+    """FIRE relaxation of 4096 Au clusters (13..309 atoms): FP32 pair terms
+    until max|F| < 1e-2 eV/A, FP64 polish to 1e-4 eV/A; the same batch relaxed
+    in plain FP64 gives the final-energy error."""
+    from gems_amd import clusters as CL
+    off, x = CL.au_fire_batch(nclusters)
+    out = {}
+    for prec in (hip.GH_MIXED, hip.GH_FP64):
+        b = hip.Batch(0, prec)
+        b.set_systems(off, x)
+        b.set_smatb(1, AU["A"], AU["xi"], AU["p"], AU["q"], AU["r0"], A_LAT, A_LAT * np.sqrt(1.5))
+        b.set_masses([MASS_AU])
+        t0 = time.perf_counter()
+        r = b.fire(max_iter=20000, dt_max=0.005, alpha0=0.1, fswitch=1e-2, ftol=1e-4)
+        out[prec] = (time.perf_counter() - t0, r)
+        b.close()
+    dt, r = out[hip.GH_MIXED]
+    dt64, r64 = out[hip.GH_FP64]
+    n = np.diff(off)
+    same = np.abs(r["epot"] - r64["epot"]) < 1e-6 * n
+    return {"clusters": nclusters, "atoms": int(off[-1]), "seconds": dt,
+            "clusters_relaxed_per_s": nclusters / dt,
+            "converged": float(np.mean(r["converged"])),
+            "iters_mean": float(np.mean(r["iters"])), "iters_max": int(np.max(r["iters"])),
+            "fp64_seconds": dt64, "fp64_clusters_relaxed_per_s": nclusters / dt64,
+            "same_minimum_as_fp64": float(np.mean(same)),
+            "final_energy_err_ev_per_atom_same_minimum":
+                float(np.max(np.abs(r["epot"] - r64["epot"])[same] / n[same])) if same.any()
+                else None,
+            "workload": "4096 Au clusters, sizes uniform 13..309, half fcc spheres / half "
+                        "Mackay fragments, 0.1 A Gaussian perturbation; FIRE dt_max 5 fs, "
+                        "alpha0 0.1, FP32 pair terms to 1e-2 eV/A then FP64 to 1e-4 eV/A"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +175,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--re-rounds", type=int, default=4)
+    ap.add_argument("--fire-clusters", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -193,6 +261,9 @@ def main():
         "rebuilds": nupd_mx,
         "step_breakdown_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in tm_mx.items()},
     }
+    if not args.no_batch:
+        out["batched_re"] = batch_re(hip, args.re_rounds)
+        out["batched_fire"] = batch_fire(hip, args.fire_clusters)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, L, args.skin, args.cpu_budget)
     print(json.dumps(out))

@@ -81,7 +81,7 @@ def agcu_re_batch(n_geom=64, rungs=8, nmin=38, nmax=2000, t_lo=300.0, t_hi=900.0
     rng = np.random.default_rng(seed)
     a = AGCU["r0"] * np.sqrt(2.0)
     sizes = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_geom))).astype(int)
-    T = t_lo * (t_hi / t_lo) ** (np.arange(rungs) / (rungs - 1))
+    T = t_lo * (t_hi / t_lo) ** (np.arange(rungs) / max(rungs - 1, 1))
     xs, ts, off, temps, lad = [], [], [0], [], []
     for g, n in enumerate(sizes):
         c = rng.uniform(-0.5, 0.5, 3) * a

@@ -286,5 +286,43 @@ def main():
         json.dump(meta, f, indent=1, default=float)
 
 
+def au_bulk(nx, seed=20260101):
+    """Au fcc nx^3 cells, a = 4.078 A, N(0, 0.02 A) jitter from `seed`, wrapped."""
+    a = 4.078
+    x = fcc(nx, a) + np.random.default_rng(seed).normal(0.0, 0.02, (4 * nx ** 3, 3))
+    L = nx * a
+    return np.mod(x, L), L
+
+
+def drift_main(nsteps=10000, each=100):
+    """NVE energy trace of the reference over `nsteps` at 32,000 Au atoms
+    (20^3 cells, 300 K from lcg seed 4242, dt 2 fs, skin 1 A).  Stored without
+    positions: au_bulk(20) regenerates them."""
+    x, L = au_bulk(20)
+    a = 4.078
+    wd = tempfile.mkdtemp(prefix="gold_")
+    try:
+        write_xyz(os.path.join(wd, "x.xyz"), ["Au"] * len(x), x)
+        write_prm(os.path.join(wd, "p.prm"), [("Au", "Au", AU, a, a * np.sqrt(1.5))])
+        s = head(4242) + "out float 25.16\n>< read x.xyz\nsys add\n"
+        s += "box size %s %s %s\n" % ((fmt(L),) * 3)
+        s += "> sys\nset pbc T T T\n> sys\ngroup 1 add\ninteract 1 under tb read p.prm\n"
+        s += "time step 0.002\n> sys\nset tempgdist 300.\n"
+        s += "outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each %d\n" % each
+        s += "> sys\nevolve v_verlet\ndinamica %d\n" % nsteps
+        t0 = __import__("time").time()
+        log, nupd = run(wd, s)
+        E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3)
+        np.savez_compressed(os.path.join(HERE, "au32k_nve_drift.npz"), energy=E, nupd=nupd,
+                            nsteps=nsteps, each=each, seed=4242, nx=20, jitter_seed=20260101)
+        print("au32k_nve_drift lines=%d nupd=%d E0=%.16e E1=%.16e (%.0f s)"
+              % (len(E), nupd, E[0, 2], E[-1, 2], __import__("time").time() - t0))
+    finally:
+        shutil.rmtree(wd)
+
+
 if __name__ == "__main__":
-    main()
+    if "--drift" in sys.argv:
+        drift_main()
+    else:
+        main()
