@@ -192,6 +192,7 @@ def main():
 
     # mixed precision (the headline)
     c = make_ctx(hip, hip.GH_MIXED, x, L, args.skin)
+    tile_mx = c.autotune(reps=3)
     F_mx, _, et_mx = c.compute()
     e0 = c.energies()
     dt_mx, tm_mx, nupd_mx = timed_run(c, args.steps, args.warmup)
@@ -201,6 +202,7 @@ def main():
     # plain FP64 GPU form, same input
     k64 = args.fp64_steps or max(10, args.steps // 2)
     c = make_ctx(hip, hip.GH_FP64, x, L, args.skin)
+    tile_64 = c.autotune(reps=3)
     F_64, _, et_64 = c.compute()
     dt_64, tm_64, nupd_64 = timed_run(c, k64, max(2, args.warmup // 2))
     c.close()
@@ -238,7 +240,7 @@ def main():
             "precision": "mixed",
         },
         "roofline": {
-            "kernel": "k_tb_force_tiled<mixed> (SMA-TB pass 2)",
+            "kernel": "k_tb_force_fx (SMA-TB pass 2, mixed, fixed-point LDS tiles)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -248,7 +250,7 @@ def main():
             "bytes_per_atom": BYTES_PASS2,
             "avg_launch_us": t_force * 1e6,
         },
-        "pass1": {"kernel": "k_tb_rho_tiled<mixed>", "avg_launch_us": t_rho * 1e6,
+        "pass1": {"kernel": "k_tb_rho_fx", "avg_launch_us": t_rho * 1e6,
                   "achieved_GBs": BYTES_PASS1 * n / t_rho / 1e9},
         "fp64": {"value": value64, "steps": k64, "ms_per_step": dt_64 / k64 * 1e3,
                  "force_pass_us": tm_64["force"][0] / tm_64["force"][1] * 1e3,
@@ -259,6 +261,8 @@ def main():
         "epot_rel_err_vs_fp64": abs(et_mx / et_64 - 1.0),
         "energy_change_ev_per_atom": (e1[0] + e1[1] - e0[0] - e0[1]) / n,
         "rebuilds": nupd_mx,
+        "tiles": {"mixed": {"brick_cells": tile_mx[0], "force_eval_us": tile_mx[1]},
+                  "fp64": {"brick_cells": tile_64[0], "force_eval_us": tile_64[1]}},
         "step_breakdown_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in tm_mx.items()},
     }
     if not args.no_batch:

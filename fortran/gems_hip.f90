@@ -99,6 +99,21 @@ module gems_hip
       real(c_double), value :: skin
     end function
 
+    ! cell-tile shape of the device passes (no reference counterpart)
+    integer(c_int) function gh_set_tile(ctx, bd) bind(C, name='gh_set_tile')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: bd(3)
+    end function
+
+    integer(c_int) function gh_autotune(ctx, reps, bd_out, us_out) bind(C, name='gh_autotune')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      integer(c_int), intent(out) :: bd_out(3)
+      real(c_double), intent(out) :: us_out
+    end function
+
     integer(c_int) function gh_compute(ctx, F, e_atom, e_tot) bind(C, name='gh_compute')
       import :: c_ptr, c_double, c_int
       type(c_ptr), value :: ctx

@@ -163,8 +163,32 @@ int gh_upload_tb(gh_ctx *c) {
       d.r2f = (float)t.r2;
       d.r3f = (float)t.r3;
     }
+  TbF f[GH_MAXT * GH_MAXT];
+  memset(f, 0, sizeof(f));
+  for (int k = 0; k < GH_MAXT * GH_MAXT; ++k) {
+    const TbDev &d = h[k];
+    TbF &o = f[k];
+    o.rce2q = d.rce2 * GH_FXS2;
+    o.r0f = d.r0f;
+    o.rep_a = d.rep_a;
+    o.rep_b = d.rep_b;
+    o.rho_a = d.rho_a;
+    o.rho_b = d.rho_b;
+    o.frep = d.frep;
+    o.fband = d.fband;
+    o.rci_f = d.rci_f;
+    o.rce_f = d.rce_f;
+    o.b1f = d.b1f;
+    o.b2f = d.b2f;
+    o.b3f = d.b3f;
+    o.r1f = d.r1f;
+    o.r2f = d.r2f;
+    o.r3f = d.r3f;
+  }
   GH_TRY(dalloc(c, &c->tbd, GH_MAXT * GH_MAXT));
   GH_HIP(c, hipMemcpy(c->tbd, h, sizeof(h), hipMemcpyHostToDevice));
+  GH_TRY(dalloc(c, &c->tbf, GH_MAXT * GH_MAXT));
+  GH_HIP(c, hipMemcpy(c->tbf, f, sizeof(f), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -179,12 +203,15 @@ int gh_upload_tb(gh_ctx *c) {
 // a cell length are still binned exactly.
 static const int GH_LDS_BYTES = 65536;
 
-// LDS bytes a workgroup of the tiled force pass needs per halo slot, and the
-// static LDS it uses besides (halo tables, parameter table).
-static int slot_bytes(const gh_ctx *c) { return 24 + 8 + (c->nt > 1 ? 4 : 0); }
-static int static_lds(const gh_ctx *c) { return 8192 + (c->nt > 1 ? 12800 : 256); }
+// Halo slots a workgroup of the tiled passes can stage: the LDS left after
+// the static tables, over the larger per-slot need of the two passes.
+size_t gh_tile_slot_bytes(int pass, int precision, int nt);
+size_t gh_tile_static_bytes(int precision, int nt);
 static int max_slots(const gh_ctx *c) {
-  return (GH_LDS_BYTES - static_lds(c)) / slot_bytes(c);
+  const size_t per = std::max(gh_tile_slot_bytes(0, c->precision, c->nt),
+                              gh_tile_slot_bytes(1, c->precision, c->nt));
+  const size_t stat = gh_tile_static_bytes(c->precision, c->nt);
+  return (int)std::min<size_t>((GH_LDS_BYTES - stat) / per, 65535);
 }
 
 // Brick edges (cells) for the tiled passes: the largest brick whose expected
@@ -296,7 +323,8 @@ static int ensure_ready(gh_ctx *c) {
     // until the halo fits the LDS budget
     for (int attempt = 0;; ++attempt) {
       GH_TRY(alloc_cells(c));
-      c->tg.slot_cap = (GH_LDS_BYTES - 8192) / 24;  // count pass: positions only
+      GH_TRY(gh_fill_int(c, c->cell_count, 0, c->tg.nkey));
+      c->tg.slot_cap = 65535;  // count pass: any halo (the build reads L2 past its LDS)
       c->cap = 0;
       GH_TRY(gh_launch_rebuild(c, true));
       int max_halo = 0, max_brick = 0;
@@ -470,7 +498,7 @@ void gh_destroy(gh_ctx *c) {
   void *dev[] = {c->pos, c->pos_t, c->vel, c->vel_t, c->acc, c->acc_t, c->frc, c->epa,
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
-                 c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->status,
+                 c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
                  c->red, c->dispblk, c->list16};
   for (void *p : dev)
     if (p) (void)hipFree(p);
@@ -483,10 +511,85 @@ void gh_destroy(gh_ctx *c) {
 
 const char *gh_last_error(const gh_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+int gh_set_tile(gh_ctx *c, const int *bd) {
+  if (!c) return GH_EINVAL;
+  for (int d = 0; d < 3; ++d)
+    if (bd && (bd[d] < 0 || bd[d] > 4)) return gh_fail(c, GH_EINVAL, "brick edges must be 0..4");
+  for (int d = 0; d < 3; ++d) c->bd_req[d] = bd ? bd[d] : 0;
+  c->ready = false;
+  return 0;
+}
+
+int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
+  if (!c || reps <= 0) return GH_EINVAL;
+  GH_HIP(c, hipSetDevice(c->device));
+  GH_TRY(ensure_ready(c));
+  if (c->list_mode != 1) {
+    if (bd_out) bd_out[0] = bd_out[1] = bd_out[2] = 0;
+    if (us_out) *us_out = 0.0;
+    return 0;
+  }
+  GhStatus h0;
+  GH_TRY(read_status(c, &h0));  // the trial rebuilds are not counted in nupd
+  double cells = 1.0;
+  for (int d = 0; d < 3; ++d) cells *= c->nc[d];
+  const double apc = c->n / cells;
+  hipEvent_t e0, e1;
+  GH_HIP(c, hipEventCreate(&e0));
+  GH_HIP(c, hipEventCreate(&e1));
+  int best[3] = {0, 0, 0};
+  double best_us = 1e30;
+  for (int bz = 1; bz <= 4; ++bz)
+    for (int by = bz; by <= 4; ++by)
+      for (int bx = by; bx <= 4; ++bx) {
+        const int b[3] = {bx, by, bz};
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) ok = ok && b[d] <= c->gn[d];
+        const double halo = apc * (bx + 2) * (by + 2) * (bz + 2);
+        if (!ok || halo * 1.15 > max_slots(c)) continue;
+        for (int d = 0; d < 3; ++d) c->bd_req[d] = b[d];
+        c->ready = false;
+        if (ensure_ready(c) != 0) continue;
+        bool same = true;  // the sizing may have shrunk the request
+        for (int d = 0; d < 3; ++d) same = same && c->tg.bd[d] == b[d];
+        if (!same) continue;
+        GH_TRY(gh_launch_forces(c, nullptr));
+        GH_HIP(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) GH_TRY(gh_launch_forces(c, nullptr));
+        GH_HIP(c, hipEventRecord(e1, c->stream));
+        GH_HIP(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        if (check_status(c) != 0) continue;
+        const double us = 1e3 * ms / reps;
+        if (us < best_us) {
+          best_us = us;
+          for (int d = 0; d < 3; ++d) best[d] = b[d];
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int d = 0; d < 3; ++d) c->bd_req[d] = best[d];
+  c->ready = false;
+  c->err.clear();
+  GH_TRY(ensure_ready(c));
+  GH_TRY(gh_launch_forces(c, nullptr));  // forces of the kept shape
+  GH_TRY(check_status(c));
+  GhStatus h1;
+  GH_TRY(read_status(c, &h1));
+  h1.nupd = h0.nupd;
+  GH_TRY(write_status(c, &h1));
+  if (bd_out)
+    for (int d = 0; d < 3; ++d) bd_out[d] = c->tg.bd[d];
+  if (us_out) *us_out = best_us;
+  return 0;
+}
+
 int gh_set_precision(gh_ctx *c, int precision) {
   if (!c) return GH_EINVAL;
   if (precision != GH_FP64 && precision != GH_MIXED)
     return gh_fail(c, GH_EINVAL, "precision must be GH_FP64 or GH_MIXED");
+  if (precision != c->precision) c->ready = false;  // LDS budgets depend on it
   c->precision = precision;
   return 0;
 }
@@ -887,17 +990,24 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
   int iout = 0;
   const bool ermak = c->integ == GH_INT_ERMAK;
   const bool lcg = ermak && c->rng_mode == GH_RNG_LCG;
+  // velocity Verlet: stepb of step n and stepa of step n+1 run as one kernel
+  // unless the state after stepb is observed (energy output, end of the run)
+  bool pending_b = false;
   for (int ns = 1; ns <= nsteps; ++ns) {
     c->step += 1;
     if (lcg) GH_TRY(lcg_step_normals(c));
     GH_TRY(rec(c, 0));
-    GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
+    if (pending_b) GH_TRY(gh_launch_vv_ba(c));
+    else GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
+    pending_b = false;
     GH_TRY(rec(c, 1));
-    GH_TRY(gh_launch_rebuild(c, false));
+    GH_TRY(gh_launch_rebuild(c, false, true));
     GH_TRY(rec(c, 2));
     GH_TRY(gh_launch_forces(c, ev_at(c, 3)));
     GH_TRY(rec(c, 4));
-    GH_TRY(ermak ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
+    const bool observed = ns == nsteps || (each > 0 && ns % each == 0);
+    if (ermak || observed) GH_TRY(ermak ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
+    else pending_b = true;
     GH_TRY(rec(c, 5));
     if (c->timing) {
       c->evused += 1;

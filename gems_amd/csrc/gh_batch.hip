@@ -682,6 +682,8 @@ int gh_batch_set_smatb(gh_batch *b, int nt, const double *A, const double *xi, c
   if (b->tbd) BH(b, hipFree(b->tbd));
   b->tbd = tmp.tbd;  // ownership moves to the batch
   tmp.tbd = nullptr;
+  if (tmp.tbf) BH(b, hipFree(tmp.tbf));  // the batch kernels use the FP64/FP32 block only
+  tmp.tbf = nullptr;
   b->nt = nt;
   b->rcut = tmp.rcut;
   b->have_pot = true;

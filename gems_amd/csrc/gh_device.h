@@ -60,7 +60,9 @@ __device__ __forceinline__ double tb_band64(const TbDev &P, double dr) {
   return b * b;
 }
 
-__device__ __forceinline__ float tb_band32(const TbDev &P, float r) {
+// FP32 forms take TbDev or TbF (same FP32 field names).
+template <class TP>
+__device__ __forceinline__ float tb_band32(const TP &P, float r) {
   if (r < P.rci_f) return P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * (r - P.r0f));
   const float d = r - P.rce_f;
   const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
@@ -84,7 +86,8 @@ __device__ __forceinline__ void tb_pair64(const TbDev &P, double dr, double ebs,
   }
 }
 
-__device__ __forceinline__ void tb_pair32(const TbDev &P, float r, float ebs, float &factor,
+template <class TP>
+__device__ __forceinline__ void tb_pair32(const TP &P, float r, float ebs, float &factor,
                                           float &repul) {
   if (r <= P.rci_f) {
     const float s0 = r - P.r0f;

@@ -63,6 +63,16 @@ struct TbDev {
   float b1f, b2f, b3f, r1f, r2f, r3f;
 };
 
+// Compact FP32 form for the mixed-precision tiled passes (positions in LDS
+// as fixed point, 2^23 steps per Angstrom; rce2q = rce^2 in those units).
+struct TbF {
+  double rce2q;
+  float r0f, rep_a, rep_b, rho_a, rho_b, frep, fband, rci_f, rce_f;
+  float b1f, b2f, b3f, r1f, r2f, r3f, pad;
+};
+#define GH_FXS 8388608.0         // fixed-point steps per Angstrom (2^23)
+#define GH_FXS2 70368744177664.0 // GH_FXS^2 (2^46)
+
 // Cell grid and its grouping into bricks.  Atoms are sorted by key =
 // brick * cpb + cell-within-brick, so each brick's atoms are contiguous and
 // one workgroup stages the brick's halo (the bricks' cells grown by one cell)
@@ -80,6 +90,7 @@ struct TileGeom {
   double one_cell[3];
 };
 #define GH_MAXHALO 216  // (4 + 2)^3 halo cells at most (brick edge <= 4)
+#define GH_BUILD_LDS 61440  // dynamic LDS the list build may stage positions in
 
 struct GhStatus {                     // device-resident counters
   unsigned long long maxdisp2_bits;   // max |dx|^2 since the last build
@@ -92,7 +103,7 @@ struct GhStatus {                     // device-resident counters
   int tile_overflow;                  // a brick's halo exceeded slot_cap (sticky)
   int max_halo;                       // largest halo (atoms) at the last build
   int max_brick;                      // largest brick (atoms) at the last build
-  int pad;
+  unsigned ticket;                    // workgroups of an integrator kernel done
 };
 
 struct gh_ctx {
@@ -164,6 +175,7 @@ struct gh_ctx {
   int *list = nullptr;                        // [cap][ncap] ELL (O(N^2) mode)
   unsigned short *list16 = nullptr;           // [cap][ncap] ELL of LDS slots (cell mode)
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
+  TbF *tbf = nullptr;                         // [MAXT*MAXT]
   GhStatus *status = nullptr;
   double *dispblk = nullptr;                  // per-workgroup max displacement^2
   int ndispblk = 0;
@@ -195,10 +207,11 @@ int gh_fail(gh_ctx *c, int code, const std::string &msg);
   } while (0)
 
 // launchers (gh_kernels.hip)
-int gh_launch_rebuild(gh_ctx *c, bool force);
+int gh_launch_rebuild(gh_ctx *c, bool force, bool decided = false);
 int gh_launch_forces(gh_ctx *c, hipEvent_t mid);
 int gh_launch_vv_a(gh_ctx *c);
 int gh_launch_vv_b(gh_ctx *c);
+int gh_launch_vv_ba(gh_ctx *c);
 int gh_launch_ermak_a(gh_ctx *c);
 int gh_launch_ermak_b(gh_ctx *c);
 int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui);

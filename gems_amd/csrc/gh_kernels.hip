@@ -83,11 +83,6 @@ struct GridArgs {
 };
 
 
-__global__ void k_zero_cells(const GhStatus *st, int *__restrict__ count, int ncell) {
-  if (!st->rebuild) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncell) count[c] = 0;
-}
 
 __device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, int *outside) {
   int c = (int)floor(x * t.one_cell[d]);
@@ -102,23 +97,41 @@ __device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, in
   return c;
 }
 
-// Sort key of each atom: brick * cpb + cell within the brick.
-__global__ void k_bin(GhStatus *st, const double4 *__restrict__ pos, int n, TileGeom t,
+// Wraps periodic coordinates into the box (do_pbc_1, Set_Properties.f90:444-470)
+// and computes each atom's sort key: brick * cpb + cell within the brick.
+__global__ void k_bin(GhStatus *st, double4 *__restrict__ pos, int n, TileGeom t,
                       int *__restrict__ cellid, int *__restrict__ count) {
   if (!st->rebuild) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double4 p = pos[i];
-  int out = 0;
-  const int cx = tile_coord(p.x, t, 0, &out);
-  const int cy = tile_coord(p.y, t, 1, &out);
-  const int cz = tile_coord(p.z, t, 2, &out);
-  if (out) st->outside = 1;
-  const int bx = cx / t.bd[0], by = cy / t.bd[1], bz = cz / t.bd[2];
-  const int lx = cx - bx * t.bd[0], ly = cy - by * t.bd[1], lz = cz - bz * t.bd[2];
-  const int key = ((bz * t.nbr[1] + by) * t.nbr[0] + bx) * t.cpb + (lz * t.bd[1] + ly) * t.bd[0] + lx;
-  cellid[i] = key;
-  atomicAdd(&count[key], 1);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+    bool moved = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (t.pbc[d]) {
+        const double w0 = x[d];
+        x[d] = x[d] - t.L[d] * floor(x[d] / t.L[d]);
+        if (x[d] >= t.L[d]) x[d] -= t.L[d];
+        moved |= x[d] != w0;
+      }
+    if (moved) {
+      p.x = x[0];
+      p.y = x[1];
+      p.z = x[2];
+      pos[i] = p;
+    }
+    int out = 0;
+    const int cx = tile_coord(x[0], t, 0, &out);
+    const int cy = tile_coord(x[1], t, 1, &out);
+    const int cz = tile_coord(x[2], t, 2, &out);
+    if (out) st->outside = 1;
+    const int bx = cx / t.bd[0], by = cy / t.bd[1], bz = cz / t.bd[2];
+    const int lx = cx - bx * t.bd[0], ly = cy - by * t.bd[1], lz = cz - bz * t.bd[2];
+    const int key =
+        ((bz * t.nbr[1] + by) * t.nbr[0] + bx) * t.cpb + (lz * t.bd[1] + ly) * t.bd[0] + lx;
+    cellid[i] = key;
+    atomicAdd(&count[key], 1);
+  }
 }
 
 // Exclusive scan of the cell counts in one 1024-thread workgroup.
@@ -152,30 +165,32 @@ __global__ __launch_bounds__(1024) void k_scan(const GhStatus *st, const int *__
 __global__ void k_scatter(const GhStatus *st, const int *__restrict__ cellid, int n,
                           int *__restrict__ cursor, int *__restrict__ perm) {
   if (!st->rebuild) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int slot = atomicAdd(&cursor[cellid[i]], 1);
-  perm[slot] = i;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int slot = atomicAdd(&cursor[cellid[i]], 1);
+    perm[slot] = i;
+  }
 }
 
 // Order the atoms of each cell by their caller index: the sorted layout is
 // then independent of the atomic scatter order (deterministic results).
+// The cell counts are consumed by then: they are zeroed for the next build.
 __global__ void k_sort_cells(const GhStatus *st, const int *__restrict__ start,
                              const int *__restrict__ orig, int *__restrict__ perm,
-                             int ncell) {
+                             int *__restrict__ count, int ncell) {
   if (!st->rebuild) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int b = start[c], e = start[c + 1];
-  for (int k = b + 1; k < e; ++k) {
-    int v = perm[k];
-    int key = orig[v];
-    int m = k - 1;
-    while (m >= b && orig[perm[m]] > key) {
-      perm[m + 1] = perm[m];
-      --m;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+    const int b = start[c], e = start[c + 1];
+    for (int k = b + 1; k < e; ++k) {
+      const int v = perm[k];
+      const int key = orig[v];
+      int m = k - 1;
+      while (m >= b && orig[perm[m]] > key) {
+        perm[m + 1] = perm[m];
+        --m;
+      }
+      perm[m + 1] = v;
     }
-    perm[m + 1] = v;
+    count[c] = 0;
   }
 }
 
@@ -187,18 +202,18 @@ __global__ void k_gather(const GhStatus *st, const int *__restrict__ perm, int n
                          double *__restrict__ acc_t, int *__restrict__ type_t,
                          int *__restrict__ orig_t, int *__restrict__ cellof) {
   if (!st->rebuild) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int o = perm[s];
-  pos_t[s] = pos[o];
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int o = perm[s];
+    pos_t[s] = pos[o];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    vel_t[d * ncap + s] = vel[d * ncap + o];
-    acc_t[d * ncap + s] = acc[d * ncap + o];
+    for (int d = 0; d < 3; ++d) {
+      vel_t[d * ncap + s] = vel[d * ncap + o];
+      acc_t[d * ncap + s] = acc[d * ncap + o];
+    }
+    type_t[s] = type[o];
+    orig_t[s] = orig[o];
+    cellof[s] = cellid[o];
   }
-  type_t[s] = type[o];
-  orig_t[s] = orig[o];
-  cellof[s] = cellid[o];
 }
 
 __global__ void k_copyback(const GhStatus *st, int n, int ncap, const double4 *__restrict__ pos_t,
@@ -208,16 +223,16 @@ __global__ void k_copyback(const GhStatus *st, int n, int ncap, const double4 *_
                            double *__restrict__ acc, int *__restrict__ type,
                            int *__restrict__ orig) {
   if (!st->rebuild) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  pos[s] = pos_t[s];
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    pos[s] = pos_t[s];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    vel[d * ncap + s] = vel_t[d * ncap + s];
-    acc[d * ncap + s] = acc_t[d * ncap + s];
+    for (int d = 0; d < 3; ++d) {
+      vel[d * ncap + s] = vel_t[d * ncap + s];
+      acc[d * ncap + s] = acc_t[d * ncap + s];
+    }
+    type[s] = type_t[s];
+    orig[s] = orig_t[s];
   }
-  type[s] = type_t[s];
-  orig[s] = orig_t[s];
 }
 
 // O(N^2) build (verlet_self, Neighbour.F90:676-759): rd > rl^2 -> skip, so
@@ -616,24 +631,55 @@ struct WrapArgs {
 // pair's periodic image, which stays valid while atoms move continuously.
 // They are wrapped into the box at every rebuild (k_wrap_rebuild) and when
 // copied out, which reproduces do_pbc_1 (Set_Properties.f90:444-470).
-// Per-workgroup maximum of |x - x_old|^2, one value per workgroup (the
-// rebuild decision reduces them; no single-address atomics).
-__device__ __forceinline__ void disp_reduce(double d2, double *__restrict__ dispblk) {
+// Rebuild decision at the end of an integrator kernel.  Each workgroup
+// stores its maximum |x - x_old|^2; the last workgroup to finish (ticket
+// counter) reduces them and sets the flag the rebuild kernels read, as
+// k_decide does when no integrator step precedes a force evaluation.
+__device__ __forceinline__ void disp_decide(double d2, double *__restrict__ dispblk,
+                                            GhStatus *st, double skin) {
   __shared__ double part[GH_BLOCK / 64];
+  __shared__ int last;
   d2 = wave_max(d2);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < GH_BLOCK / 64; ++w) d2 = fmax(d2, part[w]);
-    dispblk[blockIdx.x] = d2;
+    __hip_atomic_store(&dispblk[blockIdx.x], d2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned t =
+        __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  double m = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    m = fmax(m, __hip_atomic_load(&dispblk[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    dispblk[b] = 0.0;  // consumed, as in k_decide
+  }
+  m = wave_max(m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GH_BLOCK / 64; ++w) m = fmax(m, part[w]);
+    const double dd = fmax(m, __longlong_as_double((long long)st->maxdisp2_bits));
+    const int rb = st->force_rebuild || (2.0 * sqrt(dd) > skin);
+    st->rebuild = rb;
+    if (rb) {
+      st->nupd += 1;
+      st->force_rebuild = 0;
+      st->maxnn = 0;
+    }
+    st->maxdisp2_bits = 0ull;
+    st->ticket = 0u;
   }
 }
 
 // velocity_verlet_a (Integration.F90:1150-1169) + do_pbc + displacement.
 __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
-    int n, int ncap, double dt, MassArgs ma, WrapArgs wa, const int *__restrict__ type,
-    double4 *__restrict__ pos, double *__restrict__ vel, double *__restrict__ acc,
-    const double *__restrict__ frc, const double *__restrict__ xold,
+    GhStatus *st, double skin, int n, int ncap, double dt, MassArgs ma,
+    const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
+    double *__restrict__ acc, const double *__restrict__ frc, const double *__restrict__ xold,
     double *__restrict__ dispblk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
@@ -657,7 +703,40 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, dispblk);
+  disp_decide(d2, dispblk, st, skin);
+}
+
+// velocity_verlet_b of one step fused with velocity_verlet_a of the next
+// (Integration.F90:1171-1186 then 1150-1169): the same operations in the same
+// order as the two kernels, with a = F/m formed once.  The acceleration is
+// not stored (stepa recomputes it; the last stepb of a run stores it).
+__global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
+    GhStatus *st, double skin, int n, int ncap, double dt, MassArgs ma,
+    const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
+    const double *__restrict__ frc, const double *__restrict__ xold,
+    double *__restrict__ dispblk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    const double acel_dv = dt * 0.5, acel_dx = acel_dv * dt;
+    const double om = ma.om[type[i]];
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double a = frc[d * ncap + i] * om;
+      const double v = vel[d * ncap + i] + acel_dv * a;  // stepb
+      x[d] = x[d] + dt * v + acel_dx * a;                // stepa
+      vel[d * ncap + i] = v + acel_dv * a;
+      const double dd = x[d] - xold[d * ncap + i];
+      d2 += dd * dd;
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    pos[i] = p;
+  }
+  disp_decide(d2, dispblk, st, skin);
 }
 
 // velocity_verlet_b (Integration.F90:1171-1186).
@@ -698,7 +777,8 @@ __device__ __forceinline__ void ermak_gauss(const ErmakArgs &k, int id,
 
 // ermak_a (Integration.F90:963-1011) + do_pbc + displacement.
 __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
-    int n, int ncap, ErmakArgs k, MassArgs ma, WrapArgs wa, const int *__restrict__ type,
+    GhStatus *st, double skin, int n, int ncap, ErmakArgs k, MassArgs ma,
+    const int *__restrict__ type,
     const int *__restrict__ orig, const double *__restrict__ gauss, double4 *__restrict__ pos,
     const double *__restrict__ vel, const double *__restrict__ acc,
     const double *__restrict__ xold, double *__restrict__ dispblk) {
@@ -722,7 +802,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, dispblk);
+  disp_decide(d2, dispblk, st, skin);
 }
 
 // ermak_b (Integration.F90:1013-1031); vel_v is re-derived from the same
@@ -839,6 +919,7 @@ __global__ void k_wrap_rebuild(const GhStatus *st, int n, WrapArgs wa,
 }
 
 inline int nblocks(int n) { return (n + GH_BLOCK - 1) / GH_BLOCK; }
+#define GH_STRIDE_GRID 2048
 
 GridArgs make_grid(const gh_ctx *c) {
   GridArgs g;
@@ -900,7 +981,7 @@ ErmakArgs make_ermak(const gh_ctx *c) {
 // sequence can be enqueued every step without a host round trip.
 using namespace ghd;
 
-int gh_launch_rebuild(gh_ctx *c, bool force) {
+int gh_launch_rebuild(gh_ctx *c, bool force, bool decided) {
   hipStream_t s = c->stream;
   if (force) {
     GhStatus h;
@@ -912,9 +993,10 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   }
   // dispblk holds the last integrator step's per-workgroup maxima (zero after
   // a rebuild or at setup)
-  k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
+  if (force || !decided) k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
   const int n = c->n, nb = nblocks(n);
-  if (c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
+  const int gs = std::min(nb, GH_STRIDE_GRID);  // grid-stride: cheap when no rebuild is due
+  if (c->list_mode == 0 && c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
     k_wrap_rebuild<<<nb, GH_BLOCK, 0, s>>>(c->status, n, make_wrap(c), c->pos);
   const GridArgs g = make_grid(c);
   const double rl = c->rcut + c->skin;
@@ -922,16 +1004,17 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   const int ld = c->ncap;
   if (c->list_mode == 1) {
     const int nkey = c->tg.nkey;
-    k_zero_cells<<<nblocks(nkey), GH_BLOCK, 0, s>>>(c->status, c->cell_count, nkey);
-    k_bin<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, c->tg, c->cellid, c->cell_count);
+    // cell_count is all zero here (zeroed at sizing and by k_sort_cells)
+    const int gk = std::min(nblocks(nkey), GH_STRIDE_GRID);
+    k_bin<<<gs, GH_BLOCK, 0, s>>>(c->status, c->pos, n, c->tg, c->cellid, c->cell_count);
     k_scan<<<1, 1024, 0, s>>>(c->status, c->cell_count, c->cell_start, c->cell_cursor, nkey);
-    k_scatter<<<nb, GH_BLOCK, 0, s>>>(c->status, c->cellid, n, c->cell_cursor, c->perm);
-    k_sort_cells<<<nblocks(nkey), GH_BLOCK, 0, s>>>(c->status, c->cell_start, c->orig, c->perm,
-                                                    nkey);
-    k_gather<<<nb, GH_BLOCK, 0, s>>>(c->status, c->perm, n, c->ncap, c->cellid, c->pos,
+    k_scatter<<<gs, GH_BLOCK, 0, s>>>(c->status, c->cellid, n, c->cell_cursor, c->perm);
+    k_sort_cells<<<gk, GH_BLOCK, 0, s>>>(c->status, c->cell_start, c->orig, c->perm,
+                                         c->cell_count, nkey);
+    k_gather<<<gs, GH_BLOCK, 0, s>>>(c->status, c->perm, n, c->ncap, c->cellid, c->pos,
                                      c->vel, c->acc, c->type, c->orig, c->pos_t, c->vel_t,
                                      c->acc_t, c->type_t, c->orig_t, c->cellof);
-    k_copyback<<<nb, GH_BLOCK, 0, s>>>(c->status, n, c->ncap, c->pos_t, c->vel_t, c->acc_t,
+    k_copyback<<<gs, GH_BLOCK, 0, s>>>(c->status, n, c->ncap, c->pos_t, c->vel_t, c->acc_t,
                                        c->type_t, c->orig_t, c->pos, c->vel, c->acc, c->type,
                                        c->orig);
     GH_HIP(c, hipGetLastError());
@@ -1013,9 +1096,17 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
 }
 
 int gh_launch_vv_a(gh_ctx *c) {
-  k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c),
-                                                    make_wrap(c), c->type, c->pos, c->vel,
+  k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->status, c->skin, c->n, c->ncap, c->dt,
+                                                    make_mass(c), c->type, c->pos, c->vel,
                                                     c->acc, c->frc, c->xold, c->dispblk);
+  GH_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int gh_launch_vv_ba(gh_ctx *c) {
+  k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->status, c->skin, c->n, c->ncap, c->dt,
+                                                     make_mass(c), c->type, c->pos, c->vel,
+                                                     c->frc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1029,8 +1120,8 @@ int gh_launch_vv_b(gh_ctx *c) {
 
 int gh_launch_ermak_a(gh_ctx *c) {
   k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
-      c->n, c->ncap, make_ermak(c), make_mass(c), make_wrap(c), c->type, c->orig, c->gauss,
-      c->pos, c->vel, c->acc, c->xold, c->dispblk);
+      c->status, c->skin, c->n, c->ncap, make_ermak(c), make_mass(c), c->type, c->orig,
+      c->gauss, c->pos, c->vel, c->acc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
   return 0;
 }

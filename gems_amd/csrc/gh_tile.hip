@@ -9,7 +9,18 @@
 // instead of once per neighbour pair, and list rows cost 2 B per entry.
 //
 // LDS slot layout per workgroup: the halo cells in order (hx fastest), each
-// cell's atoms in sorted order.  hbase[h] is the first slot of halo cell h.
+// cell's atoms in sorted order.  hbase[h] is the first slot of halo cell h and
+// the atom in slot s of cell h is s + delta[h].
+//
+// List layout ("quad ELL"): entry k of atom a is the 16-bit word
+// ((k / 4) * ld + a) * 4 + k % 4, so one 8-byte load per lane fetches four
+// consecutive entries and a wavefront's load covers 512 contiguous bytes.
+//
+// Mixed precision stages positions as fixed point relative to the brick
+// centre, GH_FXS = 2^23 steps per Angstrom (|x - centre| < 256 A): pair
+// vectors are exact integer differences, converted exactly to FP64, and r^2 is
+// exact in FP64 for every pair within 7 A, so the only rounding of a pair's
+// geometry is the 2^-24 quantisation of each position (6e-8 A).
 #include <math.h>
 
 #include "gh_device.h"
@@ -20,17 +31,22 @@ namespace {
 using namespace ghd;
 
 struct Halo {
-  int nh, total, a0, a1, key0, hd0, hd1;
-  int hstart[GH_MAXHALO];
-  int hbase[GH_MAXHALO + 1];
+  int nh, total, a0, a1, key0, hd0, hd1, pad;
+  double org[3];                  // brick centre: origin of the fixed-point frame
+  int delta[GH_MAXHALO];          // atom in slot s of halo cell h: s + delta[h]
+  int hbase[GH_MAXHALO + 1];      // first slot of halo cell h
   int scan[GH_TBLOCK];
-  double shx[GH_MAXHALO], shy[GH_MAXHALO], shz[GH_MAXHALO];
+  signed char shc[3][GH_MAXHALO]; // periodic image shift of halo cell h (box lengths)
 };
 
 __device__ __forceinline__ int grid_key(const TileGeom &t, int cx, int cy, int cz) {
   const int bx = cx / t.bd[0], by = cy / t.bd[1], bz = cz / t.bd[2];
   const int lx = cx - bx * t.bd[0], ly = cy - by * t.bd[1], lz = cz - bz * t.bd[2];
   return ((bz * t.nbr[1] + by) * t.nbr[0] + bx) * t.cpb + (lz * t.bd[1] + ly) * t.bd[0] + lx;
+}
+
+__device__ __forceinline__ double shift(const TileGeom &t, const Halo &H, int d, int h) {
+  return (double)H.shc[d][h] * t.L[d];  // exact: 0 or +-L
 }
 
 // Halo of brick b: atom ranges, image shifts and LDS slot bases of the
@@ -43,12 +59,12 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
   const int nh = hd0 * hd1 * hd2;
   const int key0 = b * t.cpb;
   const int h = threadIdx.x;
-  int cnt = 0;
+  int cnt = 0, start = 0;
   if (h < nh) {
     const int hx = h % hd0, hy = (h / hd0) % hd1, hz = h / (hd0 * hd1);
     int g[3] = {bx * t.bd[0] + hx - 1, by * t.bd[1] + hy - 1, bz * t.bd[2] + hz - 1};
     const int bc[3] = {bx, by, bz};
-    double sh[3] = {0.0, 0.0, 0.0};
+    int sc[3] = {0, 0, 0};
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -58,25 +74,23 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
       if (t.pbc[d]) {
         if (g[d] < 0) {
           g[d] += t.nc[d];
-          sh[d] = -t.L[d];
+          sc[d] = -1;
         } else if (g[d] >= t.nc[d]) {
           g[d] -= t.nc[d];
-          sh[d] = t.L[d];
+          sc[d] = 1;
         }
       } else if (g[d] < 0 || g[d] >= t.gn[d]) {
         ok = false;
       }
     }
-    int start = 0;
     if (ok) {
       const int key = grid_key(t, g[0], g[1], g[2]);
       start = cell_start[key];
       cnt = cell_start[key + 1] - start;
     }
-    H.hstart[h] = start;
-    H.shx[h] = sh[0];
-    H.shy[h] = sh[1];
-    H.shz[h] = sh[2];
+    H.shc[0][h] = (signed char)sc[0];
+    H.shc[1][h] = (signed char)sc[1];
+    H.shc[2][h] = (signed char)sc[2];
   }
   // exclusive scan of the halo cell counts (Hillis-Steele over the block)
   H.scan[threadIdx.x] = cnt;
@@ -87,7 +101,11 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
     H.scan[threadIdx.x] += v;
     __syncthreads();
   }
-  if (h < nh) H.hbase[h] = H.scan[h] - cnt;
+  if (h < nh) {
+    const int base = H.scan[h] - cnt;
+    H.hbase[h] = base;
+    H.delta[h] = start - base;
+  }
   if (h == 0) {
     H.hbase[nh] = H.scan[nh - 1];
     H.total = H.scan[nh - 1];
@@ -97,6 +115,10 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
     H.key0 = key0;
     H.a0 = cell_start[key0];
     H.a1 = cell_start[key0 + t.cpb];
+    const int bc[3] = {bx, by, bz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      H.org[d] = ((double)(bc[d] * t.bd[d] - t.gofs[d]) + 0.5 * t.bd[d]) * (t.L[d] / t.nc[d]);
   }
   __syncthreads();
   if (H.a1 <= H.a0) return false;
@@ -105,6 +127,14 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
     return false;
   }
   return true;
+}
+
+// Slot -> halo cell map in LDS (halo cells number < 256).
+__device__ __forceinline__ void slot_map(const Halo &H, unsigned char *scell) {
+  const int h = threadIdx.x;
+  if (h < H.nh)
+    for (int s = H.hbase[h]; s < H.hbase[h + 1]; ++s) scell[s] = (unsigned char)h;
+  __syncthreads();
 }
 
 // Halo cell holding LDS slot s (hbase non-decreasing; empty cells repeat).
@@ -118,19 +148,85 @@ __device__ __forceinline__ int halo_of_slot(const Halo &H, int s) {
   return lo;
 }
 
-__device__ __forceinline__ unsigned short ld_slot(const unsigned short *p) {
+__device__ __forceinline__ size_t qidx(int k, int a, int ld) {
+  return (((size_t)(k >> 2) * ld + a) << 2) + (k & 3);
+}
+__device__ __forceinline__ unsigned long long ld_quad(const unsigned long long *p) {
   return __builtin_nontemporal_load(p);
 }
+__device__ __forceinline__ int fxq(double d) { return __double2int_rn(d * GH_FXS); }
 
 // ---------------------------------------------------------------------------
 // Neighbour list (linkedcell_self, Neighbour.F90:991-1087): for each atom of
 // the brick, the slots of its 27 neighbour cells with rd < rl^2, the atom
 // itself excluded.  vd = (x_j - x_i) + image shift, rounded exactly as the
 // reference's vdistance + dot_product, so the sets are bit-identical.
+// Positions are staged in LDS when the halo fits GH_BUILD_LDS (STAGE), else
+// read from L2.
 // ---------------------------------------------------------------------------
+template <bool STAGE>
+__device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
+                                           const double4 *__restrict__ pos,
+                                           const int *__restrict__ cellof, const double *px,
+                                           const double *py, const double *pz, double rl2,
+                                           int cap, int ld, int ncap, int *__restrict__ nn,
+                                           unsigned short *__restrict__ list16,
+                                           double *__restrict__ xold, int &mmax) {
+  unsigned long long *lq = reinterpret_cast<unsigned long long *>(list16);
+  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+    const int lc = cellof[a] - H.key0;
+    const int lx = lc % t.bd[0], ly = (lc / t.bd[0]) % t.bd[1], lz = lc / (t.bd[0] * t.bd[1]);
+    const int ha = (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
+    const int own = a - H.delta[ha];
+    const double4 pa = pos[a];
+    const double xi = pa.x, yi = pa.y, zi = pa.z;
+    int m = 0;
+    unsigned long long w = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
+          const double sx = shift(t, H, 0, h), sy = shift(t, H, 1, h), sz = shift(t, H, 2, h);
+          const int se = H.hbase[h + 1];
+          const int dl = H.delta[h];
+          for (int s = H.hbase[h]; s < se; ++s) {
+            if (s == own) continue;
+            double xj, yj, zj;
+            if (STAGE) {
+              xj = px[s];
+              yj = py[s];
+              zj = pz[s];
+            } else {
+              const double4 pj = pos[s + dl];
+              xj = pj.x;
+              yj = pj.y;
+              zj = pj.z;
+            }
+            const double vx = (xj - xi) + sx;
+            const double vy = (yj - yi) + sy;
+            const double vz = (zj - zi) + sz;
+            if (dist2_rn(vx, vy, vz) < rl2) {
+              w |= (unsigned long long)s << (16 * (m & 3));
+              if ((m & 3) == 3) {
+                if (m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
+                w = 0;
+              }
+              ++m;
+            }
+          }
+        }
+    if ((m & 3) && m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
+    nn[a] = min(m, cap);
+    xold[a] = xi;
+    xold[ncap + a] = yi;
+    xold[2 * ncap + a] = zi;
+    mmax = max(mmax, m);
+  }
+}
+
 __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
-    const int *__restrict__ cell_start, double rl2, int cap, int ld, int ncap,
+    const int *__restrict__ cell_start, double rl2, int cap, int ld, int ncap, int lds_slots,
     int *__restrict__ nn, unsigned short *__restrict__ list16, double *__restrict__ xold) {
   if (!st->rebuild) return;
   extern __shared__ double dyn[];
@@ -138,45 +234,22 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  double *px = dyn, *py = dyn + t.slot_cap, *pz = dyn + 2 * t.slot_cap;
-  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-    const int h = halo_of_slot(H, s);
-    const double4 p = pos[H.hstart[h] + (s - H.hbase[h])];
-    px[s] = p.x;
-    py[s] = p.y;
-    pz[s] = p.z;
-  }
-  __syncthreads();
   int mmax = 0;
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
-    const int lc = cellof[a] - H.key0;
-    const int lx = lc % t.bd[0], ly = (lc / t.bd[0]) % t.bd[1], lz = lc / (t.bd[0] * t.bd[1]);
-    const int ha = (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
-    const int own = H.hbase[ha] + (a - H.hstart[ha]);
-    const double xi = px[own], yi = py[own], zi = pz[own];
-    int m = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
-          const double sx = H.shx[h], sy = H.shy[h], sz = H.shz[h];
-          const int se = H.hbase[h + 1];
-          for (int s = H.hbase[h]; s < se; ++s) {
-            if (s == own) continue;
-            const double vx = (px[s] - xi) + sx;
-            const double vy = (py[s] - yi) + sy;
-            const double vz = (pz[s] - zi) + sz;
-            if (dist2_rn(vx, vy, vz) < rl2) {
-              if (m < cap) list16[(size_t)m * ld + a] = (unsigned short)s;
-              ++m;
-            }
-          }
-        }
-    nn[a] = min(m, cap);
-    xold[a] = xi;
-    xold[ncap + a] = yi;
-    xold[2 * ncap + a] = zi;
-    mmax = max(mmax, m);
+  if (H.total <= lds_slots) {
+    double *px = dyn, *py = dyn + lds_slots, *pz = dyn + 2 * lds_slots;
+    unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * lds_slots);
+    slot_map(H, scell);
+    for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+      const double4 p = pos[s + H.delta[scell[s]]];
+      px[s] = p.x;
+      py[s] = p.y;
+      pz[s] = p.z;
+    }
+    __syncthreads();
+    build_rows<true>(H, t, pos, cellof, px, py, pz, rl2, cap, ld, ncap, nn, list16, xold, mmax);
+  } else {
+    build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, rl2, cap, ld, ncap, nn,
+                      list16, xold, mmax);
   }
   mmax = wave_imax(mmax);
   if ((threadIdx.x & 63) == 0 && mmax > 0) {
@@ -199,7 +272,8 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tile_occupancy(TileGeom t,
   }
 }
 
-// Slot -> sorted atom index, for reading the list back on the host.
+// Slot -> sorted atom index, for reading the list back on the host
+// (dst is a plain ELL, entry k of atom a at k * ld + a).
 __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
     TileGeom t, const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int *__restrict__ dst, GhStatus *st) {
@@ -209,34 +283,142 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x)
     for (int k = 0; k < nn[a]; ++k) {
-      const int s = list16[(size_t)k * ld + a];
-      const int h = halo_of_slot(H, s);
-      dst[(size_t)k * ld + a] = H.hstart[h] + (s - H.hbase[h]);
+      const int s = list16[qidx(k, a, ld)];
+      dst[(size_t)k * ld + a] = s + H.delta[halo_of_slot(H, s)];
     }
 }
 
-// Stages the halo's image-shifted positions (and optionally types) into LDS.
-template <bool TYPES>
-__device__ __forceinline__ void stage_pos(const Halo &H, const double4 *__restrict__ pos,
-                                          const int *__restrict__ type, double *px, double *py,
-                                          double *pz, int *ty) {
-  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-    const int h = halo_of_slot(H, s);
-    const int src = H.hstart[h] + (s - H.hbase[h]);
-    const double4 p = pos[src];
-    px[s] = p.x + H.shx[h];
-    py[s] = p.y + H.shy[h];
-    pz[s] = p.z + H.shz[h];
-    if (TYPES) ty[s] = type[src];
+// Runs f(slot) over an atom's list, one 8-byte load per four entries, the
+// next quad's load issued before the current quad is processed.
+template <class F>
+__device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__ list16, int ld,
+                                              int a, int m, F &&f) {
+  const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
+  unsigned long long w = m > 0 ? ld_quad(lp) : 0ull;
+  for (int k0 = 0; k0 < m; k0 += 4) {
+    const unsigned long long wn = k0 + 4 < m ? ld_quad(lp + (size_t)((k0 >> 2) + 1) * ld) : 0ull;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < m) f((int)((w >> (16 * u)) & 0xFFFFull));
+    w = wn;
   }
 }
 
 // ---------------------------------------------------------------------------
-// SMA-TB pass 1 (smatb_preinteraction, TB.F90:367-455), tiled.
-// MIXED: FP32 distance and exponential, FP64 pair vector and accumulation.
+// SMA-TB, mixed precision (fixed-point LDS).  Pass 1: smatb_preinteraction
+// (TB.F90:367-455); LDS slot = {qx, qy, qz, type}.
 // ---------------------------------------------------------------------------
-template <bool MIXED, bool SINGLE>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_tiled(
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
+    const int *__restrict__ cell_start, const int *__restrict__ nn,
+    const unsigned short *__restrict__ list16, int ld, const TbF *__restrict__ tbf,
+    double *__restrict__ rho, float *__restrict__ eband32) {
+  extern __shared__ int4 q4[];
+  __shared__ Halo H;
+  __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  for (int k = threadIdx.x; k < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k += blockDim.x) sp[k] = tbf[k];
+  const int b = xcd_block();
+  if (b >= t.nbrick) return;
+  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + t.slot_cap);
+  slot_map(H, scell);
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const int src = s + H.delta[h];
+    const double4 p = pos[src];
+    q4[s] = make_int4(fxq(p.x - (H.org[0] - shift(t, H, 0, h))),
+                      fxq(p.y - (H.org[1] - shift(t, H, 1, h))),
+                      fxq(p.z - (H.org[2] - shift(t, H, 2, h))), SINGLE ? 0 : type[src]);
+  }
+  __syncthreads();
+  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+    const double4 pa = pos[a];
+    const int qx = fxq(pa.x - (H.org[0] - 0.0)), qy = fxq(pa.y - (H.org[1] - 0.0)),
+              qz = fxq(pa.z - (H.org[2] - 0.0));
+    const int ti = SINGLE ? 0 : type[a];
+    double acc = 0.0;
+    for_each_slot(list16, ld, a, nn[a], [&](int s) {
+      const int4 q = q4[s];
+      const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
+      const double r2q = fma(dx, dx, fma(dy, dy, dz * dz));
+      const TbF &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + q.w];
+      if (r2q <= P.rce2q) {
+        const float r2f = (float)(r2q * (1.0 / GH_FXS2));
+        acc += (double)tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
+      }
+    });
+    rho[a] = acc;
+    eband32[a] = (float)(1.0 / sqrt(acc));
+  }
+}
+
+// Pass 2: smatb_interact (TB.F90:457-586); LDS slot = {qx, qy, qz, eband_j}.
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
+    const int *__restrict__ cell_start, const int *__restrict__ nn,
+    const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
+    const double *__restrict__ rho, const float *__restrict__ eband32, double ev_ui,
+    double *__restrict__ frc, double *__restrict__ epa) {
+  extern __shared__ int4 q4[];
+  __shared__ Halo H;
+  __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
+  for (int k = threadIdx.x; k < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k += blockDim.x) sp[k] = tbf[k];
+  const int b = xcd_block();
+  if (b >= t.nbrick) return;
+  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + t.slot_cap);
+  unsigned char *ty = scell + t.slot_cap;
+  slot_map(H, scell);
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const int src = s + H.delta[h];
+    const double4 p = pos[src];
+    q4[s] = make_int4(fxq(p.x - (H.org[0] - shift(t, H, 0, h))),
+                      fxq(p.y - (H.org[1] - shift(t, H, 1, h))),
+                      fxq(p.z - (H.org[2] - shift(t, H, 2, h))), __float_as_int(eband32[src]));
+    if (!SINGLE) ty[s] = (unsigned char)type[src];
+  }
+  __syncthreads();
+  const double fscale = ev_ui * (1.0 / GH_FXS);
+  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+    const double4 pa = pos[a];
+    const int qx = fxq(pa.x - (H.org[0] - 0.0)), qy = fxq(pa.y - (H.org[1] - 0.0)),
+              qz = fxq(pa.z - (H.org[2] - 0.0));
+    const int ti = SINGLE ? 0 : type[a];
+    const float ebi = eband32[a];
+    double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+    for_each_slot(list16, ld, a, nn[a], [&](int s) {
+      const int4 q = q4[s];
+      const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
+      const double r2q = fma(dx, dx, fma(dy, dy, dz * dz));
+      const TbF &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
+      if (r2q <= P.rce2q) {
+        const float r2f = (float)(r2q * (1.0 / GH_FXS2));
+        const float rinv = __builtin_amdgcn_rsqf(r2f);
+        float factor, repul;
+        tb_pair32(P, r2f * rinv, ebi + __int_as_float(q.w), factor, repul);
+        const double sf = (double)(factor * rinv);  // per fixed-point step
+        fx = fma(-sf, dx, fx);
+        fy = fma(-sf, dy, fy);
+        fz = fma(-sf, dz, fz);
+        ei += (double)repul;
+      }
+    });
+    frc[a] = fx * fscale;
+    frc[ncap + a] = fy * fscale;
+    frc[2 * ncap + a] = fz * fscale;
+    epa[a] = (ei - sqrt(rho[a])) * ev_ui;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SMA-TB, plain FP64 (the reference's arithmetic; LDS holds FP64 positions
+// with the image shift applied).
+// ---------------------------------------------------------------------------
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbDev *__restrict__ tbd, int nt,
@@ -252,48 +434,31 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_tiled(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  double *px = dyn, *py = dyn + t.slot_cap, *pz = dyn + 2 * t.slot_cap;
-  int *ty = (int *)(dyn + 3 * t.slot_cap);
-  stage_pos<!SINGLE>(H, pos, type, px, py, pz, ty);
+  const int cap = t.slot_cap;
+  double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
+  int *ty = reinterpret_cast<int *>(dyn + 3 * cap);
+  unsigned char *scell = reinterpret_cast<unsigned char *>(ty + (SINGLE ? 0 : cap));
+  slot_map(H, scell);
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const int src = s + H.delta[h];
+    const double4 p = pos[src];
+    px[s] = p.x + shift(t, H, 0, h);
+    py[s] = p.y + shift(t, H, 1, h);
+    pz[s] = p.z + shift(t, H, 2, h);
+    if (!SINGLE) ty[s] = type[src];
+  }
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
-    const int m = nn[a];
     double acc = 0.0;
-    for (int k = 0; k < m; ++k) {
-      const int s = ld_slot(&list16[(size_t)k * ld + a]);
+    for_each_slot(list16, ld, a, nn[a], [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
       const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
-      if (MIXED) {
-        const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-        if (r2 > P.rce2) continue;
-        const float r2f = (float)r2;
-        const float r = r2f * __builtin_amdgcn_rsqf(r2f);
-        float phi;
-        if (r < P.rci_f) {
-          phi = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * (r - P.r0f));
-        } else {
-          const float d = r - P.rce_f;
-          const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-          phi = pp * pp;
-        }
-        acc += (double)phi;
-      } else {
-        const double dr = sqrt(dist2_rn(vx, vy, vz));
-        if (dr > P.rce) continue;
-        double banda;
-        if (dr < P.rci) {
-          banda = P.eps2 * exp(-2.0 * P.q * (dr * P.one_r0 - 1.0));
-        } else {
-          const double d = dr - P.rce;
-          const double d2 = d * d, d3 = d2 * d, d4 = d3 * d, d5 = d4 * d;
-          banda = P.b1 * d5 + P.b2 * d4 + P.b3 * d3;
-          banda = banda * banda;
-        }
-        acc += banda;
-      }
-    }
+      const double dr = sqrt(dist2_rn(vx, vy, vz));
+      if (dr <= P.rce) acc += tb_band64(P, dr);
+    });
     rho[a] = acc;
     const double eb = 1.0 / sqrt(acc);
     eband[a] = eb;
@@ -301,15 +466,13 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_tiled(
   }
 }
 
-// SMA-TB pass 2 (smatb_interact, TB.F90:457-586), tiled.
-template <bool MIXED, bool SINGLE>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_tiled(
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbDev *__restrict__ tbd,
-    int nt, const double *__restrict__ rho, const double *__restrict__ eband,
-    const float *__restrict__ eband32, double ev_ui, double *__restrict__ frc,
-    double *__restrict__ epa) {
+    int nt, const double *__restrict__ rho, const double *__restrict__ eband, double ev_ui,
+    double *__restrict__ frc, double *__restrict__ epa) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
@@ -322,89 +485,40 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_tiled(
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   const int cap = t.slot_cap;
-  double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
-  double *eb64 = dyn + 3 * cap;                 // FP64 form
-  float *eb32 = (float *)(dyn + 3 * cap);       // mixed form
-  int *ty = MIXED ? (int *)(eb32 + cap) : (int *)(eb64 + cap);
+  double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap, *eb = dyn + 3 * cap;
+  int *ty = reinterpret_cast<int *>(dyn + 4 * cap);
+  unsigned char *scell = reinterpret_cast<unsigned char *>(ty + (SINGLE ? 0 : cap));
+  slot_map(H, scell);
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-    const int h = halo_of_slot(H, s);
-    const int src = H.hstart[h] + (s - H.hbase[h]);
+    const int h = scell[s];
+    const int src = s + H.delta[h];
     const double4 p = pos[src];
-    px[s] = p.x + H.shx[h];
-    py[s] = p.y + H.shy[h];
-    pz[s] = p.z + H.shz[h];
-    if (MIXED) eb32[s] = eband32[src];
-    else eb64[s] = eband[src];
+    px[s] = p.x + shift(t, H, 0, h);
+    py[s] = p.y + shift(t, H, 1, h);
+    pz[s] = p.z + shift(t, H, 2, h);
+    eb[s] = eband[src];
     if (!SINGLE) ty[s] = type[src];
   }
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
-    const int m = nn[a];
+    const double ebi = eband[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
-    if (MIXED) {
-      const float ebi = eband32[a];
-      for (int k = 0; k < m; ++k) {
-        const int s = ld_slot(&list16[(size_t)k * ld + a]);
-        const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
-        const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
-        const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-        if (r2 > P.rce2) continue;
-        const float r2f = (float)r2;
-        const float rinv = __builtin_amdgcn_rsqf(r2f);
-        const float r = r2f * rinv;
-        const float ebs = ebi + eb32[s];
-        float factor, repul;
-        if (r <= P.rci_f) {
-          const float s0 = r - P.r0f;
-          repul = P.rep_a * __builtin_amdgcn_exp2f(P.rep_b * s0);
-          const float band = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * s0);
-          factor = fmaf(repul, P.frep, -band * P.fband * ebs);
-        } else {
-          const float d = r - P.rce_f;
-          const float d2 = d * d, d3 = d2 * d;
-          const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-          const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
-          const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
-          factor = fmaf(pp * dp, ebs, -2.0f * dr_);
-          repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
-        }
-        const double sf = (double)(factor * rinv);
-        fx = fma(-sf, vx, fx);
-        fy = fma(-sf, vy, fy);
-        fz = fma(-sf, vz, fz);
-        ei += (double)repul;
-      }
-    } else {
-      const double ebi = eband[a];
-      for (int k = 0; k < m; ++k) {
-        const int s = ld_slot(&list16[(size_t)k * ld + a]);
-        const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
-        const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
-        const double dr = sqrt(dist2_rn(vx, vy, vz));
-        if (dr > P.rce) continue;
+    for_each_slot(list16, ld, a, nn[a], [&](int s) {
+      const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
+      const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
+      const double dr = sqrt(dist2_rn(vx, vy, vz));
+      if (dr <= P.rce) {
         double factor, repul;
-        if (dr <= P.rci) {
-          const double tt = dr * P.one_r0 - 1.0;
-          repul = P.ca * exp(-P.p * tt);
-          factor = 2.0 * repul * P.p_r0 -
-                   P.eps2 * 0.5 * exp(-2.0 * P.q * tt) * P.q2_r0 * (ebi + eb64[s]);
-        } else {
-          const double d = dr - P.rce;
-          const double d2 = d * d, d3 = d2 * d, d4 = d3 * d, d5 = d4 * d;
-          factor = (P.b1 * d5 + P.b2 * d4 + P.b3 * d3) *
-                   (5.0 * P.b1 * d4 + 4.0 * P.b2 * d3 + 3.0 * P.b3 * d2) * (ebi + eb64[s]);
-          factor = factor - 2.0 * (5.0 * P.r1 * d4 + 4.0 * P.r2 * d3 + 3.0 * P.r3 * d2);
-          repul = P.r1 * d5 + P.r2 * d4 + P.r3 * d3;
-        }
+        tb_pair64(P, dr, ebi + eb[s], factor, repul);
         const double sf = factor / dr;
         fx -= sf * vx;
         fy -= sf * vy;
         fz -= sf * vz;
         ei += repul;
       }
-    }
+    });
     frc[a] = fx * ev_ui;
     frc[ncap + a] = fy * ev_ui;
     frc[2 * ncap + a] = fz * ev_ui;
@@ -412,7 +526,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_tiled(
   }
 }
 
-// Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled.
+// Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
 template <bool MIXED>
 __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
@@ -424,41 +538,50 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  double *px = dyn, *py = dyn + t.slot_cap, *pz = dyn + 2 * t.slot_cap;
-  stage_pos<false>(H, pos, nullptr, px, py, pz, nullptr);
+  const int cap = t.slot_cap;
+  double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * cap);
+  slot_map(H, scell);
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const double4 p = pos[s + H.delta[h]];
+    px[s] = p.x + shift(t, H, 0, h);
+    py[s] = p.y + shift(t, H, 1, h);
+    pz[s] = p.z + shift(t, H, 2, h);
+  }
   __syncthreads();
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
-    const int m = nn[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
-    for (int k = 0; k < m; ++k) {
-      const int s = ld_slot(&list16[(size_t)k * ld + a]);
+    for_each_slot(list16, ld, a, nn[a], [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
       if (MIXED) {
         const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-        if (r2 > rc2) continue;
-        const float ir2 = __builtin_amdgcn_rcpf((float)r2);
-        const float s2 = sig2f * ir2;
-        const float s6 = s2 * s2 * s2;
-        ei += (double)(4.0f * epsf * (s6 * s6 - s6));
-        const double sf = (double)(epsf * fmaf(48.0f, s6, -24.0f) * s6 * ir2);
-        fx = fma(-sf, vx, fx);
-        fy = fma(-sf, vy, fy);
-        fz = fma(-sf, vz, fz);
+        if (r2 <= rc2) {
+          const float ir2 = __builtin_amdgcn_rcpf((float)r2);
+          const float s2 = sig2f * ir2;
+          const float s6 = s2 * s2 * s2;
+          ei += (double)(4.0f * epsf * (s6 * s6 - s6));
+          const double sf = (double)(epsf * fmaf(48.0f, s6, -24.0f) * s6 * ir2);
+          fx = fma(-sf, vx, fx);
+          fy = fma(-sf, vy, fy);
+          fz = fma(-sf, vz, fz);
+        }
       } else {
         const double r2 = dist2_rn(vx, vy, vz);
-        if (r2 > rc2) continue;
-        const double dr = sqrt(r2);
-        const double tt = sigma / dr;
-        const double t2 = tt * tt, t4 = t2 * t2, s6 = t4 * t2;
-        ei += 4.0 * eps * (s6 * s6 - s6);
-        const double sf = eps * (48.0 * s6 - 24.0) * s6 / dr / dr;
-        fx -= sf * vx;
-        fy -= sf * vy;
-        fz -= sf * vz;
+        if (r2 <= rc2) {
+          const double dr = sqrt(r2);
+          const double tt = sigma / dr;
+          const double t2 = tt * tt, t4 = t2 * t2, s6 = t4 * t2;
+          ei += 4.0 * eps * (s6 * s6 - s6);
+          const double sf = eps * (48.0 * s6 - 24.0) * s6 / dr / dr;
+          fx -= sf * vx;
+          fy -= sf * vy;
+          fz -= sf * vz;
+        }
       }
-    }
+    });
     frc[a] = fx * ev_ui;
     frc[ncap + a] = fy * ev_ui;
     frc[2 * ncap + a] = fz * ev_ui;
@@ -466,22 +589,38 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
   }
 }
 
-inline size_t lds_bytes_force(const gh_ctx *c) {
-  const bool multi = c->nt > 1;
-  const size_t per = 24 + (c->precision == GH_MIXED ? 4 : 8) + (multi ? 4 : 0);
-  return per * (size_t)c->tg.slot_cap;
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// LDS budgets (bytes per halo slot, and static bytes) of the tiled passes.
+// ---------------------------------------------------------------------------
+size_t gh_tile_slot_bytes(int pass, int precision, int nt) {
+  const bool multi = nt > 1;
+  if (precision == GH_MIXED) {
+    if (pass == 0) return 16 + 1;                 // rho: int4 + slot map
+    return 16 + 1 + (multi ? 1 : 0);              // force: int4 + map + type
+  }
+  if (pass == 0) return 24 + 1 + (multi ? 4 : 0);  // rho: xyz + map (+ type)
+  return 32 + 1 + (multi ? 4 : 0);                 // force: xyz + eband + map (+ type)
+}
+size_t gh_tile_static_bytes(int precision, int nt) {
+  const size_t params = nt > 1 ? (precision == GH_MIXED ? sizeof(TbF) : sizeof(TbDev)) *
+                                     GH_MAXT * GH_MAXT
+                               : sizeof(TbDev);
+  return sizeof(Halo) + params;
 }
 
-}  // namespace
+static size_t build_slots() { return GH_BUILD_LDS / 25; }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
-  const size_t lds = 24 * (size_t)c->tg.slot_cap;
+  const int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
+  const size_t lds = 25 * (size_t)lds_slots;
   k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, c->stream>>>(
       c->status, c->tg, c->pos, c->cellof, c->cell_start, (c->rcut + c->skin) * (c->rcut + c->skin),
-      count_only ? 0 : cap, c->ncap, c->ncap, c->nn, c->list16, c->xold);
+      count_only ? 0 : cap, c->ncap, c->ncap, lds_slots, c->nn, c->list16, c->xold);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -517,35 +656,51 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid) {
   const double ev_ui = ghc::ev_ui();
   const bool single = c->nt == 1;
   const bool mixed = c->precision == GH_MIXED;
-  const size_t lds_pos = (24 + (single ? 0 : 4)) * (size_t)c->tg.slot_cap;
+  const size_t slots = (size_t)c->tg.slot_cap;
   if (c->pot == GH_POT_SMATB) {
-#define RHO(MX, SG)                                                                          \
-  k_tb_rho_tiled<MX, SG><<<grid, GH_TBLOCK, lds_pos, s>>>(c->status, c->tg, c->pos, c->type, \
-                                                          c->cell_start, c->nn, c->list16,   \
-                                                          ld, c->tbd, c->nt, c->rho,         \
-                                                          c->eband, c->eband32)
+    const size_t lds1 = gh_tile_slot_bytes(0, c->precision, c->nt) * slots;
+    const size_t lds2 = gh_tile_slot_bytes(1, c->precision, c->nt) * slots;
     if (mixed) {
-      if (single) RHO(true, true); else RHO(true, false);
+      if (single)
+        k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
+                                                        c->cell_start, c->nn, c->list16, ld,
+                                                        c->tbf, c->rho, c->eband32);
+      else
+        k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
+                                                         c->cell_start, c->nn, c->list16, ld,
+                                                         c->tbf, c->rho, c->eband32);
+      if (mid) GH_HIP(c, hipEventRecord(mid, s));
+      if (single)
+        k_tb_force_fx<true><<<grid, GH_TBLOCK, lds2, s>>>(
+            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
+            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa);
+      else
+        k_tb_force_fx<false><<<grid, GH_TBLOCK, lds2, s>>>(
+            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
+            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa);
     } else {
-      if (single) RHO(false, true); else RHO(false, false);
+      if (single)
+        k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
+            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
+            c->nt, c->rho, c->eband, c->eband32);
+      else
+        k_tb_rho64_tiled<false><<<grid, GH_TBLOCK, lds1, s>>>(
+            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
+            c->nt, c->rho, c->eband, c->eband32);
+      if (mid) GH_HIP(c, hipEventRecord(mid, s));
+      if (single)
+        k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
+            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa);
+      else
+        k_tb_force64_tiled<false><<<grid, GH_TBLOCK, lds2, s>>>(
+            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa);
     }
-#undef RHO
-    if (mid) GH_HIP(c, hipEventRecord(mid, s));
-    const size_t lds = lds_bytes_force(c);
-#define FRC(MX, SG)                                                                     \
-  k_tb_force_tiled<MX, SG><<<grid, GH_TBLOCK, lds, s>>>(                                \
-      c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,  \
-      c->tbd, c->nt, c->rho, c->eband, c->eband32, ev_ui, c->frc, c->epa)
-    if (mixed) {
-      if (single) FRC(true, true); else FRC(true, false);
-    } else {
-      if (single) FRC(false, true); else FRC(false, false);
-    }
-#undef FRC
   } else if (c->pot == GH_POT_LJ) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
-    const size_t lds = 24 * (size_t)c->tg.slot_cap;
+    const size_t lds = 25 * slots;
     if (mixed)
       k_lj_tiled<true><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
                                                     c->nn, c->list16, ld, c->ncap, c->lj_eps,

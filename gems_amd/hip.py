@@ -22,7 +22,7 @@ SYMBOLS = [
     "gh_set_atoms", "gh_set_masses", "gh_set_velocities", "gh_set_smatb", "gh_set_lj",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
-    "gh_get_stats", "gh_set_timing", "gh_get_timing",
+    "gh_get_stats", "gh_set_timing", "gh_get_timing", "gh_set_tile", "gh_autotune",
     "gh_batch_create", "gh_batch_destroy", "gh_batch_last_error", "gh_batch_set_precision",
     "gh_batch_set_systems", "gh_batch_set_smatb", "gh_batch_set_masses", "gh_batch_set_skin",
     "gh_batch_set_velocities", "gh_batch_set_temperatures", "gh_batch_compute",
@@ -69,6 +69,8 @@ def load():
     L.gh_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
     L.gh_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.gh_set_skin.argtypes = [vp, C.c_double]
+    L.gh_set_tile.argtypes = [vp, _ip]
+    L.gh_autotune.argtypes = [vp, C.c_int, _ip, _dp]
     L.gh_compute.argtypes = [vp, _dp, _dp, _dp]
     L.gh_neighbors.argtypes = [vp, _ip, _ip, C.c_int]
     L.gh_prng_lcg.argtypes = [vp, C.c_int]
@@ -177,6 +179,19 @@ class Context:
 
     def set_skin(self, skin):
         self._ck(self.L.gh_set_skin(self.h, skin))
+
+    def set_tile(self, bd=None):
+        """Brick edges (cells) of the tiled passes; None = automatic."""
+        b = None if bd is None else np.ascontiguousarray(bd, np.int32)
+        self._ck(self.L.gh_set_tile(self.h, _p(b, _ip)))
+
+    def autotune(self, reps=5):
+        """Times the force passes per brick shape and keeps the fastest;
+        returns (shape, microseconds per force evaluation)."""
+        bd = np.zeros(3, np.int32)
+        us = C.c_double(0.0)
+        self._ck(self.L.gh_autotune(self.h, reps, _p(bd, _ip), C.byref(us)))
+        return tuple(int(v) for v in bd), us.value
 
     # --- evaluation ---
     def compute(self):

@@ -94,6 +94,18 @@ int gh_set_lj(gh_ctx *ctx, double eps, double sigma, double rc);
 /* Verlet skin `nb_dcut` (Neighbour.F90:237), A. */
 int gh_set_skin(gh_ctx *ctx, double skin);
 
+/* Cell-tile shape of the tiled passes.  The reference fixes only the cell
+ * grid (intergroup_setcells, Neighbour.F90:587-664); the device groups cells
+ * into bricks of bd[0] x bd[1] x bd[2] cells (1..4 each, 0 = automatic), one
+ * workgroup per brick.  No counterpart in the reference. */
+int gh_set_tile(gh_ctx *ctx, const int bd[3]);
+/* Autotuner: times `reps` force evaluations for each brick shape whose halo
+ * fits the LDS budget and keeps the fastest.  Writes the chosen shape and its
+ * time per force evaluation (us); bd_out = {0,0,0} when the O(N^2) list is in
+ * use (nothing to tune).  Positions and velocities are unchanged.  No
+ * counterpart in the reference. */
+int gh_autotune(gh_ctx *ctx, int reps, int bd_out[3], double *us_out);
+
 /* One force evaluation == interact() (Interaction.f90:135-241): skin test
  * and neighbour rebuild (test_update/update, Neighbour.F90:1242-1346), then
  * smatb_interact (TB.F90:457-586) or lj_interact (Pairs.F90:207-279).

@@ -187,6 +187,47 @@ def test_reference_kat_md_ag13():
     c.close()
 
 
+def test_tile_shapes_and_autotune():
+    # brick shapes change only how cells are grouped per workgroup: the
+    # neighbour sets, and the FP64 forces bit for bit, must not depend on them
+    x, L = _au_big(nx=12, seed=3)
+    s = O.smatb_params(cm.au_params(), 1)
+    box = O.make_box([L] * 3, (1, 1, 1))
+    nn, lst, _ = O.neighbours(x, box, O.smatb_rcut(s) + 1.0)
+    Fo, _, epo, _ = O.smatb_force(x, np.zeros(len(x), np.int32), box, s, nn, lst)
+    ref = {}
+    for prec in (hip.GH_FP64, hip.GH_MIXED):
+        for bd in [(1, 1, 1), (2, 2, 2), (4, 4, 3), (3, 2, 1), None]:
+            c = _ctx(prec)
+            c.set_box([L] * 3, (1, 1, 1))
+            c.set_atoms(x)
+            c.set_masses([cm.MASS_AU])
+            c.set_smatb(1, *cm.matrices(cm.au_params(), 1))
+            c.set_tile(bd)
+            F, e, et = c.compute()
+            assert c.stats()["list_mode"] == 1
+            gn, gl = c.neighbors(cap=256)
+            assert np.array_equal(gn, nn)
+            for i in range(0, len(nn), 97):
+                assert np.array_equal(gl[i, :gn[i]], np.sort(lst[i, :nn[i]])), (bd, i)
+            if prec == hip.GH_FP64:
+                assert cm.force_err(F, Fo) <= 1e-10
+            else:
+                assert np.sqrt(np.mean((F - Fo) ** 2)) / np.sqrt(np.mean(Fo ** 2)) <= 1e-5
+            if prec in ref:
+                tol = 0.0 if prec == hip.GH_FP64 else 2e-5
+                assert np.max(np.abs(F - ref[prec])) <= tol * np.abs(Fo).max(), bd
+            else:
+                ref[prec] = F
+            if bd is None:
+                shape, us = c.autotune(reps=2)
+                assert all(1 <= v <= 4 for v in shape) and us > 0
+                F2, _, _ = c.compute()
+                tol = 0.0 if prec == hip.GH_FP64 else 2e-5
+                assert np.max(np.abs(F2 - ref[prec])) <= tol * np.abs(Fo).max()
+            c.close()
+
+
 def test_results_are_deterministic():
     out = []
     for _ in range(2):
