@@ -165,6 +165,18 @@ def batch_fire(hip, nclusters=4096):
                         "alpha0 0.1, FP32 pair terms to 1e-2 eV/A then FP64 to 1e-4 eV/A"}
 
 
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size: its FETCH_SIZE/WRITE_SIZE
+    counters calibrate the HBM traffic read for the force kernels."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,7 +190,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--re-rounds", type=int, default=4)
     ap.add_argument("--fire-clusters", type=int, default=4096)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(multi-GPU sharding is described in DESIGN.md, not run)" % args.gpus,

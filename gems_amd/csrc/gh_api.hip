@@ -1001,7 +1001,7 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
     else GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
     pending_b = false;
     GH_TRY(rec(c, 1));
-    GH_TRY(gh_launch_rebuild(c, false, true));
+    GH_TRY(gh_launch_rebuild(c, false));
     GH_TRY(rec(c, 2));
     GH_TRY(gh_launch_forces(c, ev_at(c, 3)));
     GH_TRY(rec(c, 4));

@@ -103,7 +103,7 @@ struct GhStatus {                     // device-resident counters
   int tile_overflow;                  // a brick's halo exceeded slot_cap (sticky)
   int max_halo;                       // largest halo (atoms) at the last build
   int max_brick;                      // largest brick (atoms) at the last build
-  unsigned ticket;                    // workgroups of an integrator kernel done
+  int pad;
 };
 
 struct gh_ctx {
@@ -207,7 +207,7 @@ int gh_fail(gh_ctx *c, int code, const std::string &msg);
   } while (0)
 
 // launchers (gh_kernels.hip)
-int gh_launch_rebuild(gh_ctx *c, bool force, bool decided = false);
+int gh_launch_rebuild(gh_ctx *c, bool force);
 int gh_launch_forces(gh_ctx *c, hipEvent_t mid);
 int gh_launch_vv_a(gh_ctx *c);
 int gh_launch_vv_b(gh_ctx *c);

@@ -631,53 +631,24 @@ struct WrapArgs {
 // pair's periodic image, which stays valid while atoms move continuously.
 // They are wrapped into the box at every rebuild (k_wrap_rebuild) and when
 // copied out, which reproduces do_pbc_1 (Set_Properties.f90:444-470).
-// Rebuild decision at the end of an integrator kernel.  Each workgroup
-// stores its maximum |x - x_old|^2; the last workgroup to finish (ticket
-// counter) reduces them and sets the flag the rebuild kernels read, as
-// k_decide does when no integrator step precedes a force evaluation.
-__device__ __forceinline__ void disp_decide(double d2, double *__restrict__ dispblk,
-                                            GhStatus *st, double skin) {
+// Per-workgroup maximum of |x - x_old|^2, one value per workgroup; k_decide
+// reduces them in its own launch.  (A last-workgroup-done reduction inside
+// the integrator needs an agent-scope release per workgroup, which writes
+// back the XCD's L2 each time: measured 3x slower than the extra launch.)
+__device__ __forceinline__ void disp_reduce(double d2, double *__restrict__ dispblk) {
   __shared__ double part[GH_BLOCK / 64];
-  __shared__ int last;
   d2 = wave_max(d2);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < GH_BLOCK / 64; ++w) d2 = fmax(d2, part[w]);
-    __hip_atomic_store(&dispblk[blockIdx.x], d2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned t =
-        __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = t == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  double m = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    m = fmax(m, __hip_atomic_load(&dispblk[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    dispblk[b] = 0.0;  // consumed, as in k_decide
-  }
-  m = wave_max(m);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < GH_BLOCK / 64; ++w) m = fmax(m, part[w]);
-    const double dd = fmax(m, __longlong_as_double((long long)st->maxdisp2_bits));
-    const int rb = st->force_rebuild || (2.0 * sqrt(dd) > skin);
-    st->rebuild = rb;
-    if (rb) {
-      st->nupd += 1;
-      st->force_rebuild = 0;
-      st->maxnn = 0;
-    }
-    st->maxdisp2_bits = 0ull;
-    st->ticket = 0u;
+    dispblk[blockIdx.x] = d2;
   }
 }
 
 // velocity_verlet_a (Integration.F90:1150-1169) + do_pbc + displacement.
 __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
-    GhStatus *st, double skin, int n, int ncap, double dt, MassArgs ma,
+    int n, int ncap, double dt, MassArgs ma,
     const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
     double *__restrict__ acc, const double *__restrict__ frc, const double *__restrict__ xold,
     double *__restrict__ dispblk) {
@@ -703,7 +674,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_decide(d2, dispblk, st, skin);
+  disp_reduce(d2, dispblk);
 }
 
 // velocity_verlet_b of one step fused with velocity_verlet_a of the next
@@ -711,7 +682,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
 // order as the two kernels, with a = F/m formed once.  The acceleration is
 // not stored (stepa recomputes it; the last stepb of a run stores it).
 __global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
-    GhStatus *st, double skin, int n, int ncap, double dt, MassArgs ma,
+    int n, int ncap, double dt, MassArgs ma,
     const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
     const double *__restrict__ frc, const double *__restrict__ xold,
     double *__restrict__ dispblk) {
@@ -736,7 +707,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_decide(d2, dispblk, st, skin);
+  disp_reduce(d2, dispblk);
 }
 
 // velocity_verlet_b (Integration.F90:1171-1186).
@@ -777,7 +748,7 @@ __device__ __forceinline__ void ermak_gauss(const ErmakArgs &k, int id,
 
 // ermak_a (Integration.F90:963-1011) + do_pbc + displacement.
 __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
-    GhStatus *st, double skin, int n, int ncap, ErmakArgs k, MassArgs ma,
+    int n, int ncap, ErmakArgs k, MassArgs ma,
     const int *__restrict__ type,
     const int *__restrict__ orig, const double *__restrict__ gauss, double4 *__restrict__ pos,
     const double *__restrict__ vel, const double *__restrict__ acc,
@@ -802,7 +773,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_decide(d2, dispblk, st, skin);
+  disp_reduce(d2, dispblk);
 }
 
 // ermak_b (Integration.F90:1013-1031); vel_v is re-derived from the same
@@ -981,7 +952,7 @@ ErmakArgs make_ermak(const gh_ctx *c) {
 // sequence can be enqueued every step without a host round trip.
 using namespace ghd;
 
-int gh_launch_rebuild(gh_ctx *c, bool force, bool decided) {
+int gh_launch_rebuild(gh_ctx *c, bool force) {
   hipStream_t s = c->stream;
   if (force) {
     GhStatus h;
@@ -993,7 +964,7 @@ int gh_launch_rebuild(gh_ctx *c, bool force, bool decided) {
   }
   // dispblk holds the last integrator step's per-workgroup maxima (zero after
   // a rebuild or at setup)
-  if (force || !decided) k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
+  k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
   const int n = c->n, nb = nblocks(n);
   const int gs = std::min(nb, GH_STRIDE_GRID);  // grid-stride: cheap when no rebuild is due
   if (c->list_mode == 0 && c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
@@ -1096,7 +1067,7 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
 }
 
 int gh_launch_vv_a(gh_ctx *c) {
-  k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->status, c->skin, c->n, c->ncap, c->dt,
+  k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                     make_mass(c), c->type, c->pos, c->vel,
                                                     c->acc, c->frc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
@@ -1104,7 +1075,7 @@ int gh_launch_vv_a(gh_ctx *c) {
 }
 
 int gh_launch_vv_ba(gh_ctx *c) {
-  k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->status, c->skin, c->n, c->ncap, c->dt,
+  k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                      make_mass(c), c->type, c->pos, c->vel,
                                                      c->frc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
@@ -1120,7 +1091,7 @@ int gh_launch_vv_b(gh_ctx *c) {
 
 int gh_launch_ermak_a(gh_ctx *c) {
   k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
-      c->status, c->skin, c->n, c->ncap, make_ermak(c), make_mass(c), c->type, c->orig,
+      c->n, c->ncap, make_ermak(c), make_mass(c), c->type, c->orig,
       c->gauss, c->pos, c->vel, c->acc, c->xold, c->dispblk);
   GH_HIP(c, hipGetLastError());
   return 0;

@@ -288,19 +288,31 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
     }
 }
 
-// Runs f(slot) over an atom's list, one 8-byte load per four entries, the
-// next quad's load issued before the current quad is processed.
+// Runs f(slot) over an atom's list, one 8-byte load per four entries.  The
+// loads run two quads ahead of use and are unconditional (the index is
+// clamped to the row's last quad), so no wait on the list stream sits
+// between a quad's load and its use two quads later.
 template <class F>
 __device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__ list16, int ld,
                                               int a, int m, F &&f) {
+  const int ng = (m + 3) >> 2;
+  if (ng == 0) return;
   const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
-  unsigned long long w = m > 0 ? ld_quad(lp) : 0ull;
-  for (int k0 = 0; k0 < m; k0 += 4) {
-    const unsigned long long wn = k0 + 4 < m ? ld_quad(lp + (size_t)((k0 >> 2) + 1) * ld) : 0ull;
+  const size_t sld = (size_t)ld;
+  unsigned long long w0 = ld_quad(lp);
+  unsigned long long w1 = ld_quad(lp + (size_t)min(1, ng - 1) * sld);
+  for (int g = 0; g < ng; g += 2) {
+    const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
+    const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
+    const int k0 = g * 4;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (k0 + u < m) f((int)((w >> (16 * u)) & 0xFFFFull));
-    w = wn;
+      if (k0 + u < m) f((int)((w0 >> (16 * u)) & 0xFFFFull));
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + 4 + u < m) f((int)((w1 >> (16 * u)) & 0xFFFFull));
+    w0 = n0;
+    w1 = n1;
   }
 }
 
