@@ -211,7 +211,7 @@ static int max_slots(const gh_ctx *c) {
   const size_t per = std::max(gh_tile_slot_bytes(0, c->precision, c->nt),
                               gh_tile_slot_bytes(1, c->precision, c->nt));
   const size_t stat = gh_tile_static_bytes(c->precision, c->nt);
-  return (int)std::min<size_t>((GH_LDS_BYTES - stat) / per, 65535);
+  return (int)std::min<size_t>((GH_LDS_BYTES - stat) / per - 1, 65534);  // - the sentinel
 }
 
 // Brick edges (cells) for the tiled passes: the largest brick whose expected

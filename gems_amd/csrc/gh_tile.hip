@@ -155,12 +155,20 @@ __device__ __forceinline__ unsigned long long ld_quad(const unsigned long long *
   return __builtin_nontemporal_load(p);
 }
 __device__ __forceinline__ int fxq(double d) { return __double2int_rn(d * GH_FXS); }
+#define GH_FXFAR (1 << 30)  // sentinel slot, fixed point: > 100 A from any staged atom
+#define GH_FAR 1.0e30       // sentinel slot, FP64
 
 // ---------------------------------------------------------------------------
 // Neighbour list (linkedcell_self, Neighbour.F90:991-1087): for each atom of
 // the brick, the slots of its 27 neighbour cells with rd < rl^2, the atom
 // itself excluded.  vd = (x_j - x_i) + image shift, rounded exactly as the
 // reference's vdistance + dot_product, so the sets are bit-identical.
+//
+// Row order: two sweeps of the 27 cells, first the pairs with rd < near2
+// (inside the potential's inner branch: r < rci for SMA-TB, r < rc for LJ),
+// then the rest, each sweep in stencil order.  Lanes of a wavefront then take
+// the same branch of the pair function on most iterations.  The last quad is
+// padded with `pad`, a sentinel LDS slot far from every atom.
 // Positions are staged in LDS when the halo fits GH_BUILD_LDS (STAGE), else
 // read from L2.
 // ---------------------------------------------------------------------------
@@ -169,7 +177,8 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
                                            const double4 *__restrict__ pos,
                                            const int *__restrict__ cellof, const double *px,
                                            const double *py, const double *pz, double rl2,
-                                           int cap, int ld, int ncap, int *__restrict__ nn,
+                                           double near2, int cap, int pad, int ld, int ncap,
+                                           int *__restrict__ nn,
                                            unsigned short *__restrict__ list16,
                                            double *__restrict__ xold, int &mmax) {
   unsigned long long *lq = reinterpret_cast<unsigned long long *>(list16);
@@ -182,40 +191,45 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
     const double xi = pa.x, yi = pa.y, zi = pa.z;
     int m = 0;
     unsigned long long w = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
-          const double sx = shift(t, H, 0, h), sy = shift(t, H, 1, h), sz = shift(t, H, 2, h);
-          const int se = H.hbase[h + 1];
-          const int dl = H.delta[h];
-          for (int s = H.hbase[h]; s < se; ++s) {
-            if (s == own) continue;
-            double xj, yj, zj;
-            if (STAGE) {
-              xj = px[s];
-              yj = py[s];
-              zj = pz[s];
-            } else {
-              const double4 pj = pos[s + dl];
-              xj = pj.x;
-              yj = pj.y;
-              zj = pj.z;
-            }
-            const double vx = (xj - xi) + sx;
-            const double vy = (yj - yi) + sy;
-            const double vz = (zj - zi) + sz;
-            if (dist2_rn(vx, vy, vz) < rl2) {
-              w |= (unsigned long long)s << (16 * (m & 3));
-              if ((m & 3) == 3) {
-                if (m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
-                w = 0;
+    for (int sweep = 0; sweep < 2; ++sweep)
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
+            const double sx = shift(t, H, 0, h), sy = shift(t, H, 1, h), sz = shift(t, H, 2, h);
+            const int se = H.hbase[h + 1];
+            const int dl = H.delta[h];
+            for (int s = H.hbase[h]; s < se; ++s) {
+              if (s == own) continue;
+              double xj, yj, zj;
+              if (STAGE) {
+                xj = px[s];
+                yj = py[s];
+                zj = pz[s];
+              } else {
+                const double4 pj = pos[s + dl];
+                xj = pj.x;
+                yj = pj.y;
+                zj = pj.z;
               }
-              ++m;
+              const double vx = (xj - xi) + sx;
+              const double vy = (yj - yi) + sy;
+              const double vz = (zj - zi) + sz;
+              const double rd = dist2_rn(vx, vy, vz);
+              if (rd < rl2 && ((rd < near2) == (sweep == 0))) {
+                w |= (unsigned long long)s << (16 * (m & 3));
+                if ((m & 3) == 3) {
+                  if (m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
+                  w = 0;
+                }
+                ++m;
+              }
             }
           }
-        }
-    if ((m & 3) && m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
+    if ((m & 3) && m < cap) {
+      for (int k = m & 3; k < 4; ++k) w |= (unsigned long long)pad << (16 * k);
+      lq[(size_t)(m >> 2) * ld + a] = w;
+    }
     nn[a] = min(m, cap);
     xold[a] = xi;
     xold[ncap + a] = yi;
@@ -226,8 +240,9 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
 
 __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
-    const int *__restrict__ cell_start, double rl2, int cap, int ld, int ncap, int lds_slots,
-    int *__restrict__ nn, unsigned short *__restrict__ list16, double *__restrict__ xold) {
+    const int *__restrict__ cell_start, double rl2, double near2, int cap, int ld, int ncap,
+    int lds_slots, int *__restrict__ nn, unsigned short *__restrict__ list16,
+    double *__restrict__ xold) {
   if (!st->rebuild) return;
   extern __shared__ double dyn[];
   __shared__ Halo H;
@@ -235,6 +250,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   int mmax = 0;
+  const int pad = t.slot_cap;  // sentinel slot of the force passes
   if (H.total <= lds_slots) {
     double *px = dyn, *py = dyn + lds_slots, *pz = dyn + 2 * lds_slots;
     unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * lds_slots);
@@ -246,10 +262,11 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
       pz[s] = p.z;
     }
     __syncthreads();
-    build_rows<true>(H, t, pos, cellof, px, py, pz, rl2, cap, ld, ncap, nn, list16, xold, mmax);
+    build_rows<true>(H, t, pos, cellof, px, py, pz, rl2, near2, cap, pad, ld, ncap, nn, list16,
+                     xold, mmax);
   } else {
-    build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, rl2, cap, ld, ncap, nn,
-                      list16, xold, mmax);
+    build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, rl2, near2, cap, pad, ld,
+                      ncap, nn, list16, xold, mmax);
   }
   mmax = wave_imax(mmax);
   if ((threadIdx.x & 63) == 0 && mmax > 0) {
@@ -288,10 +305,11 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
     }
 }
 
-// Runs f(slot) over an atom's list, one 8-byte load per four entries.  The
-// loads run two quads ahead of use and are unconditional (the index is
-// clamped to the row's last quad), so no wait on the list stream sits
-// between a quad's load and its use two quads later.
+// Runs f(slot) over an atom's list, one 8-byte load per four entries; the
+// row's last quad is padded with the sentinel slot, so every entry of every
+// quad is processed without a bounds test.  The loads run two quads ahead of
+// use and are unconditional (the index is clamped to the row's last quad), so
+// no wait on the list stream sits between a quad's load and its use.
 template <class F>
 __device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__ list16, int ld,
                                               int a, int m, F &&f) {
@@ -304,13 +322,12 @@ __device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__
   for (int g = 0; g < ng; g += 2) {
     const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
     const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
-    const int k0 = g * 4;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + u < m) f((int)((w0 >> (16 * u)) & 0xFFFFull));
+    for (int u = 0; u < 4; ++u) f((int)((w0 >> (16 * u)) & 0xFFFFull));
+    if (g + 1 < ng) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + 4 + u < m) f((int)((w1 >> (16 * u)) & 0xFFFFull));
+      for (int u = 0; u < 4; ++u) f((int)((w1 >> (16 * u)) & 0xFFFFull));
+    }
     w0 = n0;
     w1 = n1;
   }
@@ -333,7 +350,8 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + t.slot_cap);
+  const int ns = t.slot_cap + 1;  // slots + the sentinel
+  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + ns);
   slot_map(H, scell);
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
@@ -343,6 +361,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
                       fxq(p.y - (H.org[1] - shift(t, H, 1, h))),
                       fxq(p.z - (H.org[2] - shift(t, H, 2, h))), SINGLE ? 0 : type[src]);
   }
+  if (threadIdx.x == 0) q4[t.slot_cap] = make_int4(GH_FXFAR, GH_FXFAR, GH_FXFAR, 0);
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
@@ -380,8 +399,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + t.slot_cap);
-  unsigned char *ty = scell + t.slot_cap;
+  const int ns = t.slot_cap + 1;  // slots + the sentinel
+  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + ns);
+  unsigned char *ty = scell + ns;
   slot_map(H, scell);
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
@@ -391,6 +411,10 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
                       fxq(p.y - (H.org[1] - shift(t, H, 1, h))),
                       fxq(p.z - (H.org[2] - shift(t, H, 2, h))), __float_as_int(eband32[src]));
     if (!SINGLE) ty[s] = (unsigned char)type[src];
+  }
+  if (threadIdx.x == 0) {
+    q4[t.slot_cap] = make_int4(GH_FXFAR, GH_FXFAR, GH_FXFAR, 0);
+    if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
   const double fscale = ev_ui * (1.0 / GH_FXS);
@@ -446,7 +470,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  const int cap = t.slot_cap;
+  const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
   int *ty = reinterpret_cast<int *>(dyn + 3 * cap);
   unsigned char *scell = reinterpret_cast<unsigned char *>(ty + (SINGLE ? 0 : cap));
@@ -459,6 +483,10 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
     py[s] = p.y + shift(t, H, 1, h);
     pz[s] = p.z + shift(t, H, 2, h);
     if (!SINGLE) ty[s] = type[src];
+  }
+  if (threadIdx.x == 0) {
+    px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
+    if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
@@ -496,7 +524,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  const int cap = t.slot_cap;
+  const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap, *eb = dyn + 3 * cap;
   int *ty = reinterpret_cast<int *>(dyn + 4 * cap);
   unsigned char *scell = reinterpret_cast<unsigned char *>(ty + (SINGLE ? 0 : cap));
@@ -510,6 +538,11 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
     pz[s] = p.z + shift(t, H, 2, h);
     eb[s] = eband[src];
     if (!SINGLE) ty[s] = type[src];
+  }
+  if (threadIdx.x == 0) {
+    px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
+    eb[t.slot_cap] = 0.0;
+    if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
@@ -550,7 +583,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
-  const int cap = t.slot_cap;
+  const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
   unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * cap);
   slot_map(H, scell);
@@ -561,6 +594,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     py[s] = p.y + shift(t, H, 1, h);
     pz[s] = p.z + shift(t, H, 2, h);
   }
+  if (threadIdx.x == 0) px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
   __syncthreads();
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
@@ -624,14 +658,25 @@ size_t gh_tile_static_bytes(int precision, int nt) {
 
 static size_t build_slots() { return GH_BUILD_LDS / 25; }
 
+// Squared distance below which a pair takes the inner branch of the pair
+// function for every type pair: min rci^2 (SMA-TB), rc^2 (LJ).
+static double gh_near2(const gh_ctx *c) {
+  if (c->pot == GH_POT_LJ) return c->lj_rc * c->lj_rc;
+  double m = 1e300;
+  for (int a = 0; a < c->nt; ++a)
+    for (int b = 0; b < c->nt; ++b) m = std::min(m, c->tb[a * GH_MAXT + b].rci);
+  return m * m;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
   const int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
   const size_t lds = 25 * (size_t)lds_slots;
+  const double rl = c->rcut + c->skin;
   k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, c->stream>>>(
-      c->status, c->tg, c->pos, c->cellof, c->cell_start, (c->rcut + c->skin) * (c->rcut + c->skin),
+      c->status, c->tg, c->pos, c->cellof, c->cell_start, rl * rl, gh_near2(c),
       count_only ? 0 : cap, c->ncap, c->ncap, lds_slots, c->nn, c->list16, c->xold);
   GH_HIP(c, hipGetLastError());
   return 0;
@@ -668,7 +713,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid) {
   const double ev_ui = ghc::ev_ui();
   const bool single = c->nt == 1;
   const bool mixed = c->precision == GH_MIXED;
-  const size_t slots = (size_t)c->tg.slot_cap;
+  const size_t slots = (size_t)c->tg.slot_cap + 1;  // + the sentinel slot
   if (c->pot == GH_POT_SMATB) {
     const size_t lds1 = gh_tile_slot_bytes(0, c->precision, c->nt) * slots;
     const size_t lds2 = gh_tile_slot_bytes(1, c->precision, c->nt) * slots;
