@@ -61,6 +61,19 @@ def make_ctx(hip, precision, x, L, skin, lcg_seed=4242, T=300.0, dt=0.002):
     return c
 
 
+def profiled_traffic(kernel="k_tb_force_fx"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated against a 1 GiB device copy)."""
+    path = os.path.join(REPO, "profiles", "k_tb_force_fx_au1m_counters.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch %s:" % kernel):
+                return int(ln.split(":")[1]), os.path.relpath(path, REPO)
+    except OSError:
+        pass
+    return None, None
+
+
 def timed_run(c, steps, warmup, timing=True):
     c.run(warmup)
     st0 = c.stats()
@@ -235,6 +248,7 @@ def main():
     t_force = ms_force / cnt_force * 1e-3
     t_rho = ms_rho / cnt_rho * 1e-3
     achieved = BYTES_PASS2 * n / t_force / 1e9
+    traffic, traffic_src = profiled_traffic()
     out = {
         "metric": METRIC,
         "value": value,
@@ -263,7 +277,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_atom": BYTES_PASS2,
             "avg_launch_us": t_force * 1e6,
         },
