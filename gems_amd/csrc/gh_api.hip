@@ -546,7 +546,8 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
         bool ok = true;
         for (int d = 0; d < 3; ++d) ok = ok && b[d] <= c->gn[d];
         const double halo = apc * (bx + 2) * (by + 2) * (bz + 2);
-        if (!ok || halo * 1.15 > max_slots(c)) continue;
+        // bricks under 4 cells stage >8 halo atoms per own atom: never faster
+        if (!ok || halo * 1.15 > max_slots(c) || bx * by * bz < 4) continue;
         for (int d = 0; d < 3; ++d) c->bd_req[d] = b[d];
         c->ready = false;
         if (ensure_ready(c) != 0) continue;

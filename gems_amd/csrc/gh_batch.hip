@@ -18,7 +18,7 @@
 
 #include "gh_device.h"
 
-#define GH_BBLOCK 256
+#define GH_BBLOCK 512
 
 struct gh_batch {
   int device = 0;
@@ -43,8 +43,11 @@ struct gh_batch {
   double *vel = nullptr, *acc = nullptr, *frc = nullptr, *e = nullptr, *rho = nullptr,
          *eband = nullptr, *xold = nullptr, *T = nullptr, *out = nullptr, *scale = nullptr;
   float *eband32 = nullptr;
+  float *gn = nullptr;  // [6][ld] this step's normals (stepa -> stepb)
   unsigned short *list = nullptr;
   TbDev *tbd = nullptr;
+  TbF *tbf = nullptr;
+  int nst = 0;  // LDS staging slots per workgroup (largest system + sentinel)
 };
 
 namespace {
@@ -55,13 +58,17 @@ struct BatchArgs {
   double4 *pos;
   double *vel, *acc, *frc, *e, *rho, *eband, *xold;
   float *eband32;
+  float *gn;
   int *nn;
   unsigned short *list;
   int cap, ld;
   const TbDev *tbd;
+  const TbF *tbf;
   int nt;
   double om[GH_MAXT], osm[GH_MAXT];
   double rl2;
+  float near2f, lo2f, hi2f;  // build: inner-branch bound, FP32 prefilter band
+  int nst;                   // LDS staging slots (largest system + sentinel); 0 = none
   double ev_ui;
   const double *T;
   double *out;  // per system: [0] epot ui, [1] ekin ui, [2] rebuilds, [3] fmax eV/A,
@@ -70,10 +77,10 @@ struct BatchArgs {
 };
 
 #define OUTW 6
+#define GH_BW (GH_BBLOCK / 64)
 
 struct BlockRed {
-  double v[GH_BBLOCK / 64][8];  // up to 8 values per reduction
-  int flag;
+  double v[GH_BW][8];  // up to 8 values per reduction
 };
 
 // Block-wide sums of NV values; the result is returned to every thread.
@@ -90,7 +97,7 @@ __device__ __forceinline__ void bsum(BlockRed &R, double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int w = 0; w < GH_BBLOCK / 64; ++w) s += R.v[w][k];
+    for (int w = 0; w < GH_BW; ++w) s += R.v[w][k];
     v[k] = s;
   }
   __syncthreads();
@@ -102,26 +109,87 @@ __device__ __forceinline__ double bmax(BlockRed &R, double v) {
   if ((threadIdx.x & 63) == 0) R.v[threadIdx.x >> 6][0] = v;
   __syncthreads();
   double m = R.v[0][0];
-  for (int w = 1; w < GH_BBLOCK / 64; ++w) m = fmax(m, R.v[w][0]);
+  for (int w = 1; w < GH_BW; ++w) m = fmax(m, R.v[w][0]);
   __syncthreads();
   return m;
 }
 
-// verlet_self list of one system (all pairs, rd <= rl^2, i excluded).
-__device__ void sys_build(const BatchArgs &B, int a0, int a1) {
-  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+// Per-workgroup state of one system: its atom range, the LDS staging of its
+// positions (fixed point around its first atom) and the parameter tables.
+struct Sys {
+  int a0, n;
+  bool staged;          // positions are in q (all within 64 A of the origin)
+  int4 *q;              // [n + 1]: positions, payload; q[n] = sentinel
+  unsigned char *ty;    // [n + 1] types
+  const TbDev *sp;      // FP64 parameters
+  const TbF *sf;        // FP32 parameters
+};
+
+// Stages positions (and types) of the system; `staged` is uniform.
+__device__ void sys_stage(const BatchArgs &B, Sys &S, int *flag, bool multi) {
+  if (B.nst == 0) {
+    S.staged = false;
+    return;
+  }
+  const double4 o = B.pos[S.a0];
+  if (threadIdx.x == 0) *flag = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < S.n; k += blockDim.x) {
+    const double4 p = B.pos[S.a0 + k];
+    const double dx = (p.x - o.x) * GH_FXS, dy = (p.y - o.y) * GH_FXS,
+                 dz = (p.z - o.z) * GH_FXS;
+    if (fabs(dx) >= GH_FXLIM || fabs(dy) >= GH_FXLIM || fabs(dz) >= GH_FXLIM) *flag = 1;
+    const int t = B.type[S.a0 + k];
+    S.q[k] = make_int4(__double2int_rn(dx), __double2int_rn(dy), __double2int_rn(dz), t);
+    if (multi) S.ty[k] = (unsigned char)t;
+  }
+  if (threadIdx.x == 0) {
+    S.q[S.n] = make_int4(GH_FXFAR, GH_FXFAR, GH_FXFAR, 0);
+    if (multi) S.ty[S.n] = 0;
+  }
+  __syncthreads();
+  S.staged = *flag == 0;
+  __syncthreads();
+}
+
+// verlet_self list of one system (Neighbour.F90:676-759: all pairs with
+// rd <= rl^2, i excluded; rd rounded as the reference's dot_product).  Rows
+// hold system-local indices, the pairs with r^2 < near2 first; the sentinel
+// index is n.  Staged: an FP32 prefilter on the exact fixed-point differences
+// decides all pairs but those within the band [lo2f, hi2f] around rl^2, which
+// the FP64 rd decides.  Ends with a barrier.
+__device__ void sys_build(const BatchArgs &B, const Sys &S) {
+  const int a0 = S.a0, n = S.n;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int i = a0 + k;
     const double4 pi = B.pos[i];
-    int m = 0;
-    for (int j = a0; j < a1; ++j) {
-      if (j == i) continue;
-      const double4 pj = B.pos[j];
-      if (dist2_rn(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) <= B.rl2) {
-        if (m < B.cap) B.list[(size_t)m * B.ld + i] = (unsigned short)(j - a0);
-        ++m;
+    RowWriter row{reinterpret_cast<unsigned long long *>(B.list), B.ld, i, B.cap, 0, 0ull};
+    if (S.staged) {
+      const int4 qi = S.q[k];
+      for (int sweep = 0; sweep < 2; ++sweep)
+        for (int j = 0; j < n; ++j) {
+          if (j == k) continue;
+          const int4 qj = S.q[j];
+          const float ux = (float)(qj.x - qi.x), uy = (float)(qj.y - qi.y),
+                      uz = (float)(qj.z - qi.z);
+          const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (float)(1.0 / GH_FXS2);
+          if (r2f > B.hi2f || ((r2f < B.near2f) != (sweep == 0))) continue;
+          if (r2f >= B.lo2f) {
+            const double4 pj = B.pos[a0 + j];
+            if (dist2_rn(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) > B.rl2) continue;
+          }
+          row.put(j);
+        }
+    } else {
+      for (int j = 0; j < n; ++j) {
+        if (j == k) continue;
+        const double4 pj = B.pos[a0 + j];
+        if (dist2_rn(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) <= B.rl2) row.put(j);
       }
     }
-    if (m > B.cap) B.flags[0] = 1;
-    B.nn[i] = min(m, B.cap);
+    row.finish(n);
+    if (row.m > B.cap) B.flags[0] = 1;
+    B.nn[i] = min(row.m, B.cap);
     B.xold[i] = pi.x;
     B.xold[B.ld + i] = pi.y;
     B.xold[2 * B.ld + i] = pi.z;
@@ -129,29 +197,48 @@ __device__ void sys_build(const BatchArgs &B, int a0, int a1) {
   __syncthreads();
 }
 
-// Both SMA-TB passes of one system; ends with a barrier.
+// Both SMA-TB passes of one system (TB.F90:367-586); ends with a barrier.
+// MIXED and staged: LDS fixed-point pair vectors and the FP32 pair forms;
+// otherwise FP64 positions from global memory (L1/L2 resident per system).
 template <bool MIXED, bool SINGLE>
-__device__ void sys_forces(const BatchArgs &B, const TbDev *sp, int a0, int a1) {
-  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
-    const double4 pi = B.pos[i];
+__device__ void sys_forces(const BatchArgs &B, const Sys &S) {
+  const int a0 = S.a0, n = S.n;
+  const bool fx = MIXED && S.staged;
+  // pass 1: density
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int i = a0 + k;
     const int ti = SINGLE ? 0 : B.type[i];
-    const int m = B.nn[i];
     double acc = 0.0;
-    for (int k = 0; k < m; ++k) {
-      const int j = a0 + B.list[(size_t)k * B.ld + i];
-      const double4 pj = B.pos[j];
-      const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + B.type[j]];
-      const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
-      if (MIXED) {
-        const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-        if (r2 > P.rce2) continue;
-        const float r2f = (float)r2;
-        acc += (double)tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
-      } else {
-        const double dr = sqrt(dist2_rn(vx, vy, vz));
-        if (dr > P.rce) continue;
-        acc += tb_band64(P, dr);
-      }
+    if (fx) {
+      const int4 qi = S.q[k];
+      for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
+        const int4 q = S.q[s];
+        const double dx = (double)(q.x - qi.x), dy = (double)(q.y - qi.y),
+                     dz = (double)(q.z - qi.z);
+        float band;
+        if (tb_rho_fx(SINGLE ? S.sf[0] : S.sf[ti * GH_MAXT + q.w],
+                      fma(dx, dx, fma(dy, dy, dz * dz)), band))
+          acc += (double)band;
+      });
+    } else {
+      const double4 pi = B.pos[i];
+      for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
+        if (s >= n) return;  // sentinel
+        const int j = a0 + s;
+        const double4 pj = B.pos[j];
+        const TbDev &P = SINGLE ? S.sp[0] : S.sp[ti * GH_MAXT + B.type[j]];
+        const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+        if (MIXED) {
+          const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+          if (r2 <= P.rce2) {
+            const float r2f = (float)r2;
+            acc += (double)tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
+          }
+        } else {
+          const double dr = sqrt(dist2_rn(vx, vy, vz));
+          if (dr <= P.rce) acc += tb_band64(P, dr);
+        }
+      });
     }
     B.rho[i] = acc;
     const double eb = 1.0 / sqrt(acc);
@@ -159,59 +246,101 @@ __device__ void sys_forces(const BatchArgs &B, const TbDev *sp, int a0, int a1) 
     B.eband32[i] = (float)eb;
   }
   __syncthreads();
-  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
-    const double4 pi = B.pos[i];
+  if (fx) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) S.q[k].w = __float_as_int(B.eband32[a0 + k]);
+    if (threadIdx.x == 0) S.q[n].w = 0;
+    __syncthreads();
+  }
+  // pass 2: forces and per-atom energies
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int i = a0 + k;
     const int ti = SINGLE ? 0 : B.type[i];
-    const int m = B.nn[i];
-    double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
-    const double ebi = B.eband[i];
-    const float ebi32 = B.eband32[i];
-    for (int k = 0; k < m; ++k) {
-      const int j = a0 + B.list[(size_t)k * B.ld + i];
-      const double4 pj = B.pos[j];
-      const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + B.type[j]];
-      const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
-      if (MIXED) {
-        const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-        if (r2 > P.rce2) continue;
-        const float r2f = (float)r2;
-        const float rinv = __builtin_amdgcn_rsqf(r2f);
-        float factor, repul;
-        tb_pair32(P, r2f * rinv, ebi32 + B.eband32[j], factor, repul);
-        const double sf = (double)(factor * rinv);
-        fx = fma(-sf, vx, fx);
-        fy = fma(-sf, vy, fy);
-        fz = fma(-sf, vz, fz);
-        ei += (double)repul;
-      } else {
-        const double dr = sqrt(dist2_rn(vx, vy, vz));
-        if (dr > P.rce) continue;
-        double factor, repul;
-        tb_pair64(P, dr, ebi + B.eband[j], factor, repul);
-        const double sf = factor / dr;
-        fx -= sf * vx;
-        fy -= sf * vy;
-        fz -= sf * vz;
-        ei += repul;
-      }
+    double fxs = 0.0, fys = 0.0, fzs = 0.0, ei = 0.0, scale = B.ev_ui;
+    if (fx) {
+      const int4 qi = S.q[k];
+      const float ebi = __int_as_float(qi.w);
+      for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
+        const int4 q = S.q[s];
+        const double dx = (double)(q.x - qi.x), dy = (double)(q.y - qi.y),
+                     dz = (double)(q.z - qi.z);
+        float sf, repul;
+        if (tb_force_fx(SINGLE ? S.sf[0] : S.sf[ti * GH_MAXT + S.ty[s]],
+                        fma(dx, dx, fma(dy, dy, dz * dz)), ebi + __int_as_float(q.w), sf,
+                        repul)) {
+          const double sfd = (double)sf;
+          fxs = fma(-sfd, dx, fxs);
+          fys = fma(-sfd, dy, fys);
+          fzs = fma(-sfd, dz, fzs);
+          ei += (double)repul;
+        }
+      });
+      scale = B.ev_ui * (1.0 / GH_FXS);
+    } else {
+      const double4 pi = B.pos[i];
+      const double ebi = B.eband[i];
+      const float ebi32 = B.eband32[i];
+      for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
+        if (s >= n) return;  // sentinel
+        const int j = a0 + s;
+        const double4 pj = B.pos[j];
+        const TbDev &P = SINGLE ? S.sp[0] : S.sp[ti * GH_MAXT + B.type[j]];
+        const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+        if (MIXED) {
+          const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
+          if (r2 <= P.rce2) {
+            const float r2f = (float)r2;
+            const float rinv = __builtin_amdgcn_rsqf(r2f);
+            float factor, repul;
+            tb_pair32(P, r2f * rinv, ebi32 + B.eband32[j], factor, repul);
+            const double sf = (double)(factor * rinv);
+            fxs = fma(-sf, vx, fxs);
+            fys = fma(-sf, vy, fys);
+            fzs = fma(-sf, vz, fzs);
+            ei += (double)repul;
+          }
+        } else {
+          const double dr = sqrt(dist2_rn(vx, vy, vz));
+          if (dr <= P.rce) {
+            double factor, repul;
+            tb_pair64(P, dr, ebi + B.eband[j], factor, repul);
+            const double sf = factor / dr;
+            fxs -= sf * vx;
+            fys -= sf * vy;
+            fzs -= sf * vz;
+            ei += repul;
+          }
+        }
+      });
     }
-    B.frc[i] = fx * B.ev_ui;
-    B.frc[B.ld + i] = fy * B.ev_ui;
-    B.frc[2 * B.ld + i] = fz * B.ev_ui;
+    B.frc[i] = fxs * scale;
+    B.frc[B.ld + i] = fys * scale;
+    B.frc[2 * B.ld + i] = fzs * scale;
     B.e[i] = (ei - sqrt(B.rho[i])) * B.ev_ui;
   }
   __syncthreads();
 }
 
-template <bool SINGLE>
-__device__ __forceinline__ void load_sp(TbDev *sp, const BatchArgs &B) {
-  if (SINGLE) {
-    if (threadIdx.x == 0) sp[0] = B.tbd[0];
-  } else {
-    load_params(sp, B.tbd, B.nt);
-  }
+// Shared tables and the system of this workgroup.
+#define SYS_SETUP(SINGLE)                                                                  \
+  extern __shared__ int4 dyn_q[];                                                          \
+  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];                                     \
+  __shared__ TbF sf[SINGLE ? 1 : GH_MAXT * GH_MAXT];                                       \
+  __shared__ BlockRed R;                                                                   \
+  __shared__ int sflag;                                                                    \
+  for (int k_ = threadIdx.x; k_ < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k_ += blockDim.x) {    \
+    sp[k_] = B.tbd[k_];                                                                    \
+    sf[k_] = B.tbf[k_];                                                                    \
+  }                                                                                        \
+  const int s = B.order[blockIdx.x];                                                       \
+  Sys S;                                                                                   \
+  S.a0 = B.off[s];                                                                         \
+  S.n = B.off[s + 1] - S.a0;                                                               \
+  S.q = dyn_q;                                                                             \
+  S.ty = reinterpret_cast<unsigned char *>(dyn_q + B.nst);                                 \
+  S.sp = sp;                                                                               \
+  S.sf = sf;                                                                               \
+  S.staged = false;                                                                        \
   __syncthreads();
-}
 
 // epot (ui) and ekin about the centre of mass (ui) of one system.
 __device__ void sys_energies(const BatchArgs &B, BlockRed &R, int a0, int a1, double &epot,
@@ -240,15 +369,12 @@ __device__ void sys_energies(const BatchArgs &B, BlockRed &R, int a0, int a1, do
 
 template <bool MIXED, bool SINGLE>
 __global__ __launch_bounds__(GH_BBLOCK) void k_batch_compute(BatchArgs B) {
-  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
-  __shared__ BlockRed R;
-  load_sp<SINGLE>(sp, B);
-  const int s = B.order[blockIdx.x];
-  const int a0 = B.off[s], a1 = B.off[s + 1];
-  sys_build(B, a0, a1);
-  sys_forces<MIXED, SINGLE>(B, sp, a0, a1);
+  SYS_SETUP(SINGLE)
+  sys_stage(B, S, &sflag, !SINGLE);
+  sys_build(B, S);
+  sys_forces<MIXED, SINGLE>(B, S);
   double ep, ek;
-  sys_energies(B, R, a0, a1, ep, ek);
+  sys_energies(B, R, S.a0, S.a0 + S.n, ep, ek);
   if (threadIdx.x == 0) {
     B.out[s * OUTW + 0] = ep;
     B.out[s * OUTW + 1] = ek;
@@ -267,11 +393,8 @@ __global__ __launch_bounds__(GH_BBLOCK) void k_batch_langevin(BatchArgs B, int n
                                                               long long step0, ErmakC k,
                                                               unsigned long long seed,
                                                               double skin) {
-  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
-  __shared__ BlockRed R;
-  load_sp<SINGLE>(sp, B);
-  const int s = B.order[blockIdx.x];
-  const int a0 = B.off[s], a1 = B.off[s + 1];
+  SYS_SETUP(SINGLE)
+  const int a0 = S.a0, a1 = S.a0 + S.n;
   const double skt = sqrt(k.kB_ui * B.T[s]);
   int nreb = 0;
   for (int st = 0; st < nsteps; ++st) {
@@ -280,6 +403,8 @@ __global__ __launch_bounds__(GH_BBLOCK) void k_batch_langevin(BatchArgs B, int n
     for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
       double g[6];
       philox_normals(seed, step, i, g);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) B.gn[q * B.ld + i] = (float)g[q];  // exact: FP32 normals
       const double f_lan = skt * B.osm[B.type[i]];
       double4 p = B.pos[i];
       double x[3] = {p.x, p.y, p.z};
@@ -296,14 +421,17 @@ __global__ __launch_bounds__(GH_BBLOCK) void k_batch_langevin(BatchArgs B, int n
       B.pos[i] = p;
     }
     const double dmax2 = bmax(R, d2);
+    if (MIXED) sys_stage(B, S, &sflag, !SINGLE);
     if (2.0 * sqrt(dmax2) > skin) {
-      sys_build(B, a0, a1);
+      if (!MIXED) sys_stage(B, S, &sflag, !SINGLE);
+      sys_build(B, S);
       ++nreb;
     }
-    sys_forces<MIXED, SINGLE>(B, sp, a0, a1);
+    sys_forces<MIXED, SINGLE>(B, S);
     for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
       double g[6];
-      philox_normals(seed, step, i, g);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) g[q] = (double)B.gn[q * B.ld + i];  // this step's stepa draw
       const int t = B.type[i];
       const double f_lan = skt * B.osm[t];
       const double om = B.om[t];
@@ -333,10 +461,13 @@ struct FireC {
 };
 
 template <bool SINGLE>
-__device__ __forceinline__ void fire_forces(const BatchArgs &B, const TbDev *sp, int a0, int a1,
-                                            bool mixed) {
-  if (mixed) sys_forces<true, SINGLE>(B, sp, a0, a1);
-  else sys_forces<false, SINGLE>(B, sp, a0, a1);
+__device__ __forceinline__ void fire_forces(const BatchArgs &B, Sys &S, int *flag, bool mixed) {
+  if (mixed) {
+    sys_stage(B, S, flag, !SINGLE);
+    sys_forces<true, SINGLE>(B, S);
+  } else {
+    sys_forces<false, SINGLE>(B, S);
+  }
 }
 
 __device__ double sys_fmax(const BatchArgs &B, BlockRed &R, int a0, int a1) {
@@ -355,16 +486,14 @@ __device__ double sys_fmax(const BatchArgs &B, BlockRed &R, int a0, int a1) {
 // max|F| < fswitch; convergence is max|F| < ftol in FP64.
 template <bool SINGLE>
 __global__ __launch_bounds__(GH_BBLOCK) void k_batch_fire(BatchArgs B, FireC fc, double skin) {
-  __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
-  __shared__ BlockRed R;
-  load_sp<SINGLE>(sp, B);
-  const int s = B.order[blockIdx.x];
-  const int a0 = B.off[s], a1 = B.off[s + 1];
+  SYS_SETUP(SINGLE)
+  const int a0 = S.a0, a1 = S.a0 + S.n;
   bool mixed = fc.mixed != 0;
   for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x)
     for (int d = 0; d < 3; ++d) B.vel[d * B.ld + i] = 0.0;
-  sys_build(B, a0, a1);
-  fire_forces<SINGLE>(B, sp, a0, a1, mixed);
+  sys_stage(B, S, &sflag, !SINGLE);
+  sys_build(B, S);
+  fire_forces<SINGLE>(B, S, &sflag, mixed);
   double dt = fc.dt_max * 0.1, alpha = fc.alpha0;
   int npos = 0, it = 0, conv = 0;
   double fmax = 0.0;
@@ -372,7 +501,7 @@ __global__ __launch_bounds__(GH_BBLOCK) void k_batch_fire(BatchArgs B, FireC fc,
     fmax = sys_fmax(B, R, a0, a1) * fc.ui_ev;  // eV/A
     if (mixed && fmax < fc.fswitch) {
       mixed = false;
-      fire_forces<SINGLE>(B, sp, a0, a1, mixed);
+      fire_forces<SINGLE>(B, S, &sflag, mixed);
       fmax = sys_fmax(B, R, a0, a1) * fc.ui_ev;
     }
     if (!mixed && fmax < fc.ftol) {
@@ -399,8 +528,11 @@ __global__ __launch_bounds__(GH_BBLOCK) void k_batch_fire(BatchArgs B, FireC fc,
       B.pos[i] = p;
     }
     const double dmax2 = bmax(R, d2);
-    if (2.0 * sqrt(dmax2) > skin) sys_build(B, a0, a1);
-    fire_forces<SINGLE>(B, sp, a0, a1, mixed);
+    if (2.0 * sqrt(dmax2) > skin) {
+      sys_stage(B, S, &sflag, !SINGLE);
+      sys_build(B, S);
+    }
+    fire_forces<SINGLE>(B, S, &sflag, mixed);
     double pv[3] = {0.0, 0.0, 0.0};  // F.v, |v|^2, |F|^2
     for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
       const double om = B.om[B.type[i]];
@@ -501,18 +633,30 @@ static BatchArgs bargs(const gh_batch *b) {
   A.eband = b->eband;
   A.xold = b->xold;
   A.eband32 = b->eband32;
+  A.gn = b->gn;
   A.nn = b->nn;
   A.list = b->list;
   A.cap = b->cap;
   A.ld = b->ld;
   A.tbd = b->tbd;
+  A.tbf = b->tbf;
   A.nt = b->nt;
+  A.nst = b->nst;
   for (int t = 0; t < GH_MAXT; ++t) {
     A.om[t] = 1.0 / b->mass_t[t];
     A.osm[t] = sqrt(A.om[t]);
   }
   const double rl = b->rcut + b->skin;
   A.rl2 = rl * rl;
+  // FP32 prefilter of the build: its error on r^2 stays below 1e-4 A^2 for
+  // staged clusters (< 64 A); the band is thirty times that
+  const double margin = std::max(3e-3, 1e-4 * A.rl2);
+  A.lo2f = (float)(A.rl2 - margin);
+  A.hi2f = (float)(A.rl2 + margin);
+  double rci = 1e300;
+  for (int i = 0; i < b->nt; ++i)
+    for (int j = 0; j < b->nt; ++j) rci = std::min(rci, b->tb[i * GH_MAXT + j].rci);
+  A.near2f = (float)(rci * rci);
   A.ev_ui = ghc::ev_ui();
   A.T = b->T;
   A.out = b->out;
@@ -559,9 +703,17 @@ static int batch_ready(gh_batch *b) {
     b->cap = std::max(8, std::min(nmax, est));
     b->cap = (b->cap + 7) / 8 * 8;
     BT(balloc(b, &b->list, (size_t)b->cap * b->ld));
+    // LDS staging of the largest system (int4 + type byte per atom, and the
+    // sentinel) beside the static tables; larger systems use global memory
+    const size_t stat = (b->nt > 1 ? GH_MAXT * GH_MAXT : 1) * (sizeof(TbDev) + sizeof(TbF)) +
+                        sizeof(double) * 8 * (GH_BBLOCK / 64) + 64;
+    const size_t need = 17 * (size_t)(nmax + 1);
+    b->nst = stat + need <= 65536 ? nmax + 1 : 0;
   }
   return 0;
 }
+
+static size_t batch_lds(const gh_batch *b) { return 17 * (size_t)b->nst; }
 
 extern "C" {
 
@@ -591,7 +743,7 @@ void gh_batch_destroy(gh_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   void *p[] = {b->off, b->order, b->type, b->nn, b->flags, b->pos, b->vel, b->acc, b->frc,
                b->e, b->rho, b->eband, b->xold, b->T, b->out, b->scale, b->eband32, b->list,
-               b->tbd};
+               b->tbd, b->tbf, b->gn};
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -640,6 +792,7 @@ int gh_batch_set_systems(gh_batch *b, int nsys, const int *off, const double *x,
   BT(balloc(b, &b->rho, ld));
   BT(balloc(b, &b->eband, ld));
   BT(balloc(b, &b->eband32, ld));
+  BT(balloc(b, &b->gn, 6 * (size_t)ld));
   BT(balloc(b, &b->xold, 3 * (size_t)ld));
   BT(balloc(b, &b->T, nsys));
   BT(balloc(b, &b->out, (size_t)nsys * OUTW));
@@ -682,7 +835,8 @@ int gh_batch_set_smatb(gh_batch *b, int nt, const double *A, const double *xi, c
   if (b->tbd) BH(b, hipFree(b->tbd));
   b->tbd = tmp.tbd;  // ownership moves to the batch
   tmp.tbd = nullptr;
-  if (tmp.tbf) BH(b, hipFree(tmp.tbf));  // the batch kernels use the FP64/FP32 block only
+  if (b->tbf) BH(b, hipFree(b->tbf));
+  b->tbf = tmp.tbf;
   tmp.tbf = nullptr;
   b->nt = nt;
   b->rcut = tmp.rcut;
@@ -734,11 +888,11 @@ int gh_batch_compute(gh_batch *b, double *F, double *epot) {
   const BatchArgs A = bargs(b);
   const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
   if (mixed) {
-    if (single) k_batch_compute<true, true><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
-    else k_batch_compute<true, false><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
+    if (single) k_batch_compute<true, true><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
+    else k_batch_compute<true, false><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
   } else {
-    if (single) k_batch_compute<false, true><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
-    else k_batch_compute<false, false><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A);
+    if (single) k_batch_compute<false, true><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
+    else k_batch_compute<false, false><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
   }
   BH(b, hipGetLastError());
   BT(check_flags(b));
@@ -778,7 +932,8 @@ int gh_batch_langevin(gh_batch *b, int nsteps, double dt, double gamma, uint64_t
   const BatchArgs A = bargs(b);
   const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
 #define LAUNCH(MX, SG)                                                                 \
-  k_batch_langevin<MX, SG><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A, nsteps, b->step + 1, k, \
+  k_batch_langevin<MX, SG><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A, nsteps,       \
+                                                                           b->step + 1, k,  \
                                                                  seed, b->skin)
   if (mixed) {
     if (single) LAUNCH(true, true); else LAUNCH(true, false);
@@ -860,8 +1015,10 @@ int gh_batch_fire(gh_batch *b, int max_iter, double dt_max, double alpha0, doubl
   fc.max_iter = max_iter;
   fc.mixed = b->precision == GH_MIXED;
   const BatchArgs A = bargs(b);
-  if (b->nt == 1) k_batch_fire<true><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A, fc, b->skin);
-  else k_batch_fire<false><<<b->nsys, GH_BBLOCK, 0, b->stream>>>(A, fc, b->skin);
+  if (b->nt == 1)
+    k_batch_fire<true><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A, fc, b->skin);
+  else
+    k_batch_fire<false><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A, fc, b->skin);
   BH(b, hipGetLastError());
   BT(check_flags(b));
   b->lists_valid = true;

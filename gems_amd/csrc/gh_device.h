@@ -105,6 +105,96 @@ __device__ __forceinline__ void tb_pair32(const TP &P, float r, float ebs, float
   }
 }
 
+// ---------------------------------------------------------------------------
+// Neighbour rows ("quad ELL"): entry k of atom a is the 16-bit word
+// ((k / 4) * ld + a) * 4 + k % 4, so one 8-byte load per lane fetches four
+// consecutive entries.  Rows end with a padded quad whose spare entries name
+// a sentinel slot far from every atom.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ld_quad(const unsigned long long *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// Fixed-point positions (GH_FXS steps per Angstrom).  Staged atoms lie within
+// 2^29 steps (64 A) of their frame's origin and the sentinel at 2^30, so no
+// difference of two staged values overflows 32 bits.
+__device__ __forceinline__ int fxq(double d) { return __double2int_rn(d * GH_FXS); }
+#define GH_FXFAR (1 << 30)
+#define GH_FXLIM 536870912.0  // 2^29
+#define GH_FAR 1.0e30         // sentinel slot, FP64
+
+struct RowWriter {  // quad-packed row of one atom
+  unsigned long long *lq;
+  int ld, a, cap, m;
+  unsigned long long w;
+  __device__ void put(int s) {
+    w |= (unsigned long long)s << (16 * (m & 3));
+    if ((m & 3) == 3) {
+      if (m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
+      w = 0;
+    }
+    ++m;
+  }
+  __device__ void finish(int pad) {
+    if ((m & 3) && m < cap) {
+      for (int k = m & 3; k < 4; ++k) w |= (unsigned long long)pad << (16 * k);
+      lq[(size_t)(m >> 2) * ld + a] = w;
+    }
+  }
+};
+
+
+// Runs f(slot) over an atom's list, one 8-byte load per four entries; the
+// row's last quad is padded with the sentinel slot, so every entry of every
+// quad is processed without a bounds test.  The loads run two quads ahead of
+// use and are unconditional (the index is clamped to the row's last quad), so
+// no wait on the list stream sits between a quad's load and its use.
+template <class F>
+__device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__ list16, int ld,
+                                              int a, int m, F &&f) {
+  const int ng = (m + 3) >> 2;
+  if (ng == 0) return;
+  const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
+  const size_t sld = (size_t)ld;
+  unsigned long long w0 = ld_quad(lp);
+  unsigned long long w1 = ld_quad(lp + (size_t)min(1, ng - 1) * sld);
+  for (int g = 0; g < ng; g += 2) {
+    const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
+    const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) f((int)((w0 >> (16 * u)) & 0xFFFFull));
+    if (g + 1 < ng) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) f((int)((w1 >> (16 * u)) & 0xFFFFull));
+    }
+    w0 = n0;
+    w1 = n1;
+  }
+}
+
+
+// SMA-TB pair terms on fixed-point pair vectors (integer steps, exact in
+// FP64): r^2 in FP64, then the FP32 forms.  rho: the density term;
+// force: factor * (1/r) per step of the pair vector, and the repulsion.
+template <class TP>
+__device__ __forceinline__ bool tb_rho_fx(const TP &P, double r2q, float &band) {
+  if (r2q > P.rce2q) return false;
+  const float r2f = (float)(r2q * (1.0 / GH_FXS2));
+  band = tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
+  return true;
+}
+template <class TP>
+__device__ __forceinline__ bool tb_force_fx(const TP &P, double r2q, float ebs, float &sf,
+                                            float &repul) {
+  if (r2q > P.rce2q) return false;
+  const float r2f = (float)(r2q * (1.0 / GH_FXS2));
+  const float rinv = __builtin_amdgcn_rsqf(r2f);
+  float factor;
+  tb_pair32(P, r2f * rinv, ebs, factor, repul);
+  sf = factor * rinv;
+  return true;
+}
+
 // Philox4x32-10 (Salmon et al., SC'11): counter-based stream.
 __device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
 #pragma unroll
@@ -120,31 +210,24 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
   return c;
 }
 
-__device__ __forceinline__ double u01(unsigned a, unsigned b) {
-  // 53-bit uniform in (0,1)
-  const unsigned long long v = ((unsigned long long)a << 21) ^ (unsigned long long)b;
-  return ((double)(v & ((1ull << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-// Six standard normals for atom `id` at step `step` (Box-Muller).
+// Six standard normals for atom `id` at step `step`: two Philox4x32-10 draws
+// (counter (id, b, step), key = seed) give 24-bit uniforms, turned into three
+// Box-Muller pairs in FP32 with the fast hardware log and sin/cos (the noise
+// needs no more precision than FP32; its tail reaches 5.8 sigma).
 __device__ __forceinline__ void philox_normals(unsigned long long seed, long long step, int id,
                                                double g[6]) {
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  double u[6];
+  const unsigned s0 = (unsigned)step, s1 = (unsigned)((unsigned long long)step >> 32);
+  const uint4 c0 = philox(make_uint4((unsigned)id, 0u, s0, s1), key);
+  const uint4 c1 = philox(make_uint4((unsigned)id, 1u, s0, s1), key);
+  const unsigned w[6] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y};
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    const uint4 c = philox(make_uint4((unsigned)id, (unsigned)b, (unsigned)step,
-                                      (unsigned)((unsigned long long)step >> 32)), key);
-    u[2 * b] = u01(c.x, c.y);
-    u[2 * b + 1] = u01(c.z, c.w);
-  }
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const double r = sqrt(-2.0 * log(u[2 * b]));
-    double sn, cs;
-    sincospi(2.0 * u[2 * b + 1], &sn, &cs);
-    g[2 * b] = r * cs;
-    g[2 * b + 1] = r * sn;
+    const float u1 = ((float)(w[2 * b] >> 8) + 0.5f) * 5.9604644775390625e-08f;  // (0,1)
+    const float a = (float)(w[2 * b + 1] >> 8) * 3.7450702829239286e-07f;        // 2 pi u
+    const float r = __builtin_sqrtf(-2.0f * __logf(u1));
+    g[2 * b] = (double)(r * __cosf(a));
+    g[2 * b + 1] = (double)(r * __sinf(a));
   }
 }
 

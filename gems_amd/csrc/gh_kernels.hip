@@ -134,32 +134,51 @@ __global__ void k_bin(GhStatus *st, double4 *__restrict__ pos, int n, TileGeom t
   }
 }
 
-// Exclusive scan of the cell counts in one 1024-thread workgroup.
+// Exclusive scan of the cell counts in one 1024-thread workgroup, in tiles of
+// 4096 keys: each thread scans four consecutive counts (one coalesced 16-B
+// load), wavefront prefix sums and a scan of the 16 wave totals give the
+// tile's offsets, and a running carry links the tiles.
 __global__ __launch_bounds__(1024) void k_scan(const GhStatus *st, const int *__restrict__ count,
                                                int *__restrict__ start, int *__restrict__ cursor,
                                                int ncell) {
   if (!st->rebuild) return;
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (ncell + 1023) / 1024;
-  const int b = t * per, e = min(b + per, ncell);
-  int s = 0;
-  for (int c = b; c < e; ++c) s += count[c];
-  part[t] = s;
+  __shared__ int wsum[16];
+  __shared__ int carry_s;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry_s = 0;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = (t >= o) ? part[t - o] : 0;
+  for (int base = 0; base < ncell; base += 4096) {
+    const int k0 = base + 4 * t;
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = k0 + u < ncell ? count[k0 + u] : 0;
+    const int tot = v[0] + v[1] + v[2] + v[3];
+    int inc = tot;  // inclusive prefix over the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wv] = inc;
     __syncthreads();
-    part[t] += v;
+    int woff = 0, tile = 0;
+    for (int w = 0; w < 16; ++w) {
+      const int x = wsum[w];
+      if (w < wv) woff += x;
+      tile += x;
+    }
+    int run = carry_s + woff + inc - tot;  // exclusive prefix of this thread's four
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < ncell) {
+        start[k0 + u] = run;
+        cursor[k0 + u] = run;
+        run += v[u];
+      }
+    __syncthreads();
+    if (t == 0) carry_s += tile;
     __syncthreads();
   }
-  int run = part[t] - s;  // exclusive prefix of this thread's chunk
-  for (int c = b; c < e; ++c) {
-    start[c] = run;
-    cursor[c] = run;
-    run += count[c];
-  }
-  if (t == 1023) start[ncell] = part[1023];
+  if (t == 0) start[ncell] = carry_s;
 }
 
 __global__ void k_scatter(const GhStatus *st, const int *__restrict__ cellid, int n,

@@ -151,37 +151,36 @@ __device__ __forceinline__ int halo_of_slot(const Halo &H, int s) {
 __device__ __forceinline__ size_t qidx(int k, int a, int ld) {
   return (((size_t)(k >> 2) * ld + a) << 2) + (k & 3);
 }
-__device__ __forceinline__ unsigned long long ld_quad(const unsigned long long *p) {
-  return __builtin_nontemporal_load(p);
-}
-__device__ __forceinline__ int fxq(double d) { return __double2int_rn(d * GH_FXS); }
-#define GH_FXFAR (1 << 30)  // sentinel slot, fixed point: > 100 A from any staged atom
-#define GH_FAR 1.0e30       // sentinel slot, FP64
+
 
 // ---------------------------------------------------------------------------
 // Neighbour list (linkedcell_self, Neighbour.F90:991-1087): for each atom of
 // the brick, the slots of its 27 neighbour cells with rd < rl^2, the atom
-// itself excluded.  vd = (x_j - x_i) + image shift, rounded exactly as the
-// reference's vdistance + dot_product, so the sets are bit-identical.
+// itself excluded, where rd is rounded exactly as the reference's vdistance +
+// dot_product of vd = (x_j - x_i) + image shift, so the sets are bit-identical.
 //
-// Row order: two sweeps of the 27 cells, first the pairs with rd < near2
-// (inside the potential's inner branch: r < rci for SMA-TB, r < rc for LJ),
-// then the rest, each sweep in stencil order.  Lanes of a wavefront then take
-// the same branch of the pair function on most iterations.  The last quad is
-// padded with `pad`, a sentinel LDS slot far from every atom.
-// Positions are staged in LDS when the halo fits GH_BUILD_LDS (STAGE), else
-// read from L2.
+// One sweep.  With the halo staged (STAGE), a candidate is first tested in
+// FP32 on brick-local coordinates; only candidates within `margin` of rl^2
+// (FP32 error bound x 10) are decided by the exact FP64 rd from L2.
+// Row order: pairs with r^2 < near2 (the potential's inner branch: r < rci
+// for SMA-TB, r < rc for LJ) first, then the rest, which wait in an LDS
+// buffer (GH_FARMAX per thread; any excess joins the first run, which only
+// costs order).  Lanes of a wavefront then take the same branch of the pair
+// function on most iterations.  The last quad is padded with `pad`, a
+// sentinel LDS slot far from every atom.
 // ---------------------------------------------------------------------------
+#define GH_FARMAX 48
+
 template <bool STAGE>
 __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
                                            const double4 *__restrict__ pos,
-                                           const int *__restrict__ cellof, const double *px,
-                                           const double *py, const double *pz, double rl2,
-                                           double near2, int cap, int pad, int ld, int ncap,
-                                           int *__restrict__ nn,
+                                           const int *__restrict__ cellof, const float *xf,
+                                           const float *yf, const float *zf,
+                                           unsigned short *far, double rl2, float near2f,
+                                           float lo2f, float hi2f, int cap, int pad, int ld,
+                                           int ncap, int *__restrict__ nn,
                                            unsigned short *__restrict__ list16,
                                            double *__restrict__ xold, int &mmax) {
-  unsigned long long *lq = reinterpret_cast<unsigned long long *>(list16);
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const int lc = cellof[a] - H.key0;
     const int lx = lc % t.bd[0], ly = (lc / t.bd[0]) % t.bd[1], lz = lc / (t.bd[0] * t.bd[1]);
@@ -189,84 +188,90 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
     const int own = a - H.delta[ha];
     const double4 pa = pos[a];
     const double xi = pa.x, yi = pa.y, zi = pa.z;
-    int m = 0;
-    unsigned long long w = 0;
-    for (int sweep = 0; sweep < 2; ++sweep)
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx) {
-            const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
-            const double sx = shift(t, H, 0, h), sy = shift(t, H, 1, h), sz = shift(t, H, 2, h);
-            const int se = H.hbase[h + 1];
-            const int dl = H.delta[h];
-            for (int s = H.hbase[h]; s < se; ++s) {
-              if (s == own) continue;
-              double xj, yj, zj;
-              if (STAGE) {
-                xj = px[s];
-                yj = py[s];
-                zj = pz[s];
-              } else {
-                const double4 pj = pos[s + dl];
-                xj = pj.x;
-                yj = pj.y;
-                zj = pj.z;
-              }
-              const double vx = (xj - xi) + sx;
-              const double vy = (yj - yi) + sy;
-              const double vz = (zj - zi) + sz;
-              const double rd = dist2_rn(vx, vy, vz);
-              if (rd < rl2 && ((rd < near2) == (sweep == 0))) {
-                w |= (unsigned long long)s << (16 * (m & 3));
-                if ((m & 3) == 3) {
-                  if (m < cap) lq[(size_t)(m >> 2) * ld + a] = w;
-                  w = 0;
-                }
-                ++m;
-              }
-            }
-          }
-    if ((m & 3) && m < cap) {
-      for (int k = m & 3; k < 4; ++k) w |= (unsigned long long)pad << (16 * k);
-      lq[(size_t)(m >> 2) * ld + a] = w;
+    float xfi = 0.f, yfi = 0.f, zfi = 0.f;
+    if (STAGE) {
+      xfi = xf[own];
+      yfi = yf[own];
+      zfi = zf[own];
     }
-    nn[a] = min(m, cap);
+    RowWriter row{reinterpret_cast<unsigned long long *>(list16), ld, a, cap, 0, 0ull};
+    int nfar = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
+          const int se = H.hbase[h + 1];
+          const int dl = H.delta[h];
+          for (int s = H.hbase[h]; s < se; ++s) {
+            if (s == own) continue;
+            bool near;
+            if (STAGE) {
+              const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
+              const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+              if (r2f > hi2f) continue;
+              if (r2f >= lo2f) {  // near the cut: the reference's rounding decides
+                const double4 pj = pos[s + dl];
+                const double vx = (pj.x - xi) + shift(t, H, 0, h);
+                const double vy = (pj.y - yi) + shift(t, H, 1, h);
+                const double vz = (pj.z - zi) + shift(t, H, 2, h);
+                if (!(dist2_rn(vx, vy, vz) < rl2)) continue;
+              }
+              near = r2f < near2f;
+            } else {
+              const double4 pj = pos[s + dl];
+              const double vx = (pj.x - xi) + shift(t, H, 0, h);
+              const double vy = (pj.y - yi) + shift(t, H, 1, h);
+              const double vz = (pj.z - zi) + shift(t, H, 2, h);
+              const double rd = dist2_rn(vx, vy, vz);
+              if (!(rd < rl2)) continue;
+              near = rd < (double)near2f;
+            }
+            if (near || nfar == GH_FARMAX) row.put(s);
+            else far[(nfar++) * GH_TBLOCK + threadIdx.x] = (unsigned short)s;
+          }
+        }
+    for (int k = 0; k < nfar; ++k) row.put(far[k * GH_TBLOCK + threadIdx.x]);
+    row.finish(pad);
+    nn[a] = min(row.m, cap);
     xold[a] = xi;
     xold[ncap + a] = yi;
     xold[2 * ncap + a] = zi;
-    mmax = max(mmax, m);
+    mmax = max(mmax, row.m);
   }
 }
 
 __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
-    const int *__restrict__ cell_start, double rl2, double near2, int cap, int ld, int ncap,
-    int lds_slots, int *__restrict__ nn, unsigned short *__restrict__ list16,
-    double *__restrict__ xold) {
+    const int *__restrict__ cell_start, double rl2, float near2f, float lo2f, float hi2f,
+    int cap, int ld, int ncap, int lds_slots, int *__restrict__ nn,
+    unsigned short *__restrict__ list16, double *__restrict__ xold) {
   if (!st->rebuild) return;
-  extern __shared__ double dyn[];
+  extern __shared__ float dynf[];
   __shared__ Halo H;
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   int mmax = 0;
   const int pad = t.slot_cap;  // sentinel slot of the force passes
+  unsigned short *far = reinterpret_cast<unsigned short *>(dynf);
+  float *xf = dynf + GH_FARMAX * GH_TBLOCK / 2;
   if (H.total <= lds_slots) {
-    double *px = dyn, *py = dyn + lds_slots, *pz = dyn + 2 * lds_slots;
-    unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * lds_slots);
+    float *yf = xf + lds_slots, *zf = xf + 2 * lds_slots;
+    unsigned char *scell = reinterpret_cast<unsigned char *>(xf + 3 * lds_slots);
     slot_map(H, scell);
     for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-      const double4 p = pos[s + H.delta[scell[s]]];
-      px[s] = p.x;
-      py[s] = p.y;
-      pz[s] = p.z;
+      const int h = scell[s];
+      const double4 p = pos[s + H.delta[h]];
+      xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
+      yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
+      zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
     }
     __syncthreads();
-    build_rows<true>(H, t, pos, cellof, px, py, pz, rl2, near2, cap, pad, ld, ncap, nn, list16,
-                     xold, mmax);
+    build_rows<true>(H, t, pos, cellof, xf, yf, zf, far, rl2, near2f, lo2f, hi2f, cap, pad, ld,
+                     ncap, nn, list16, xold, mmax);
   } else {
-    build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, rl2, near2, cap, pad, ld,
-                      ncap, nn, list16, xold, mmax);
+    build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, rl2, near2f, lo2f,
+                      hi2f, cap, pad, ld, ncap, nn, list16, xold, mmax);
   }
   mmax = wave_imax(mmax);
   if ((threadIdx.x & 63) == 0 && mmax > 0) {
@@ -303,34 +308,6 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
       const int s = list16[qidx(k, a, ld)];
       dst[(size_t)k * ld + a] = s + H.delta[halo_of_slot(H, s)];
     }
-}
-
-// Runs f(slot) over an atom's list, one 8-byte load per four entries; the
-// row's last quad is padded with the sentinel slot, so every entry of every
-// quad is processed without a bounds test.  The loads run two quads ahead of
-// use and are unconditional (the index is clamped to the row's last quad), so
-// no wait on the list stream sits between a quad's load and its use.
-template <class F>
-__device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__ list16, int ld,
-                                              int a, int m, F &&f) {
-  const int ng = (m + 3) >> 2;
-  if (ng == 0) return;
-  const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
-  const size_t sld = (size_t)ld;
-  unsigned long long w0 = ld_quad(lp);
-  unsigned long long w1 = ld_quad(lp + (size_t)min(1, ng - 1) * sld);
-  for (int g = 0; g < ng; g += 2) {
-    const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
-    const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) f((int)((w0 >> (16 * u)) & 0xFFFFull));
-    if (g + 1 < ng) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) f((int)((w1 >> (16 * u)) & 0xFFFFull));
-    }
-    w0 = n0;
-    w1 = n1;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -656,7 +633,10 @@ size_t gh_tile_static_bytes(int precision, int nt) {
   return sizeof(Halo) + params;
 }
 
-static size_t build_slots() { return GH_BUILD_LDS / 25; }
+// LDS of the build: the far-entry buffer, then FP32 local coordinates (12 B)
+// and the slot map (1 B) per staged slot.
+static size_t build_far_bytes() { return (size_t)GH_FARMAX * GH_TBLOCK * 2; }
+static size_t build_slots() { return (GH_BUILD_LDS - build_far_bytes()) / 13; }
 
 // Squared distance below which a pair takes the inner branch of the pair
 // function for every type pair: min rci^2 (SMA-TB), rc^2 (LJ).
@@ -673,11 +653,16 @@ static double gh_near2(const gh_ctx *c) {
 // ---------------------------------------------------------------------------
 int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
   const int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
-  const size_t lds = 25 * (size_t)lds_slots;
+  const size_t lds = build_far_bytes() + 13 * (size_t)lds_slots;
   const double rl = c->rcut + c->skin;
+  const double rl2 = rl * rl;
+  // FP32 prefilter: |r^2_f32 - r^2| stays below 3e-4 A^2 for local
+  // coordinates within 40 A of the brick centre; the margin is ten times that
+  const double margin = std::max(3e-3, 1e-4 * rl2);
   k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, c->stream>>>(
-      c->status, c->tg, c->pos, c->cellof, c->cell_start, rl * rl, gh_near2(c),
-      count_only ? 0 : cap, c->ncap, c->ncap, lds_slots, c->nn, c->list16, c->xold);
+      c->status, c->tg, c->pos, c->cellof, c->cell_start, rl2, (float)gh_near2(c),
+      (float)(rl2 - margin), (float)(rl2 + margin), count_only ? 0 : cap, c->ncap, c->ncap,
+      lds_slots, c->nn, c->list16, c->xold);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
