@@ -18,7 +18,7 @@
 
 #include "gh_device.h"
 
-#define GH_BBLOCK 512
+#define GH_BBLOCK 512  // largest workgroup of the batch kernels
 
 struct gh_batch {
   int device = 0;
@@ -47,7 +47,8 @@ struct gh_batch {
   unsigned short *list = nullptr;
   TbDev *tbd = nullptr;
   TbF *tbf = nullptr;
-  int nst = 0;  // LDS staging slots per workgroup (largest system + sentinel)
+  int nst = 0;   // LDS staging slots per workgroup (largest system + sentinel)
+  int nmax = 0;  // atoms of the largest system
 };
 
 namespace {
@@ -77,15 +78,15 @@ struct BatchArgs {
 };
 
 #define OUTW 6
-#define GH_BW (GH_BBLOCK / 64)
 
+template <int BS>
 struct BlockRed {
-  double v[GH_BW][8];  // up to 8 values per reduction
+  double v[BS / 64][8];  // up to 8 values per reduction
 };
 
 // Block-wide sums of NV values; the result is returned to every thread.
-template <int NV>
-__device__ __forceinline__ void bsum(BlockRed &R, double (&v)[NV]) {
+template <int BS, int NV>
+__device__ __forceinline__ void bsum(BlockRed<BS> &R, double (&v)[NV]) {
   static_assert(NV <= 8, "BlockRed holds 8 values per wave");
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -97,19 +98,20 @@ __device__ __forceinline__ void bsum(BlockRed &R, double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int w = 0; w < GH_BW; ++w) s += R.v[w][k];
+    for (int w = 0; w < BS / 64; ++w) s += R.v[w][k];
     v[k] = s;
   }
   __syncthreads();
 }
 
-__device__ __forceinline__ double bmax(BlockRed &R, double v) {
+template <int BS>
+__device__ __forceinline__ double bmax(BlockRed<BS> &R, double v) {
   v = wave_max(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) R.v[threadIdx.x >> 6][0] = v;
   __syncthreads();
   double m = R.v[0][0];
-  for (int w = 1; w < GH_BW; ++w) m = fmax(m, R.v[w][0]);
+  for (int w = 1; w < BS / 64; ++w) m = fmax(m, R.v[w][0]);
   __syncthreads();
   return m;
 }
@@ -325,7 +327,7 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
   extern __shared__ int4 dyn_q[];                                                          \
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];                                     \
   __shared__ TbF sf[SINGLE ? 1 : GH_MAXT * GH_MAXT];                                       \
-  __shared__ BlockRed R;                                                                   \
+  __shared__ BlockRed<BS> R;                                                               \
   __shared__ int sflag;                                                                    \
   for (int k_ = threadIdx.x; k_ < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k_ += blockDim.x) {    \
     sp[k_] = B.tbd[k_];                                                                    \
@@ -343,7 +345,8 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
   __syncthreads();
 
 // epot (ui) and ekin about the centre of mass (ui) of one system.
-__device__ void sys_energies(const BatchArgs &B, BlockRed &R, int a0, int a1, double &epot,
+template <int BS>
+__device__ void sys_energies(const BatchArgs &B, BlockRed<BS> &R, int a0, int a1, double &epot,
                              double &ekin) {
   double s[5] = {0, 0, 0, 0, 0};
   for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
@@ -354,7 +357,7 @@ __device__ void sys_energies(const BatchArgs &B, BlockRed &R, int a0, int a1, do
     s[3] += m * B.vel[B.ld + i];
     s[4] += m * B.vel[2 * B.ld + i];
   }
-  bsum<5>(R, s);
+  bsum<BS, 5>(R, s);
   const double cx = s[2] / s[1], cy = s[3] / s[1], cz = s[4] / s[1];
   double k[1] = {0.0};
   for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
@@ -362,13 +365,13 @@ __device__ void sys_energies(const BatchArgs &B, BlockRed &R, int a0, int a1, do
     const double wx = B.vel[i] - cx, wy = B.vel[B.ld + i] - cy, wz = B.vel[2 * B.ld + i] - cz;
     k[0] += (wx * wx + wy * wy + wz * wz) * m;
   }
-  bsum<1>(R, k);
+  bsum<BS, 1>(R, k);
   epot = s[0];
   ekin = 0.5 * k[0];
 }
 
-template <bool MIXED, bool SINGLE>
-__global__ __launch_bounds__(GH_BBLOCK) void k_batch_compute(BatchArgs B) {
+template <bool MIXED, bool SINGLE, int BS>
+__global__ __launch_bounds__(BS) void k_batch_compute(BatchArgs B) {
   SYS_SETUP(SINGLE)
   sys_stage(B, S, &sflag, !SINGLE);
   sys_build(B, S);
@@ -388,8 +391,8 @@ struct ErmakC {
 // nsteps Ermak steps of every system (ermak_a / ermak_b with the Philox
 // stream keyed by (seed, step, atom)); lists rebuilt per system when
 // 2 max|x - x_old| exceeds the skin.
-template <bool MIXED, bool SINGLE>
-__global__ __launch_bounds__(GH_BBLOCK) void k_batch_langevin(BatchArgs B, int nsteps,
+template <bool MIXED, bool SINGLE, int BS>
+__global__ __launch_bounds__(BS) void k_batch_langevin(BatchArgs B, int nsteps,
                                                               long long step0, ErmakC k,
                                                               unsigned long long seed,
                                                               double skin) {
@@ -470,7 +473,8 @@ __device__ __forceinline__ void fire_forces(const BatchArgs &B, Sys &S, int *fla
   }
 }
 
-__device__ double sys_fmax(const BatchArgs &B, BlockRed &R, int a0, int a1) {
+template <int BS>
+__device__ double sys_fmax(const BatchArgs &B, BlockRed<BS> &R, int a0, int a1) {
   double m = 0.0;
   for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
     const double fx = B.frc[i], fy = B.frc[B.ld + i], fz = B.frc[2 * B.ld + i];
@@ -484,8 +488,8 @@ __device__ double sys_fmax(const BatchArgs &B, BlockRed &R, int a0, int a1) {
 // dt (x1.1, capped at dt_max) and shrinks a (x0.99); P <= 0 halves dt, zeroes
 // v and resets a.  Mixed precision switches to FP64 pair terms once
 // max|F| < fswitch; convergence is max|F| < ftol in FP64.
-template <bool SINGLE>
-__global__ __launch_bounds__(GH_BBLOCK) void k_batch_fire(BatchArgs B, FireC fc, double skin) {
+template <bool SINGLE, int BS>
+__global__ __launch_bounds__(BS) void k_batch_fire(BatchArgs B, FireC fc, double skin) {
   SYS_SETUP(SINGLE)
   const int a0 = S.a0, a1 = S.a0 + S.n;
   bool mixed = fc.mixed != 0;
@@ -546,7 +550,7 @@ __global__ __launch_bounds__(GH_BBLOCK) void k_batch_fire(BatchArgs B, FireC fc,
         pv[2] += f * f;
       }
     }
-    bsum<3>(R, pv);
+    bsum<BS, 3>(R, pv);
     const double mixv = pv[2] > 0.0 ? alpha * sqrt(pv[1]) / sqrt(pv[2]) : 0.0;
     const bool up = pv[0] > 0.0;
     for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x)
@@ -708,6 +712,7 @@ static int batch_ready(gh_batch *b) {
     const size_t stat = (b->nt > 1 ? GH_MAXT * GH_MAXT : 1) * (sizeof(TbDev) + sizeof(TbF)) +
                         sizeof(double) * 8 * (GH_BBLOCK / 64) + 64;
     const size_t need = 17 * (size_t)(nmax + 1);
+    b->nmax = nmax;
     b->nst = stat + need <= 65536 ? nmax + 1 : 0;
   }
   return 0;
@@ -887,13 +892,17 @@ int gh_batch_compute(gh_batch *b, double *F, double *epot) {
   BT(batch_ready(b));
   const BatchArgs A = bargs(b);
   const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
+#define LAUNCH(MX, SG)                                                                   \
+  if (b->nmax > 512)                                                                     \
+    k_batch_compute<MX, SG, 512><<<b->nsys, 512, batch_lds(b), b->stream>>>(A);        \
+  else                                                                                   \
+    k_batch_compute<MX, SG, 256><<<b->nsys, 256, batch_lds(b), b->stream>>>(A)
   if (mixed) {
-    if (single) k_batch_compute<true, true><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
-    else k_batch_compute<true, false><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
+    if (single) LAUNCH(true, true); else LAUNCH(true, false);
   } else {
-    if (single) k_batch_compute<false, true><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
-    else k_batch_compute<false, false><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A);
+    if (single) LAUNCH(false, true); else LAUNCH(false, false);
   }
+#undef LAUNCH
   BH(b, hipGetLastError());
   BT(check_flags(b));
   b->lists_valid = true;
@@ -931,10 +940,13 @@ int gh_batch_langevin(gh_batch *b, int nsteps, double dt, double gamma, uint64_t
   k.kB_ui = ghc::kB_ui();
   const BatchArgs A = bargs(b);
   const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
-#define LAUNCH(MX, SG)                                                                 \
-  k_batch_langevin<MX, SG><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A, nsteps,       \
-                                                                           b->step + 1, k,  \
-                                                                 seed, b->skin)
+#define LAUNCH(MX, SG)                                                                   \
+  if (b->nmax > 512)                                                                     \
+    k_batch_langevin<MX, SG, 512><<<b->nsys, 512, batch_lds(b), b->stream>>>(           \
+        A, nsteps, b->step + 1, k, seed, b->skin);                                       \
+  else                                                                                   \
+    k_batch_langevin<MX, SG, 256><<<b->nsys, 256, batch_lds(b), b->stream>>>(             \
+        A, nsteps, b->step + 1, k, seed, b->skin)
   if (mixed) {
     if (single) LAUNCH(true, true); else LAUNCH(true, false);
   } else {
@@ -994,7 +1006,7 @@ int gh_batch_exchange(gh_batch *b, int nladder, int R, int *sys, int parity, uin
                        b->stream));
   BH(b, hipMemcpyAsync(b->scale, scale.data(), sizeof(double) * b->nsys, hipMemcpyHostToDevice,
                        b->stream));
-  k_batch_scale<<<b->nsys, GH_BBLOCK, 0, b->stream>>>(b->nsys, b->off, b->scale, b->vel, b->ld);
+  k_batch_scale<<<b->nsys, 256, 0, b->stream>>>(b->nsys, b->off, b->scale, b->vel, b->ld);
   BH(b, hipGetLastError());
   BH(b, hipStreamSynchronize(b->stream));
   if (naccept) *naccept = acc;
@@ -1015,10 +1027,17 @@ int gh_batch_fire(gh_batch *b, int max_iter, double dt_max, double alpha0, doubl
   fc.max_iter = max_iter;
   fc.mixed = b->precision == GH_MIXED;
   const BatchArgs A = bargs(b);
-  if (b->nt == 1)
-    k_batch_fire<true><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A, fc, b->skin);
-  else
-    k_batch_fire<false><<<b->nsys, GH_BBLOCK, batch_lds(b), b->stream>>>(A, fc, b->skin);
+#define LAUNCH(SG)                                                                       \
+  if (b->nmax > 512)                                                                     \
+    k_batch_fire<SG, 512><<<b->nsys, 512, batch_lds(b), b->stream>>>(A, fc, b->skin);  \
+  else                                                                                   \
+    k_batch_fire<SG, 256><<<b->nsys, 256, batch_lds(b), b->stream>>>(A, fc, b->skin)
+  if (b->nt == 1) {
+    LAUNCH(true);
+  } else {
+    LAUNCH(false);
+  }
+#undef LAUNCH
   BH(b, hipGetLastError());
   BT(check_flags(b));
   b->lists_valid = true;

@@ -12,8 +12,8 @@ pytestmark = pytest.mark.gpu
 hip = pytest.importorskip("gems_amd.hip")
 
 
-def _agcu_small(ngeom=12, rungs=3, seed=7):
-    return CL.agcu_re_batch(n_geom=ngeom, rungs=rungs, nmin=13, nmax=400, seed=seed)
+def _agcu_small(ngeom=12, rungs=3, seed=7, nmax=400):
+    return CL.agcu_re_batch(n_geom=ngeom, rungs=rungs, nmin=13, nmax=nmax, seed=seed)
 
 
 def _oracle_agcu():
@@ -33,9 +33,11 @@ def _batch_agcu(prec, off, x, t):
     return b
 
 
-@pytest.mark.parametrize("prec,tol", [(hip.GH_FP64, 1e-10), (hip.GH_MIXED, 1e-4)])
-def test_batch_forces_match_oracle(prec, tol):
-    off, x, t, T, lad = _agcu_small()
+@pytest.mark.parametrize("prec,tol,nmax", [(hip.GH_FP64, 1e-10, 400), (hip.GH_MIXED, 1e-4, 400),
+                                           (hip.GH_FP64, 1e-10, 1200), (hip.GH_MIXED, 1e-4, 1200)])
+def test_batch_forces_match_oracle(prec, tol, nmax):
+    # nmax 1200: batches with a system over 512 atoms run 512-thread workgroups
+    off, x, t, T, lad = _agcu_small(nmax=nmax, ngeom=8)
     with _batch_agcu(prec, off, x, t) as b:
         F, ep = b.compute()
     s = _oracle_agcu()
