@@ -376,6 +376,11 @@ static int ensure_ready(gh_ctx *c) {
     GH_TRY(dalloc(c, &c->list, (size_t)cap * c->ncap));
   }
   c->cap = cap;
+  if (c->list_mode == 1 && c->tg.nbrick > c->ndispblk) {  // fused force passes: one per brick
+    c->ndispblk = c->tg.nbrick;
+    GH_TRY(dalloc(c, &c->dispblk, (size_t)c->ndispblk));
+    GH_HIP(c, hipMemset(c->dispblk, 0, sizeof(double) * c->ndispblk));
+  }
   h.overflow = 0;
   h.tile_overflow = 0;
   h.nupd = nupd0;
@@ -495,7 +500,7 @@ void gh_destroy(gh_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void *dev[] = {c->pos, c->pos_t, c->vel, c->vel_t, c->acc, c->acc_t, c->frc, c->epa,
+  void *dev[] = {c->pos, c->pos_t, c->pos_next, c->vel, c->vel_t, c->acc, c->acc_t, c->frc, c->epa,
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
@@ -626,6 +631,7 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
     c->ncap = ncap;
     GH_TRY(dalloc(c, &c->pos, ncap));
     GH_TRY(dalloc(c, &c->pos_t, ncap));
+    GH_TRY(dalloc(c, &c->pos_next, ncap));
     GH_TRY(dalloc(c, &c->vel, 3 * (size_t)ncap));
     GH_TRY(dalloc(c, &c->vel_t, 3 * (size_t)ncap));
     GH_TRY(dalloc(c, &c->acc, 3 * (size_t)ncap));
@@ -993,22 +999,29 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
   const bool lcg = ermak && c->rng_mode == GH_RNG_LCG;
   // velocity Verlet: stepb of step n and stepa of step n+1 run as one kernel
   // unless the state after stepb is observed (energy output, end of the run)
-  bool pending_b = false;
+  // In the tiled path the force pass itself applies stepb of step n and stepa
+  // of step n+1 (fused); elsewhere one kernel does both (k_vv_ba).
+  bool pending_b = false, stepa_done = false;
   for (int ns = 1; ns <= nsteps; ++ns) {
     c->step += 1;
     if (lcg) GH_TRY(lcg_step_normals(c));
     GH_TRY(rec(c, 0));
     if (pending_b) GH_TRY(gh_launch_vv_ba(c));
-    else GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
-    pending_b = false;
+    else if (!stepa_done) GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
+    pending_b = stepa_done = false;
     GH_TRY(rec(c, 1));
     GH_TRY(gh_launch_rebuild(c, false));
     GH_TRY(rec(c, 2));
-    GH_TRY(gh_launch_forces(c, ev_at(c, 3)));
-    GH_TRY(rec(c, 4));
     const bool observed = ns == nsteps || (each > 0 && ns % each == 0);
+    const bool fuse = !ermak && !observed && gh_can_fuse_vv(c);
+    GH_TRY(gh_launch_forces(c, ev_at(c, 3), fuse));
+    if (fuse) {
+      std::swap(c->pos, c->pos_next);
+      stepa_done = true;
+    }
+    GH_TRY(rec(c, 4));
     if (ermak || observed) GH_TRY(ermak ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
-    else pending_b = true;
+    else if (!fuse) pending_b = true;
     GH_TRY(rec(c, 5));
     if (c->timing) {
       c->evused += 1;

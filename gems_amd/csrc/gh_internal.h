@@ -154,6 +154,7 @@ struct gh_ctx {
 
   // device state (cell-sorted order)
   double4 *pos = nullptr, *pos_t = nullptr;   // x,y,z,(type as double)
+  double4 *pos_next = nullptr;                // positions written by a fused force pass
   double *vel = nullptr, *vel_t = nullptr;    // [3][ncap] SoA
   double *acc = nullptr, *acc_t = nullptr;    // [3][ncap]
   double *frc = nullptr;                      // [3][ncap]
@@ -206,9 +207,23 @@ int gh_fail(gh_ctx *c, int code, const std::string &msg);
       return gh_fail(c, GH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Velocity Verlet fused into the force pass (stepb of this step, stepa of the
+// next) for the atoms whose forces the pass has just summed.
+struct FuseVV {
+  int on;
+  double dt, acel_dv, acel_dx;
+  double om[GH_MAXT];
+  double *vel;
+  const double *xold;
+  double4 *pos_next;
+  double *dispblk;
+  int ncap;
+};
+
 // launchers (gh_kernels.hip)
 int gh_launch_rebuild(gh_ctx *c, bool force);
-int gh_launch_forces(gh_ctx *c, hipEvent_t mid);
+int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv = false);
+bool gh_can_fuse_vv(const gh_ctx *c);
 int gh_launch_vv_a(gh_ctx *c);
 int gh_launch_vv_b(gh_ctx *c);
 int gh_launch_vv_ba(gh_ctx *c);
@@ -216,7 +231,7 @@ int gh_launch_ermak_a(gh_ctx *c);
 int gh_launch_ermak_b(gh_ctx *c);
 int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui);
 int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only);
-int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid);
+int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv);
 int gh_fill_int(gh_ctx *c, int *p, int v, int n);
 int gh_launch_wrap(gh_ctx *c);
 int gh_tile_occupancy(gh_ctx *c, int *max_halo, int *max_brick);

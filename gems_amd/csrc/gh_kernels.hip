@@ -1017,8 +1017,14 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   return 0;
 }
 
-int gh_launch_forces(gh_ctx *c, hipEvent_t mid) {
-  if (c->list_mode == 1) return gh_launch_forces_tiled(c, mid);
+bool gh_can_fuse_vv(const gh_ctx *c) {
+  return c->list_mode == 1 && c->integ == GH_INT_VV &&
+         (c->pot == GH_POT_SMATB || c->pot == GH_POT_LJ);
+}
+
+int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
+  if (c->list_mode == 1) return gh_launch_forces_tiled(c, mid, fuse_vv);
+  if (fuse_vv) return gh_fail(c, GH_ESTATE, "fused integration needs the tiled path");
   hipStream_t s = c->stream;
   const int n = c->n, nb = nblocks(n);
   const int ld = c->ncap;

@@ -148,6 +148,42 @@ __device__ __forceinline__ int halo_of_slot(const Halo &H, int s) {
   return lo;
 }
 
+// stepb of this step and stepa of the next for atom a (velocity_verlet_b,
+// Integration.F90:1171-1186, then velocity_verlet_a, :1150-1169): the same
+// operations in the same order as k_vv_ba, on the force just summed.
+// New positions go to pos_next (other workgroups still stage pos); returns
+// |x_new - x_old|^2 for the skin test.
+__device__ __forceinline__ double fused_vv(const FuseVV &f, int a, const double4 &p, int t,
+                                           double Fx, double Fy, double Fz) {
+  const double om = f.om[t];
+  const double F[3] = {Fx, Fy, Fz};
+  double x[3] = {p.x, p.y, p.z};
+  double d2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double acc = F[d] * om;
+    const double v = f.vel[d * f.ncap + a] + f.acel_dv * acc;  // stepb
+    x[d] = x[d] + f.dt * v + f.acel_dx * acc;                   // stepa
+    f.vel[d * f.ncap + a] = v + f.acel_dv * acc;
+    const double dd = x[d] - f.xold[d * f.ncap + a];
+    d2 += dd * dd;
+  }
+  f.pos_next[a] = make_double4(x[0], x[1], x[2], p.w);
+  return d2;
+}
+
+// Workgroup maximum of d2, stored at dst[b] (all threads call it).
+__device__ __forceinline__ void block_max_store(double d2, double *dst, int b) {
+  __shared__ double part[GH_TBLOCK / 64];
+  d2 = wave_max(d2);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GH_TBLOCK / 64; ++w) d2 = fmax(d2, part[w]);
+    dst[b] = d2;
+  }
+}
+
 __device__ __forceinline__ size_t qidx(int k, int a, int ld) {
   return (((size_t)(k >> 2) * ld + a) << 2) + (k & 3);
 }
@@ -368,7 +404,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
     const double *__restrict__ rho, const float *__restrict__ eband32, double ev_ui,
-    double *__restrict__ frc, double *__restrict__ epa) {
+    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv) {
   extern __shared__ int4 q4[];
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
@@ -395,6 +431,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
   }
   __syncthreads();
   const double fscale = ev_ui * (1.0 / GH_FXS);
+  double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - (H.org[0] - 0.0)), qy = fxq(pa.y - (H.org[1] - 0.0)),
@@ -419,11 +456,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
         ei += (double)repul;
       }
     });
-    frc[a] = fx * fscale;
-    frc[ncap + a] = fy * fscale;
-    frc[2 * ncap + a] = fz * fscale;
+    const double Fx = fx * fscale, Fy = fy * fscale, Fz = fz * fscale;
+    frc[a] = Fx;
+    frc[ncap + a] = Fy;
+    frc[2 * ncap + a] = Fz;
     epa[a] = (ei - sqrt(rho[a])) * ev_ui;
+    if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
   }
+  if (fv.on) block_max_store(dmax2, fv.dispblk, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,7 +529,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbDev *__restrict__ tbd,
     int nt, const double *__restrict__ rho, const double *__restrict__ eband, double ev_ui,
-    double *__restrict__ frc, double *__restrict__ epa) {
+    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
@@ -522,6 +562,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
     if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
+  double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
@@ -541,11 +582,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
         ei += repul;
       }
     });
-    frc[a] = fx * ev_ui;
-    frc[ncap + a] = fy * ev_ui;
-    frc[2 * ncap + a] = fz * ev_ui;
+    const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
+    frc[a] = Fx;
+    frc[ncap + a] = Fy;
+    frc[2 * ncap + a] = Fz;
     epa[a] = (ei - sqrt(rho[a])) * ev_ui;
+    if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
   }
+  if (fv.on) block_max_store(dmax2, fv.dispblk, b);
 }
 
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
@@ -554,7 +598,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
     double eps, double sigma, double rc2, double ev_ui, double *__restrict__ frc,
-    double *__restrict__ epa) {
+    double *__restrict__ epa, const int *__restrict__ type, FuseVV fv) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
   const int b = xcd_block();
@@ -574,6 +618,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
   if (threadIdx.x == 0) px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
   __syncthreads();
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
+  double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
@@ -605,11 +650,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
         }
       }
     });
-    frc[a] = fx * ev_ui;
-    frc[ncap + a] = fy * ev_ui;
-    frc[2 * ncap + a] = fz * ev_ui;
+    const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
+    frc[a] = Fx;
+    frc[ncap + a] = Fy;
+    frc[2 * ncap + a] = Fz;
     epa[a] = ei * 0.5 * ev_ui;
+    if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pi, type[a], Fx, Fy, Fz));
   }
+  if (fv.on) block_max_store(dmax2, fv.dispblk, b);
 }
 
 }  // namespace
@@ -691,8 +739,19 @@ int gh_tiled_to_global(gh_ctx *c, int *dst) {
   return 0;
 }
 
-int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid) {
+int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   hipStream_t s = c->stream;
+  FuseVV fv;
+  fv.on = fuse_vv ? 1 : 0;
+  fv.dt = c->dt;
+  fv.acel_dv = c->dt * 0.5;
+  fv.acel_dx = fv.acel_dv * c->dt;
+  for (int t = 0; t < GH_MAXT; ++t) fv.om[t] = 1.0 / c->mass_t[t];
+  fv.vel = c->vel;
+  fv.xold = c->xold;
+  fv.pos_next = c->pos_next;
+  fv.dispblk = c->dispblk;
+  fv.ncap = c->ncap;
   const int grid = xgrid(c->tg.nbrick);
   const int ld = c->ncap;
   const double ev_ui = ghc::ev_ui();
@@ -715,11 +774,11 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid) {
       if (single)
         k_tb_force_fx<true><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa);
+            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv);
       else
         k_tb_force_fx<false><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa);
+            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv);
     } else {
       if (single)
         k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
@@ -733,11 +792,11 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid) {
       if (single)
         k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa);
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv);
       else
         k_tb_force64_tiled<false><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa);
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv);
     }
   } else if (c->pot == GH_POT_LJ) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
@@ -746,11 +805,13 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid) {
     if (mixed)
       k_lj_tiled<true><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
                                                     c->nn, c->list16, ld, c->ncap, c->lj_eps,
-                                                    c->lj_sigma, rc2, ev_ui, c->frc, c->epa);
+                                                    c->lj_sigma, rc2, ev_ui, c->frc, c->epa,
+                                                    c->type, fv);
     else
       k_lj_tiled<false><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
                                                      c->nn, c->list16, ld, c->ncap, c->lj_eps,
-                                                     c->lj_sigma, rc2, ev_ui, c->frc, c->epa);
+                                                     c->lj_sigma, rc2, ev_ui, c->frc, c->epa,
+                                                    c->type, fv);
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");
   }

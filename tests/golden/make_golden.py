@@ -88,11 +88,11 @@ def read_pes(path, n):
     return vals.reshape(-1, n, 4)
 
 
-def run(workdir, script):
+def run(workdir, script, timeout=3600):
     with open(os.path.join(workdir, "in.gms"), "w") as f:
         f.write(script)
     r = subprocess.run([GEMS, "in.gms"], cwd=workdir, capture_output=True, text=True,
-                       timeout=3600)
+                       timeout=timeout)
     log = open(os.path.join(workdir, "in.log")).read()
     if "#-ERR->" in log or r.returncode != 0:
         raise RuntimeError("reference run failed:\n" + log[-3000:])
@@ -311,7 +311,7 @@ def drift_main(nsteps=10000, each=100):
         s += "outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each %d\n" % each
         s += "> sys\nevolve v_verlet\ndinamica %d\n" % nsteps
         t0 = __import__("time").time()
-        log, nupd = run(wd, s)
+        log, nupd = run(wd, s, timeout=4 * 3600)  # about 75 min on 6 cores
         E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3)
         np.savez_compressed(os.path.join(HERE, "au32k_nve_drift.npz"), energy=E, nupd=nupd,
                             nsteps=nsteps, each=each, seed=4242, nx=20, jitter_seed=20260101)
