@@ -276,10 +276,78 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
   }
 }
 
+// Staged build with the rows assembled in LDS (rb: entry k of this thread's
+// row at rb[k * GH_TBLOCK], k = cap is a dump slot): every candidate is
+// written through a computed index, near pairs from the front and the rest
+// from the back, so the candidate loop has no per-pair branch but the rare
+// exact test.  The rows are then stored as whole quads.
+__device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
+                                               const double4 *__restrict__ pos,
+                                               const int *__restrict__ cellof, const float *xf,
+                                               const float *yf, const float *zf,
+                                               unsigned short *rowbuf, double rl2,
+                                               float near2f, float lo2f, float hi2f, int cap,
+                                               int pad, int ld, int ncap, int *__restrict__ nn,
+                                               unsigned short *__restrict__ list16,
+                                               double *__restrict__ xold, int &mmax) {
+  unsigned short *rb = rowbuf + threadIdx.x;
+  unsigned long long *lq = reinterpret_cast<unsigned long long *>(list16);
+  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+    const int lc = cellof[a] - H.key0;
+    const int lx = lc % t.bd[0], ly = (lc / t.bd[0]) % t.bd[1], lz = lc / (t.bd[0] * t.bd[1]);
+    const int ha = (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
+    const int own = a - H.delta[ha];
+    const double4 pa = pos[a];
+    const float xfi = xf[own], yfi = yf[own], zfi = zf[own];
+    int mn = 0, mf = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
+          const int se = H.hbase[h + 1];
+          for (int s = H.hbase[h]; s < se; ++s) {
+            const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
+            const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+            bool keep = s != own && r2f <= hi2f;
+            if (keep && r2f >= lo2f) {  // near the cut: the reference's rounding decides
+              const double4 pj = pos[s + H.delta[h]];
+              const double vx = (pj.x - pa.x) + shift(t, H, 0, h);
+              const double vy = (pj.y - pa.y) + shift(t, H, 1, h);
+              const double vz = (pj.z - pa.z) + shift(t, H, 2, h);
+              keep = dist2_rn(vx, vy, vz) < rl2;
+            }
+            const bool near = r2f < near2f;
+            int k = near ? mn : cap - 1 - mf;
+            if (!keep || (unsigned)k >= (unsigned)cap) k = cap;
+            rb[k * GH_TBLOCK] = (unsigned short)s;
+            mn += (keep && near) ? 1 : 0;
+            mf += (keep && !near) ? 1 : 0;
+          }
+        }
+    const int m = mn + mf, mw = min(m, cap);
+    for (int k0 = 0; k0 < mw; k0 += 4) {
+      unsigned long long w = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u;
+        const int sl = k < mw ? (k < mn ? rb[k * GH_TBLOCK] : rb[(cap - 1 - (k - mn)) * GH_TBLOCK])
+                              : pad;
+        w |= (unsigned long long)sl << (16 * u);
+      }
+      lq[(size_t)(k0 >> 2) * ld + a] = w;
+    }
+    nn[a] = mw;
+    xold[a] = pa.x;
+    xold[ncap + a] = pa.y;
+    xold[2 * ncap + a] = pa.z;
+    mmax = max(mmax, m);
+  }
+}
+
 __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
     const int *__restrict__ cell_start, double rl2, float near2f, float lo2f, float hi2f,
-    int cap, int ld, int ncap, int lds_slots, int *__restrict__ nn,
+    int cap, int ld, int ncap, int lds_slots, int lds_rows, int *__restrict__ nn,
     unsigned short *__restrict__ list16, double *__restrict__ xold) {
   if (!st->rebuild) return;
   extern __shared__ float dynf[];
@@ -289,6 +357,37 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   int mmax = 0;
   const int pad = t.slot_cap;  // sentinel slot of the force passes
+  if (lds_rows) {
+    // LDS: rows [(cap + 1) * GH_TBLOCK] u16, then xf, yf, zf, slot map
+    unsigned short *rowbuf = reinterpret_cast<unsigned short *>(dynf);
+    float *xf = dynf + ((cap + 1) * GH_TBLOCK + 1) / 2;
+    if (H.total <= lds_slots) {
+      float *yf = xf + lds_slots, *zf = xf + 2 * lds_slots;
+      unsigned char *scell = reinterpret_cast<unsigned char *>(xf + 3 * lds_slots);
+      slot_map(H, scell);
+      for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+        const int h = scell[s];
+        const double4 p = pos[s + H.delta[h]];
+        xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
+        yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
+        zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
+      }
+      __syncthreads();
+      build_rows_lds(H, t, pos, cellof, xf, yf, zf, rowbuf, rl2, near2f, lo2f, hi2f, cap, pad,
+                     ld, ncap, nn, list16, xold, mmax);
+    } else {
+      // halo too large for the slots left beside the rows: unstaged path
+      unsigned short *far = rowbuf;  // GH_FARMAX * GH_TBLOCK fits in the row buffer
+      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, rl2, near2f, lo2f,
+                        hi2f, cap, pad, ld, ncap, nn, list16, xold, mmax);
+    }
+    mmax = wave_imax(mmax);
+    if ((threadIdx.x & 63) == 0 && mmax > 0) {
+      atomicMax(&st->maxnn, mmax);
+      if (mmax > cap) st->overflow = 1;
+    }
+    return;
+  }
   unsigned short *far = reinterpret_cast<unsigned short *>(dynf);
   float *xf = dynf + GH_FARMAX * GH_TBLOCK / 2;
   if (H.total <= lds_slots) {
@@ -700,8 +799,18 @@ static double gh_near2(const gh_ctx *c) {
 // launchers
 // ---------------------------------------------------------------------------
 int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
-  const int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
-  const size_t lds = build_far_bytes() + 13 * (size_t)lds_slots;
+  int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
+  size_t lds = build_far_bytes() + 13 * (size_t)lds_slots;
+  // rows in LDS when they and the staged halo fit (the branch-free variant)
+  const int capb = count_only ? 0 : cap;
+  const size_t rows = ((size_t)(capb + 1) * GH_TBLOCK * 2 + 3) / 4 * 4;
+  const size_t slots_left = rows < GH_BUILD_LDS ? (GH_BUILD_LDS - rows) / 13 : 0;
+  int lds_rows = 0;
+  if (slots_left >= (size_t)c->tg.slot_cap || slots_left >= 2048) {
+    lds_rows = 1;
+    lds_slots = (int)std::min<size_t>(slots_left, (size_t)c->tg.slot_cap);
+    lds = rows + 13 * (size_t)lds_slots;
+  }
   const double rl = c->rcut + c->skin;
   const double rl2 = rl * rl;
   // FP32 prefilter: |r^2_f32 - r^2| stays below 3e-4 A^2 for local
@@ -709,8 +818,8 @@ int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
   const double margin = std::max(3e-3, 1e-4 * rl2);
   k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, c->stream>>>(
       c->status, c->tg, c->pos, c->cellof, c->cell_start, rl2, (float)gh_near2(c),
-      (float)(rl2 - margin), (float)(rl2 + margin), count_only ? 0 : cap, c->ncap, c->ncap,
-      lds_slots, c->nn, c->list16, c->xold);
+      (float)(rl2 - margin), (float)(rl2 + margin), capb, c->ncap, c->ncap, lds_slots,
+      lds_rows, c->nn, c->list16, c->xold);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
