@@ -230,7 +230,7 @@ int gh_launch_vv_ba(gh_ctx *c);
 int gh_launch_ermak_a(gh_ctx *c);
 int gh_launch_ermak_b(gh_ctx *c);
 int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui);
-int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only);
+int gh_launch_rebuild_cells(gh_ctx *c);
 int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv);
 int gh_fill_int(gh_ctx *c, int *p, int v, int n);
 int gh_launch_wrap(gh_ctx *c);

@@ -84,176 +84,6 @@ struct GridArgs {
 
 
 
-__device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, int *outside) {
-  int c = (int)floor(x * t.one_cell[d]);
-  if (t.pbc[d]) {
-    c %= t.nc[d];
-    if (c < 0) c += t.nc[d];
-  } else {
-    c += t.gofs[d];
-    if (c < 0) { c = 0; *outside = 1; }
-    if (c >= t.gn[d]) { c = t.gn[d] - 1; *outside = 1; }
-  }
-  return c;
-}
-
-// Wraps periodic coordinates into the box (do_pbc_1, Set_Properties.f90:444-470)
-// and computes each atom's sort key: brick * cpb + cell within the brick.
-__global__ void k_bin(GhStatus *st, double4 *__restrict__ pos, int n, TileGeom t,
-                      int *__restrict__ cellid, int *__restrict__ count) {
-  if (!st->rebuild) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double4 p = pos[i];
-    double x[3] = {p.x, p.y, p.z};
-    bool moved = false;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      if (t.pbc[d]) {
-        const double w0 = x[d];
-        x[d] = x[d] - t.L[d] * floor(x[d] / t.L[d]);
-        if (x[d] >= t.L[d]) x[d] -= t.L[d];
-        moved |= x[d] != w0;
-      }
-    if (moved) {
-      p.x = x[0];
-      p.y = x[1];
-      p.z = x[2];
-      pos[i] = p;
-    }
-    int out = 0;
-    const int cx = tile_coord(x[0], t, 0, &out);
-    const int cy = tile_coord(x[1], t, 1, &out);
-    const int cz = tile_coord(x[2], t, 2, &out);
-    if (out) st->outside = 1;
-    const int bx = cx / t.bd[0], by = cy / t.bd[1], bz = cz / t.bd[2];
-    const int lx = cx - bx * t.bd[0], ly = cy - by * t.bd[1], lz = cz - bz * t.bd[2];
-    const int key =
-        ((bz * t.nbr[1] + by) * t.nbr[0] + bx) * t.cpb + (lz * t.bd[1] + ly) * t.bd[0] + lx;
-    cellid[i] = key;
-    atomicAdd(&count[key], 1);
-  }
-}
-
-// Exclusive scan of the cell counts in one 1024-thread workgroup, in tiles of
-// 4096 keys: each thread scans four consecutive counts (one coalesced 16-B
-// load), wavefront prefix sums and a scan of the 16 wave totals give the
-// tile's offsets, and a running carry links the tiles.
-__global__ __launch_bounds__(1024) void k_scan(const GhStatus *st, const int *__restrict__ count,
-                                               int *__restrict__ start, int *__restrict__ cursor,
-                                               int ncell) {
-  if (!st->rebuild) return;
-  __shared__ int wsum[16];
-  __shared__ int carry_s;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (t == 0) carry_s = 0;
-  __syncthreads();
-  for (int base = 0; base < ncell; base += 4096) {
-    const int k0 = base + 4 * t;
-    int v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = k0 + u < ncell ? count[k0 + u] : 0;
-    const int tot = v[0] + v[1] + v[2] + v[3];
-    int inc = tot;  // inclusive prefix over the wavefront
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += y;
-    }
-    if (lane == 63) wsum[wv] = inc;
-    __syncthreads();
-    int woff = 0, tile = 0;
-    for (int w = 0; w < 16; ++w) {
-      const int x = wsum[w];
-      if (w < wv) woff += x;
-      tile += x;
-    }
-    int run = carry_s + woff + inc - tot;  // exclusive prefix of this thread's four
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + u < ncell) {
-        start[k0 + u] = run;
-        cursor[k0 + u] = run;
-        run += v[u];
-      }
-    __syncthreads();
-    if (t == 0) carry_s += tile;
-    __syncthreads();
-  }
-  if (t == 0) start[ncell] = carry_s;
-}
-
-__global__ void k_scatter(const GhStatus *st, const int *__restrict__ cellid, int n,
-                          int *__restrict__ cursor, int *__restrict__ perm) {
-  if (!st->rebuild) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int slot = atomicAdd(&cursor[cellid[i]], 1);
-    perm[slot] = i;
-  }
-}
-
-// Order the atoms of each cell by their caller index: the sorted layout is
-// then independent of the atomic scatter order (deterministic results).
-// The cell counts are consumed by then: they are zeroed for the next build.
-__global__ void k_sort_cells(const GhStatus *st, const int *__restrict__ start,
-                             const int *__restrict__ orig, int *__restrict__ perm,
-                             int *__restrict__ count, int ncell) {
-  if (!st->rebuild) return;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
-    const int b = start[c], e = start[c + 1];
-    for (int k = b + 1; k < e; ++k) {
-      const int v = perm[k];
-      const int key = orig[v];
-      int m = k - 1;
-      while (m >= b && orig[perm[m]] > key) {
-        perm[m + 1] = perm[m];
-        --m;
-      }
-      perm[m + 1] = v;
-    }
-    count[c] = 0;
-  }
-}
-
-__global__ void k_gather(const GhStatus *st, const int *__restrict__ perm, int n, int ncap,
-                         const int *__restrict__ cellid, const double4 *__restrict__ pos,
-                         const double *__restrict__ vel, const double *__restrict__ acc,
-                         const int *__restrict__ type, const int *__restrict__ orig,
-                         double4 *__restrict__ pos_t, double *__restrict__ vel_t,
-                         double *__restrict__ acc_t, int *__restrict__ type_t,
-                         int *__restrict__ orig_t, int *__restrict__ cellof) {
-  if (!st->rebuild) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const int o = perm[s];
-    pos_t[s] = pos[o];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      vel_t[d * ncap + s] = vel[d * ncap + o];
-      acc_t[d * ncap + s] = acc[d * ncap + o];
-    }
-    type_t[s] = type[o];
-    orig_t[s] = orig[o];
-    cellof[s] = cellid[o];
-  }
-}
-
-__global__ void k_copyback(const GhStatus *st, int n, int ncap, const double4 *__restrict__ pos_t,
-                           const double *__restrict__ vel_t, const double *__restrict__ acc_t,
-                           const int *__restrict__ type_t, const int *__restrict__ orig_t,
-                           double4 *__restrict__ pos, double *__restrict__ vel,
-                           double *__restrict__ acc, int *__restrict__ type,
-                           int *__restrict__ orig) {
-  if (!st->rebuild) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    pos[s] = pos_t[s];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      vel[d * ncap + s] = vel_t[d * ncap + s];
-      acc[d * ncap + s] = acc_t[d * ncap + s];
-    }
-    type[s] = type_t[s];
-    orig[s] = orig_t[s];
-  }
-}
-
 // O(N^2) build (verlet_self, Neighbour.F90:676-759): rd > rl^2 -> skip, so
 // equality is kept; minimum image by rounding as vdistance does.
 __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
@@ -909,7 +739,6 @@ __global__ void k_wrap_rebuild(const GhStatus *st, int n, WrapArgs wa,
 }
 
 inline int nblocks(int n) { return (n + GH_BLOCK - 1) / GH_BLOCK; }
-#define GH_STRIDE_GRID 2048
 
 GridArgs make_grid(const gh_ctx *c) {
   GridArgs g;
@@ -985,7 +814,6 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   // a rebuild or at setup)
   k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
   const int n = c->n, nb = nblocks(n);
-  const int gs = std::min(nb, GH_STRIDE_GRID);  // grid-stride: cheap when no rebuild is due
   if (c->list_mode == 0 && c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
     k_wrap_rebuild<<<nb, GH_BLOCK, 0, s>>>(c->status, n, make_wrap(c), c->pos);
   const GridArgs g = make_grid(c);
@@ -993,22 +821,7 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   const double rl2 = rl * rl;
   const int ld = c->ncap;
   if (c->list_mode == 1) {
-    const int nkey = c->tg.nkey;
-    // cell_count is all zero here (zeroed at sizing and by k_sort_cells)
-    const int gk = std::min(nblocks(nkey), GH_STRIDE_GRID);
-    k_bin<<<gs, GH_BLOCK, 0, s>>>(c->status, c->pos, n, c->tg, c->cellid, c->cell_count);
-    k_scan<<<1, 1024, 0, s>>>(c->status, c->cell_count, c->cell_start, c->cell_cursor, nkey);
-    k_scatter<<<gs, GH_BLOCK, 0, s>>>(c->status, c->cellid, n, c->cell_cursor, c->perm);
-    k_sort_cells<<<gk, GH_BLOCK, 0, s>>>(c->status, c->cell_start, c->orig, c->perm,
-                                         c->cell_count, nkey);
-    k_gather<<<gs, GH_BLOCK, 0, s>>>(c->status, c->perm, n, c->ncap, c->cellid, c->pos,
-                                     c->vel, c->acc, c->type, c->orig, c->pos_t, c->vel_t,
-                                     c->acc_t, c->type_t, c->orig_t, c->cellof);
-    k_copyback<<<gs, GH_BLOCK, 0, s>>>(c->status, n, c->ncap, c->pos_t, c->vel_t, c->acc_t,
-                                       c->type_t, c->orig_t, c->pos, c->vel, c->acc, c->type,
-                                       c->orig);
-    GH_HIP(c, hipGetLastError());
-    return gh_launch_build_tiled(c, c->cap, 0);
+    return gh_launch_rebuild_cells(c);
   } else {
     k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
                                        (unsigned *)c->list, c->xold, c->ncap);

@@ -344,20 +344,230 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
   }
 }
 
-__global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
-    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
-    const int *__restrict__ cell_start, double rl2, float near2f, float lo2f, float hi2f,
-    int cap, int ld, int ncap, int lds_slots, int lds_rows, int *__restrict__ nn,
-    unsigned short *__restrict__ list16, double *__restrict__ xold) {
-  if (!st->rebuild) return;
-  extern __shared__ float dynf[];
-  __shared__ Halo H;
-  const int b = xcd_block();
-  if (b >= t.nbrick) return;
+// ---------------------------------------------------------------------------
+// Binning (sort_in_cells, Neighbour.F90:927-989, as a counting sort by key =
+// brick * cpb + cell within the brick): wrap, count per key, exclusive scan,
+// scatter, order each cell by caller index (deterministic layout), gather
+// the per-atom state into the new order, copy it back.
+// ---------------------------------------------------------------------------
+struct CellArgs {
+  GhStatus *st;
+  TileGeom t;
+  int n, ncap, nkey;
+  double4 *pos, *pos_t;
+  double *vel, *vel_t, *acc, *acc_t;
+  int *type, *type_t, *orig, *orig_t;
+  int *cellid, *cellof, *count, *start, *cursor, *perm;
+};
+
+__device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, int *outside) {
+  int c = (int)floor(x * t.one_cell[d]);
+  if (t.pbc[d]) {
+    c %= t.nc[d];
+    if (c < 0) c += t.nc[d];
+  } else {
+    c += t.gofs[d];
+    if (c < 0) {
+      c = 0;
+      *outside = 1;
+    }
+    if (c >= t.gn[d]) {
+      c = t.gn[d] - 1;
+      *outside = 1;
+    }
+  }
+  return c;
+}
+
+// Wraps periodic coordinates into the box (do_pbc_1, Set_Properties.f90:
+// 444-470), computes each atom's key and counts atoms per key.
+__device__ void cell_bin(const CellArgs &R, int i0, int stride) {
+  const TileGeom &t = R.t;
+  for (int i = i0; i < R.n; i += stride) {
+    double4 p = R.pos[i];
+    double x[3] = {p.x, p.y, p.z};
+    bool moved = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (t.pbc[d]) {
+        const double w0 = x[d];
+        x[d] = x[d] - t.L[d] * floor(x[d] / t.L[d]);
+        if (x[d] >= t.L[d]) x[d] -= t.L[d];
+        moved |= x[d] != w0;
+      }
+    if (moved) {
+      p.x = x[0];
+      p.y = x[1];
+      p.z = x[2];
+      R.pos[i] = p;
+    }
+    int out = 0;
+    const int cx = tile_coord(x[0], t, 0, &out);
+    const int cy = tile_coord(x[1], t, 1, &out);
+    const int cz = tile_coord(x[2], t, 2, &out);
+    if (out) R.st->outside = 1;
+    const int key = grid_key(t, cx, cy, cz);
+    R.cellid[i] = key;
+    atomicAdd(&R.count[key], 1);
+  }
+}
+
+#define GH_SCANBLOCK 1024  // workgroup of the cell-count scan
+
+// Workgroup (GH_SCANBLOCK threads) exclusive scan of four values per thread
+// in thread order; returns this thread's offset and the workgroup total.
+__device__ __forceinline__ int block_scan4(const int (&v)[4], int &total) {
+  __shared__ int wsum[GH_SCANBLOCK / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tot = v[0] + v[1] + v[2] + v[3];
+  int inc = tot;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  int woff = 0;
+  total = 0;
+  for (int w = 0; w < GH_SCANBLOCK / 64; ++w) {
+    const int x = wsum[w];
+    if (w < wv) woff += x;
+    total += x;
+  }
+  __syncthreads();
+  return woff + inc - tot;
+}
+
+// Exclusive scan of the cell counts by one workgroup, in tiles of four keys
+// per thread (one coalesced 16-B load each); a running carry links the
+// tiles.  start = cursor = the prefix; start[nkey] = the total.
+__device__ void cell_scan(const CellArgs &R) {
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < R.nkey; base += 4 * GH_SCANBLOCK) {
+    const int k0 = base + 4 * threadIdx.x;
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = k0 + u < R.nkey ? R.count[k0 + u] : 0;
+    int total;
+    int run = carry + block_scan4(v, total);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < R.nkey) {
+        R.start[k0 + u] = run;
+        R.cursor[k0 + u] = run;
+        run += v[u];
+      }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) R.start[R.nkey] = carry;
+}
+
+__device__ void cell_scatter(const CellArgs &R, int i0, int stride) {
+  for (int i = i0; i < R.n; i += stride) {
+    const int slot = atomicAdd(&R.cursor[R.cellid[i]], 1);
+    R.perm[slot] = i;
+  }
+}
+
+// Orders each cell by caller index (the layout is then independent of the
+// atomic scatter order) and zeroes its count for the next binning.
+__device__ void cell_sort(const CellArgs &R, int c0, int stride) {
+  for (int c = c0; c < R.nkey; c += stride) {
+    const int b = R.start[c], e = R.start[c + 1];
+    for (int k = b + 1; k < e; ++k) {
+      const int v = R.perm[k];
+      const int key = R.orig[v];
+      int m = k - 1;
+      while (m >= b && R.orig[R.perm[m]] > key) {
+        R.perm[m + 1] = R.perm[m];
+        --m;
+      }
+      R.perm[m + 1] = v;
+    }
+    R.count[c] = 0;
+  }
+}
+
+__device__ void cell_gather(const CellArgs &R, int s0, int stride) {
+  const int ncap = R.ncap;
+  for (int s = s0; s < R.n; s += stride) {
+    const int o = R.perm[s];
+    R.pos_t[s] = R.pos[o];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R.vel_t[d * ncap + s] = R.vel[d * ncap + o];
+      R.acc_t[d * ncap + s] = R.acc[d * ncap + o];
+    }
+    R.type_t[s] = R.type[o];
+    R.orig_t[s] = R.orig[o];
+    R.cellof[s] = R.cellid[o];
+  }
+}
+
+__device__ void cell_copyback(const CellArgs &R, int s0, int stride) {
+  const int ncap = R.ncap;
+  for (int s = s0; s < R.n; s += stride) {
+    R.pos[s] = R.pos_t[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R.vel[d * ncap + s] = R.vel_t[d * ncap + s];
+      R.acc[d * ncap + s] = R.acc_t[d * ncap + s];
+    }
+    R.type[s] = R.type_t[s];
+    R.orig[s] = R.orig_t[s];
+  }
+}
+
+// The binning phases, one launch each (every launch returns at once when no
+// rebuild is due).
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_bin(CellArgs R) {
+  if (!R.st->rebuild) return;
+  cell_bin(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(GH_SCANBLOCK) void k_cell_scan(CellArgs R) {
+  if (!R.st->rebuild) return;
+  cell_scan(R);
+}
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_scatter(CellArgs R) {
+  if (!R.st->rebuild) return;
+  cell_scatter(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_sort(CellArgs R) {
+  if (!R.st->rebuild) return;
+  cell_sort(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_gather(CellArgs R) {
+  if (!R.st->rebuild) return;
+  cell_gather(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_copyback(CellArgs R) {
+  if (!R.st->rebuild) return;
+  cell_copyback(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+struct BuildArgs {
+  double rl2;
+  float near2f, lo2f, hi2f;  // inner-branch bound; FP32 prefilter band around rl^2
+  int cap, ld, ncap, lds_slots, lds_rows;
+  int *nn;
+  unsigned short *list16;
+  double *xold;
+};
+
+// Rows of the atoms of brick b (all threads of the workgroup call it; LDS
+// is free again on return).
+__device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
+                            const int *__restrict__ cellof, const int *__restrict__ cell_start,
+                            const BuildArgs &A, float *dynf, Halo &H, int b) {
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   int mmax = 0;
   const int pad = t.slot_cap;  // sentinel slot of the force passes
-  if (lds_rows) {
+  const int cap = A.cap, lds_slots = A.lds_slots;
+  if (A.lds_rows) {
     // LDS: rows [(cap + 1) * GH_TBLOCK] u16, then xf, yf, zf, slot map
     unsigned short *rowbuf = reinterpret_cast<unsigned short *>(dynf);
     float *xf = dynf + ((cap + 1) * GH_TBLOCK + 1) / 2;
@@ -373,46 +583,53 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
         zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
       }
       __syncthreads();
-      build_rows_lds(H, t, pos, cellof, xf, yf, zf, rowbuf, rl2, near2f, lo2f, hi2f, cap, pad,
-                     ld, ncap, nn, list16, xold, mmax);
+      build_rows_lds(H, t, pos, cellof, xf, yf, zf, rowbuf, A.rl2, A.near2f, A.lo2f, A.hi2f,
+                     cap, pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
     } else {
       // halo too large for the slots left beside the rows: unstaged path
       unsigned short *far = rowbuf;  // GH_FARMAX * GH_TBLOCK fits in the row buffer
-      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, rl2, near2f, lo2f,
-                        hi2f, cap, pad, ld, ncap, nn, list16, xold, mmax);
+      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A.rl2, A.near2f,
+                        A.lo2f, A.hi2f, cap, pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
     }
-    mmax = wave_imax(mmax);
-    if ((threadIdx.x & 63) == 0 && mmax > 0) {
-      atomicMax(&st->maxnn, mmax);
-      if (mmax > cap) st->overflow = 1;
-    }
-    return;
-  }
-  unsigned short *far = reinterpret_cast<unsigned short *>(dynf);
-  float *xf = dynf + GH_FARMAX * GH_TBLOCK / 2;
-  if (H.total <= lds_slots) {
-    float *yf = xf + lds_slots, *zf = xf + 2 * lds_slots;
-    unsigned char *scell = reinterpret_cast<unsigned char *>(xf + 3 * lds_slots);
-    slot_map(H, scell);
-    for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-      const int h = scell[s];
-      const double4 p = pos[s + H.delta[h]];
-      xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
-      yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
-      zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
-    }
-    __syncthreads();
-    build_rows<true>(H, t, pos, cellof, xf, yf, zf, far, rl2, near2f, lo2f, hi2f, cap, pad, ld,
-                     ncap, nn, list16, xold, mmax);
   } else {
-    build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, rl2, near2f, lo2f,
-                      hi2f, cap, pad, ld, ncap, nn, list16, xold, mmax);
+    unsigned short *far = reinterpret_cast<unsigned short *>(dynf);
+    float *xf = dynf + GH_FARMAX * GH_TBLOCK / 2;
+    if (H.total <= lds_slots) {
+      float *yf = xf + lds_slots, *zf = xf + 2 * lds_slots;
+      unsigned char *scell = reinterpret_cast<unsigned char *>(xf + 3 * lds_slots);
+      slot_map(H, scell);
+      for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+        const int h = scell[s];
+        const double4 p = pos[s + H.delta[h]];
+        xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
+        yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
+        zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
+      }
+      __syncthreads();
+      build_rows<true>(H, t, pos, cellof, xf, yf, zf, far, A.rl2, A.near2f, A.lo2f, A.hi2f, cap,
+                       pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
+    } else {
+      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A.rl2, A.near2f,
+                        A.lo2f, A.hi2f, cap, pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
+    }
   }
   mmax = wave_imax(mmax);
   if ((threadIdx.x & 63) == 0 && mmax > 0) {
     atomicMax(&st->maxnn, mmax);
     if (mmax > cap) st->overflow = 1;
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
+    const int *__restrict__ cell_start, BuildArgs A) {
+  if (!st->rebuild) return;
+  extern __shared__ float dynf[];
+  __shared__ Halo H;
+  const int b = xcd_block();
+  if (b >= t.nbrick) return;
+  build_brick(st, t, pos, cellof, cell_start, A, dynf, H, b);
 }
 
 // Largest halo and brick (atoms) over all bricks; sizes slot_cap.
@@ -798,12 +1015,13 @@ static double gh_near2(const gh_ctx *c) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
+// Build arguments and dynamic LDS of the tiled list build.
+static BuildArgs build_args(const gh_ctx *c, int cap, size_t *lds_out) {
+  BuildArgs A;
   int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
   size_t lds = build_far_bytes() + 13 * (size_t)lds_slots;
   // rows in LDS when they and the staged halo fit (the branch-free variant)
-  const int capb = count_only ? 0 : cap;
-  const size_t rows = ((size_t)(capb + 1) * GH_TBLOCK * 2 + 3) / 4 * 4;
+  const size_t rows = ((size_t)(cap + 1) * GH_TBLOCK * 2 + 3) / 4 * 4;
   const size_t slots_left = rows < GH_BUILD_LDS ? (GH_BUILD_LDS - rows) / 13 : 0;
   int lds_rows = 0;
   if (slots_left >= (size_t)c->tg.slot_cap || slots_left >= 2048) {
@@ -812,14 +1030,63 @@ int gh_launch_build_tiled(gh_ctx *c, int cap, int count_only) {
     lds = rows + 13 * (size_t)lds_slots;
   }
   const double rl = c->rcut + c->skin;
-  const double rl2 = rl * rl;
+  A.rl2 = rl * rl;
   // FP32 prefilter: |r^2_f32 - r^2| stays below 3e-4 A^2 for local
   // coordinates within 40 A of the brick centre; the margin is ten times that
-  const double margin = std::max(3e-3, 1e-4 * rl2);
-  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, c->stream>>>(
-      c->status, c->tg, c->pos, c->cellof, c->cell_start, rl2, (float)gh_near2(c),
-      (float)(rl2 - margin), (float)(rl2 + margin), capb, c->ncap, c->ncap, lds_slots,
-      lds_rows, c->nn, c->list16, c->xold);
+  const double margin = std::max(3e-3, 1e-4 * A.rl2);
+  A.near2f = (float)gh_near2(c);
+  A.lo2f = (float)(A.rl2 - margin);
+  A.hi2f = (float)(A.rl2 + margin);
+  A.cap = cap;
+  A.ld = c->ncap;
+  A.ncap = c->ncap;
+  A.lds_slots = lds_slots;
+  A.lds_rows = lds_rows;
+  A.nn = c->nn;
+  A.list16 = c->list16;
+  A.xold = c->xold;
+  *lds_out = lds;
+  return A;
+}
+
+
+int gh_launch_rebuild_cells(gh_ctx *c) {
+  CellArgs R;
+  R.st = c->status;
+  R.t = c->tg;
+  R.n = c->n;
+  R.ncap = c->ncap;
+  R.nkey = c->tg.nkey;
+  R.pos = c->pos;
+  R.pos_t = c->pos_t;
+  R.vel = c->vel;
+  R.vel_t = c->vel_t;
+  R.acc = c->acc;
+  R.acc_t = c->acc_t;
+  R.type = c->type;
+  R.type_t = c->type_t;
+  R.orig = c->orig;
+  R.orig_t = c->orig_t;
+  R.cellid = c->cellid;
+  R.cellof = c->cellof;
+  R.count = c->cell_count;
+  R.start = c->cell_start;
+  R.cursor = c->cell_cursor;
+  R.perm = c->perm;
+  size_t lds = 0;
+  const BuildArgs A = build_args(c, c->cap, &lds);
+  hipStream_t s = c->stream;
+  // grid-stride launches of at most 2048 workgroups: cheap when no rebuild is due
+  const int ga = std::min((R.n + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
+  const int gk = std::min((R.nkey + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
+  k_cell_bin<<<ga, GH_TBLOCK, 0, s>>>(R);
+  k_cell_scan<<<1, GH_SCANBLOCK, 0, s>>>(R);
+  k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(R);
+  k_cell_sort<<<gk, GH_TBLOCK, 0, s>>>(R);
+  k_cell_gather<<<ga, GH_TBLOCK, 0, s>>>(R);
+  k_cell_copyback<<<ga, GH_TBLOCK, 0, s>>>(R);
+  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cellof,
+                                                            c->cell_start, A);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
