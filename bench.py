@@ -113,6 +113,25 @@ def cpu_baseline(x, L, skin, budget_s=20.0):
                        "%d rebuilds)" % (steps, len(x), cores, md.nupd - 1))
 
 
+def nve_drift(hip, precision, x, L, skin, nsteps, tile=None, each=100):
+    """Total-energy drift of velocity-Verlet NVE over nsteps (eV/atom): the
+    energy columns of write_energy every `each` steps."""
+    c = make_ctx(hip, precision, x, L, skin)
+    if tile:
+        c.set_tile(tile)
+    ep0, ek0 = c.energies()
+    e0 = ep0 + ek0
+    t0 = time.perf_counter()
+    en = c.run(nsteps, each=each)
+    dt = time.perf_counter() - t0
+    c.close()
+    n = len(x)
+    dev = (en[:, 2] - e0) / n
+    return {"steps": nsteps, "drift_ev_per_atom": float(dev[-1]),
+            "max_abs_dev_ev_per_atom": float(np.max(np.abs(dev))),
+            "e0_ev": float(e0), "seconds": dt}
+
+
 def batch_re(hip, rounds, swap_every=500):
     """Replica-exchange Langevin of 512 Ag-Cu clusters (64 geometries x 8
     temperatures, 300..900 K geometric), friction 1/ps, dt 2 fs, swaps every
@@ -201,6 +220,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--re-rounds", type=int, default=4)
     ap.add_argument("--fire-clusters", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
@@ -297,6 +317,12 @@ def main():
                   "fp64": {"brick_cells": tile_64[0], "force_eval_us": tile_64[1]}},
         "step_breakdown_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in tm_mx.items()},
     }
+    if args.drift_steps > 0:
+        out["nve_drift"] = {
+            "mixed": nve_drift(hip, hip.GH_MIXED, x, L, args.skin, args.drift_steps, tile_mx[0]),
+            "fp64": nve_drift(hip, hip.GH_FP64, x, L, args.skin, args.drift_steps, tile_64[0]),
+            "workload": "the same 1,000,188-atom Au NVE run, energy every 100 steps",
+        }
     if not args.no_batch:
         out["batched_re"] = batch_re(hip, args.re_rounds)
         out["batched_fire"] = batch_fire(hip, args.fire_clusters)

@@ -172,6 +172,87 @@ module gems_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: x(*), v(*), F(*), e(*)
     end function
+    ! ---- batched free clusters (gh_batch_*): replica-exchange ladders and
+    !      FIRE relaxations of many systems in one launch
+    integer(c_int) function gh_batch_create(b, device, precision) bind(C, name='gh_batch_create')
+      import :: c_ptr, c_int
+      type(c_ptr), intent(out) :: b
+      integer(c_int), value :: device, precision
+    end function
+
+    subroutine gh_batch_destroy(b) bind(C, name='gh_batch_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end subroutine
+
+    integer(c_int) function gh_batch_set_systems(b, nsys, off, x, type) &
+        bind(C, name='gh_batch_set_systems')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: nsys
+      integer(c_int), intent(in) :: off(nsys + 1)
+      real(c_double), intent(in) :: x(*)
+      integer(c_int), intent(in) :: type(*)
+    end function
+
+    integer(c_int) function gh_batch_set_smatb(b, nt, A, xi, p, q, r0, rci, rce) &
+        bind(C, name='gh_batch_set_smatb')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: A(nt, nt), xi(nt, nt), p(nt, nt), q(nt, nt), &
+                                    r0(nt, nt), rci(nt, nt), rce(nt, nt)
+    end function
+
+    integer(c_int) function gh_batch_set_masses(b, nt, mass) bind(C, name='gh_batch_set_masses')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: mass(nt)
+    end function
+
+    integer(c_int) function gh_batch_set_temperatures(b, T) &
+        bind(C, name='gh_batch_set_temperatures')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: b
+      real(c_double), intent(in) :: T(*)
+    end function
+
+    integer(c_int) function gh_batch_langevin(b, nsteps, dt, gamma, seed, epot, ekin) &
+        bind(C, name='gh_batch_langevin')
+      import :: c_ptr, c_int, c_double, c_int64_t
+      type(c_ptr), value :: b
+      integer(c_int), value :: nsteps
+      real(c_double), value :: dt, gamma
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: epot(*), ekin(*)
+    end function
+
+    integer(c_int) function gh_batch_exchange(b, nladder, R, sys, parity, seed, naccept) &
+        bind(C, name='gh_batch_exchange')
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: b
+      integer(c_int), value :: nladder, R, parity
+      integer(c_int), intent(inout) :: sys(*)
+      integer(c_int64_t), value :: seed
+      integer(c_int), intent(out) :: naccept
+    end function
+
+    integer(c_int) function gh_batch_fire(b, max_iter, dt_max, alpha0, fswitch, ftol, iters, &
+                                          converged, epot, fmax) bind(C, name='gh_batch_fire')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: max_iter
+      real(c_double), value :: dt_max, alpha0, fswitch, ftol
+      integer(c_int), intent(out) :: iters(*), converged(*)
+      real(c_double), intent(out) :: epot(*), fmax(*)
+    end function
+
+    integer(c_int) function gh_batch_get_state(b, x, v) bind(C, name='gh_batch_get_state')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: b
+      real(c_double), intent(out) :: x(*), v(*)
+    end function
   end interface
 
 contains
