@@ -184,6 +184,11 @@ int gh_upload_tb(gh_ctx *c) {
     o.r1f = d.r1f;
     o.r2f = d.r2f;
     o.r3f = d.r3f;
+    o.r0q = d.r0f * 8388608.0f;
+    o.rciq = d.rci_f * 8388608.0f;
+    o.rceq = d.rce_f * 8388608.0f;
+    o.rep_bq = d.rep_b * 1.1920928955078125e-07f;  // 2^-23
+    o.rho_bq = d.rho_b * 1.1920928955078125e-07f;
   }
   GH_TRY(dalloc(c, &c->tbd, GH_MAXT * GH_MAXT));
   GH_HIP(c, hipMemcpy(c->tbd, h, sizeof(h), hipMemcpyHostToDevice));

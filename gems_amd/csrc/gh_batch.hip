@@ -276,7 +276,7 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
           ei += (double)repul;
         }
       });
-      scale = B.ev_ui * (1.0 / GH_FXS);
+      scale = B.ev_ui;  // tb_force_fx returns factor / r_q: forces come out in ui/A
     } else {
       const double4 pi = B.pos[i];
       const double ebi = B.eband[i];

@@ -173,24 +173,67 @@ __device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__
 }
 
 
+// As for_each_slot, one call per quad with its four packed slots (the
+// caller issues the four LDS reads together).
+template <class F>
+__device__ __forceinline__ void for_each_quad(const unsigned short *__restrict__ list16, int ld,
+                                              int a, int m, F &&f) {
+  const int ng = (m + 3) >> 2;
+  if (ng == 0) return;
+  const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
+  const size_t sld = (size_t)ld;
+  unsigned long long w0 = ld_quad(lp);
+  unsigned long long w1 = ld_quad(lp + (size_t)min(1, ng - 1) * sld);
+  for (int g = 0; g < ng; g += 2) {
+    const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
+    const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
+    f(w0);
+    if (g + 1 < ng) f(w1);
+    w0 = n0;
+    w1 = n1;
+  }
+}
+#define GH_QSLOT(w, u) ((int)(((w) >> (16 * (u))) & 0xFFFFull))
+
 // SMA-TB pair terms on fixed-point pair vectors (integer steps, exact in
-// FP64): r^2 in FP64, then the FP32 forms.  rho: the density term;
-// force: factor * (1/r) per step of the pair vector, and the repulsion.
-template <class TP>
-__device__ __forceinline__ bool tb_rho_fx(const TP &P, double r2q, float &band) {
+// FP64): r^2 in FP64, then the FP32 forms of TbDev/TbF evaluated in
+// fixed-point length units (r_q = r * 2^23; every scaling is by a power of
+// two, so each FP32 result equals the Angstrom-unit one bit for bit).
+// rho: the density term; force: factor/r_q (per step of the pair vector)
+// and the repulsion.
+__device__ __forceinline__ float tb_band_q(const TbF &P, float rq) {
+  if (rq < P.rciq) return P.rho_a * __builtin_amdgcn_exp2f(P.rho_bq * (rq - P.r0q));
+  const float d = (rq - P.rceq) * 1.1920928955078125e-07f;
+  const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
+  return pp * pp;
+}
+__device__ __forceinline__ bool tb_rho_fx(const TbF &P, double r2q, float &band) {
   if (r2q > P.rce2q) return false;
-  const float r2f = (float)(r2q * (1.0 / GH_FXS2));
-  band = tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
+  const float r2f = (float)r2q;
+  band = tb_band_q(P, r2f * __builtin_amdgcn_rsqf(r2f));
   return true;
 }
-template <class TP>
-__device__ __forceinline__ bool tb_force_fx(const TP &P, double r2q, float ebs, float &sf,
+__device__ __forceinline__ bool tb_force_fx(const TbF &P, double r2q, float ebs, float &sf,
                                             float &repul) {
   if (r2q > P.rce2q) return false;
-  const float r2f = (float)(r2q * (1.0 / GH_FXS2));
-  const float rinv = __builtin_amdgcn_rsqf(r2f);
+  const float r2f = (float)r2q;
+  const float rinv = __builtin_amdgcn_rsqf(r2f);  // 1 / r_q
+  const float rq = r2f * rinv;
   float factor;
-  tb_pair32(P, r2f * rinv, ebs, factor, repul);
+  if (rq <= P.rciq) {
+    const float s0 = rq - P.r0q;
+    repul = P.rep_a * __builtin_amdgcn_exp2f(P.rep_bq * s0);
+    const float band = P.rho_a * __builtin_amdgcn_exp2f(P.rho_bq * s0);
+    factor = fmaf(repul, P.frep, -band * P.fband * ebs);
+  } else {
+    const float d = (rq - P.rceq) * 1.1920928955078125e-07f;
+    const float d2 = d * d, d3 = d2 * d;
+    const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
+    const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
+    const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
+    factor = fmaf(pp * dp, ebs, -2.0f * dr_);
+    repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
+  }
   sf = factor * rinv;
   return true;
 }

@@ -68,7 +68,10 @@ struct TbDev {
 struct TbF {
   double rce2q;
   float r0f, rep_a, rep_b, rho_a, rho_b, frep, fband, rci_f, rce_f;
-  float b1f, b2f, b3f, r1f, r2f, r3f, pad;
+  float b1f, b2f, b3f, r1f, r2f, r3f;
+  // the same in fixed-point length units (r_q = r * 2^23, exact scalings):
+  // r0q, rciq, rceq = r0f, rci_f, rce_f * 2^23; rep_bq, rho_bq = rep_b, rho_b * 2^-23
+  float r0q, rciq, rceq, rep_bq, rho_bq, pad;
 };
 #define GH_FXS 8388608.0         // fixed-point steps per Angstrom (2^23)
 #define GH_FXS2 70368744177664.0 // GH_FXS^2 (2^46)

@@ -698,15 +698,20 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
               qz = fxq(pa.z - (H.org[2] - 0.0));
     const int ti = SINGLE ? 0 : type[a];
     double acc = 0.0;
-    for_each_slot(list16, ld, a, nn[a], [&](int s) {
-      const int4 q = q4[s];
+    auto pair = [&](const int4 &q) {
       const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
-      const double r2q = fma(dx, dx, fma(dy, dy, dz * dz));
-      const TbF &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + q.w];
-      if (r2q <= P.rce2q) {
-        const float r2f = (float)(r2q * (1.0 / GH_FXS2));
-        acc += (double)tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
-      }
+      float band;
+      if (tb_rho_fx(SINGLE ? sp[0] : sp[ti * GH_MAXT + q.w], fma(dx, dx, fma(dy, dy, dz * dz)),
+                    band))
+        acc += (double)band;
+    };
+    for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
+      const int4 q0 = q4[GH_QSLOT(w, 0)], q1 = q4[GH_QSLOT(w, 1)], q2 = q4[GH_QSLOT(w, 2)],
+                 q3 = q4[GH_QSLOT(w, 3)];
+      pair(q0);
+      pair(q1);
+      pair(q2);
+      pair(q3);
     });
     rho[a] = acc;
     eband32[a] = (float)(1.0 / sqrt(acc));
@@ -746,7 +751,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
-  const double fscale = ev_ui * (1.0 / GH_FXS);
+  const double fscale = ev_ui;  // tb_force_fx returns factor / r_q: forces come out in ui/A
   double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
@@ -755,22 +760,25 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     const int ti = SINGLE ? 0 : type[a];
     const float ebi = eband32[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
-    for_each_slot(list16, ld, a, nn[a], [&](int s) {
-      const int4 q = q4[s];
+    auto pair = [&](const int4 &q, int s) {
       const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
-      const double r2q = fma(dx, dx, fma(dy, dy, dz * dz));
-      const TbF &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
-      if (r2q <= P.rce2q) {
-        const float r2f = (float)(r2q * (1.0 / GH_FXS2));
-        const float rinv = __builtin_amdgcn_rsqf(r2f);
-        float factor, repul;
-        tb_pair32(P, r2f * rinv, ebi + __int_as_float(q.w), factor, repul);
-        const double sf = (double)(factor * rinv);  // per fixed-point step
-        fx = fma(-sf, dx, fx);
-        fy = fma(-sf, dy, fy);
-        fz = fma(-sf, dz, fz);
+      float sf, repul;
+      if (tb_force_fx(SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]],
+                      fma(dx, dx, fma(dy, dy, dz * dz)), ebi + __int_as_float(q.w), sf, repul)) {
+        const double sfd = (double)sf;
+        fx = fma(-sfd, dx, fx);
+        fy = fma(-sfd, dy, fy);
+        fz = fma(-sfd, dz, fz);
         ei += (double)repul;
       }
+    };
+    for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
+      const int s0 = GH_QSLOT(w, 0), s1 = GH_QSLOT(w, 1), s2 = GH_QSLOT(w, 2), s3 = GH_QSLOT(w, 3);
+      const int4 q0 = q4[s0], q1 = q4[s1], q2 = q4[s2], q3 = q4[s3];
+      pair(q0, s0);
+      pair(q1, s1);
+      pair(q2, s2);
+      pair(q3, s3);
     });
     const double Fx = fx * fscale, Fy = fy * fscale, Fz = fz * fscale;
     frc[a] = Fx;
