@@ -213,14 +213,21 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
     double acc = 0.0;
     if (fx) {
       const int4 qi = S.q[k];
-      for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
-        const int4 q = S.q[s];
+      auto pair = [&](const int4 &q) {
         const double dx = (double)(q.x - qi.x), dy = (double)(q.y - qi.y),
                      dz = (double)(q.z - qi.z);
         float band;
         if (tb_rho_fx(SINGLE ? S.sf[0] : S.sf[ti * GH_MAXT + q.w],
                       fma(dx, dx, fma(dy, dy, dz * dz)), band))
           acc += (double)band;
+      };
+      for_each_quad(B.list, B.ld, i, B.nn[i], [&](unsigned long long w) {
+        const int4 q0 = S.q[GH_QSLOT(w, 0)], q1 = S.q[GH_QSLOT(w, 1)], q2 = S.q[GH_QSLOT(w, 2)],
+                   q3 = S.q[GH_QSLOT(w, 3)];
+        pair(q0);
+        pair(q1);
+        pair(q2);
+        pair(q3);
       });
     } else {
       const double4 pi = B.pos[i];
@@ -261,8 +268,7 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
     if (fx) {
       const int4 qi = S.q[k];
       const float ebi = __int_as_float(qi.w);
-      for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
-        const int4 q = S.q[s];
+      auto pair = [&](const int4 &q, int s) {
         const double dx = (double)(q.x - qi.x), dy = (double)(q.y - qi.y),
                      dz = (double)(q.z - qi.z);
         float sf, repul;
@@ -275,6 +281,15 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
           fzs = fma(-sfd, dz, fzs);
           ei += (double)repul;
         }
+      };
+      for_each_quad(B.list, B.ld, i, B.nn[i], [&](unsigned long long w) {
+        const int s0 = GH_QSLOT(w, 0), s1 = GH_QSLOT(w, 1), s2 = GH_QSLOT(w, 2),
+                  s3 = GH_QSLOT(w, 3);
+        const int4 q0 = S.q[s0], q1 = S.q[s1], q2 = S.q[s2], q3 = S.q[s3];
+        pair(q0, s0);
+        pair(q1, s1);
+        pair(q2, s2);
+        pair(q3, s3);
       });
       scale = B.ev_ui;  // tb_force_fx returns factor / r_q: forces come out in ui/A
     } else {
