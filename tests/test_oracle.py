@@ -319,3 +319,28 @@ def test_ermak_coefficients_closed_form():
     assert k["sdr"] ** 2 == pytest.approx(dt / g * (2 - (3 - 4 * c0 + c0 ** 2) / (g * dt)),
                                           rel=1e-12)
     assert k["crv1"] ** 2 + k["crv2"] ** 2 == pytest.approx(1.0, rel=1e-14)
+
+
+def test_fire_reaches_the_reference_lbfgs_minimum():
+    """or_fire (Bitzek et al. 2006) from the start of examples/SMA-TB/md.gms
+    reaches the minimum the reference's own L-BFGS found (the ag13_lbfgs
+    fixture): same energy, same shape (sorted pair distances), and an
+    independent force evaluation there is below the tolerance."""
+    lines = open(cm.GOLDEN + "/ag13_coords.xyz").read().splitlines()[2:15]
+    x0 = np.array([[float(t) for t in ln.split()[1:4]] for ln in lines])
+    ref = cm.load("ag13_lbfgs")
+    prm = [(0, 0, 0.1031, 1.1899, 10.85, 3.18, 2.89, 2.89 * np.sqrt(2.0), 2.89 * np.sqrt(3.0))]
+    s = O.smatb_params(prm, 1)
+    r = O.fire(x0, np.zeros(13), s, np.full(13, O.element_mass(47)), ftol=1e-5)
+    assert r["converged"]
+    assert r["epot"] == pytest.approx(float(ref["epot_ev"]), abs=1e-9 * 13)
+
+    def dists(x):
+        d = np.linalg.norm(x[:, None] - x[None], axis=-1)
+        return np.sort(d[np.triu_indices(len(x), 1)])
+
+    assert np.abs(dists(r["x"]) - dists(ref["x"])).max() < 1e-4
+    nn, lst, _ = O.neighbours(r["x"], O.make_box(None), O.smatb_rcut(s) + 1.0)
+    F, _, ep, _ = O.smatb_force(r["x"], np.zeros(13), O.make_box(None), s, nn, lst)
+    assert np.sqrt((F ** 2).sum(axis=1)).max() * cm.UI_EV < 1e-5
+    assert ep * cm.UI_EV == pytest.approx(r["epot"], rel=1e-13)
