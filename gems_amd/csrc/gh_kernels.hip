@@ -54,18 +54,37 @@ __global__ void k_fill_int(int *__restrict__ p, int v, int n) {
 __global__ __launch_bounds__(1024) void k_decide(GhStatus *st, double *__restrict__ dispblk,
                                                  int nblk, double skin) {
   __shared__ double part[16];
-  double m = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-    m = fmax(m, dispblk[b]);
-    dispblk[b] = 0.0;  // consumed: the next integrator step writes fresh maxima
+  // The status words and eight maxima per thread are fetched together: the
+  // kernel is a chain of memory round trips, so it pays one instead of many.
+  int frb = 0;
+  unsigned long long md = 0ull;
+  if (threadIdx.x == 0) {
+    frb = st->force_rebuild;
+    md = st->maxdisp2_bits;
+  }
+  double m = 0.0;  // maxima are squared displacements, >= 0
+  constexpr int U = 8;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * (int)blockDim.x) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * (int)blockDim.x;
+      v[u] = b < nblk ? dispblk[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * (int)blockDim.x;
+      if (b < nblk) dispblk[b] = 0.0;  // consumed: the next integrator step writes fresh maxima
+      m = fmax(m, v[u]);
+    }
   }
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, part[w]);
-    const double d2 = fmax(m, __longlong_as_double((long long)st->maxdisp2_bits));
-    const int rb = st->force_rebuild || (2.0 * sqrt(d2) > skin);
+    const double d2 = fmax(m, __longlong_as_double((long long)md));
+    const int rb = frb || (2.0 * sqrt(d2) > skin);
     st->rebuild = rb;
     if (rb) {
       st->nupd += 1;

@@ -198,100 +198,136 @@ __device__ __forceinline__ size_t qidx(int k, int a, int ld) {
 // One sweep.  With the halo staged (STAGE), a candidate is first tested in
 // FP32 on brick-local coordinates; only candidates within `margin` of rl^2
 // (FP32 error bound x 10) are decided by the exact FP64 rd from L2.
-// Row order: pairs with r^2 < near2 (the potential's inner branch: r < rci
-// for SMA-TB, r < rc for LJ) first, then the rest, which wait in an LDS
-// buffer (GH_FARMAX per thread; any excess joins the first run, which only
-// costs order).  Lanes of a wavefront then take the same branch of the pair
-// function on most iterations.  The last quad is padded with `pad`, a
-// sentinel LDS slot far from every atom.
+//
+// Row order.  Each row lists its pairs by class of rd at build time, the
+// candidate order kept within a class: 0 below (rin - m)^2, 1 below
+// (rin + m)^2, 2 below (rout + m)^2, 3 the rest, where rin is the bound of
+// the pair function's inner branch (min rci for SMA-TB, rc for LJ), rout its
+// cutoff (max rce, rc) and m a quarter of the skin.  Entry k of every row of
+// a wavefront then mostly falls in one class, so the lanes take one branch of
+// the pair function, and the trailing class-3 quads (pairs that stay beyond
+// the cutoff until they move m) are skipped by the whole wavefront.  The
+// class is decided in FP32 unless r^2 lies within `margin` of a bound, where
+// the exact FP64 rd decides: the order, and with it the FP64 sums, do not
+// depend on the brick shape.  The last quad is padded with `pad`, a sentinel
+// LDS slot far from every atom.
 // ---------------------------------------------------------------------------
 #define GH_FARMAX 48
 
+struct BuildArgs {
+  double rl2;
+  float lo2f, hi2f;  // FP32 prefilter band around rl^2
+  double cls2[3];    // row-order class bounds on r^2 (non-decreasing)
+  float clsf[3];     // the same in FP32
+  float clsm;        // within clsm of a bound the FP64 r^2 decides the class
+  int cap, ld, ncap, lds_slots, lds_rows;
+  int *nn;
+  unsigned short *list16;
+  double *xold;
+};
+
+__device__ __forceinline__ int row_class(double rd, const BuildArgs &A) {
+  return (rd >= A.cls2[0]) + (rd >= A.cls2[1]) + (rd >= A.cls2[2]);
+}
+
+// Class from the FP32 r^2, or -1 when it lies within clsm of a bound.
+__device__ __forceinline__ int row_class_f(float r2f, const BuildArgs &A) {
+  const float lo = r2f - A.clsm, hi = r2f + A.clsm;
+  const int cl = (lo >= A.clsf[0]) + (lo >= A.clsf[1]) + (lo >= A.clsf[2]);
+  const int ch = (hi >= A.clsf[0]) + (hi >= A.clsf[1]) + (hi >= A.clsf[2]);
+  return cl == ch ? cl : -1;
+}
+
+// rd of atom a (position pa) and slot s of halo cell h, rounded as the
+// reference's vdistance + dot_product.
+__device__ __forceinline__ double pair_rd(const Halo &H, const TileGeom &t,
+                                          const double4 *__restrict__ pos, const double4 &pa,
+                                          int s, int h) {
+  const double4 pj = pos[s + H.delta[h]];
+  const double vx = (pj.x - pa.x) + shift(t, H, 0, h);
+  const double vy = (pj.y - pa.y) + shift(t, H, 1, h);
+  const double vz = (pj.z - pa.z) + shift(t, H, 2, h);
+  return dist2_rn(vx, vy, vz);
+}
+
+// Rows through a sequential writer; pairs of classes 0-2 go out in candidate
+// order, class 3 waits in an LDS buffer (GH_FARMAX per thread; any excess
+// joins the first run, which only costs order).
 template <bool STAGE>
 __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
                                            const double4 *__restrict__ pos,
                                            const int *__restrict__ cellof, const float *xf,
                                            const float *yf, const float *zf,
-                                           unsigned short *far, double rl2, float near2f,
-                                           float lo2f, float hi2f, int cap, int pad, int ld,
-                                           int ncap, int *__restrict__ nn,
-                                           unsigned short *__restrict__ list16,
-                                           double *__restrict__ xold, int &mmax) {
+                                           unsigned short *far, const BuildArgs &A, int pad,
+                                           int &mmax) {
+  const int cap = A.cap;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const int lc = cellof[a] - H.key0;
     const int lx = lc % t.bd[0], ly = (lc / t.bd[0]) % t.bd[1], lz = lc / (t.bd[0] * t.bd[1]);
     const int ha = (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
     const int own = a - H.delta[ha];
     const double4 pa = pos[a];
-    const double xi = pa.x, yi = pa.y, zi = pa.z;
     float xfi = 0.f, yfi = 0.f, zfi = 0.f;
     if (STAGE) {
       xfi = xf[own];
       yfi = yf[own];
       zfi = zf[own];
     }
-    RowWriter row{reinterpret_cast<unsigned long long *>(list16), ld, a, cap, 0, 0ull};
+    RowWriter row{reinterpret_cast<unsigned long long *>(A.list16), A.ld, a, cap, 0, 0ull};
     int nfar = 0;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
           const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
           const int se = H.hbase[h + 1];
-          const int dl = H.delta[h];
           for (int s = H.hbase[h]; s < se; ++s) {
             if (s == own) continue;
-            bool near;
+            int c;
             if (STAGE) {
               const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
               const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
-              if (r2f > hi2f) continue;
-              if (r2f >= lo2f) {  // near the cut: the reference's rounding decides
-                const double4 pj = pos[s + dl];
-                const double vx = (pj.x - xi) + shift(t, H, 0, h);
-                const double vy = (pj.y - yi) + shift(t, H, 1, h);
-                const double vz = (pj.z - zi) + shift(t, H, 2, h);
-                if (!(dist2_rn(vx, vy, vz) < rl2)) continue;
+              if (r2f > A.hi2f) continue;
+              double rd = -1.0;
+              if (r2f >= A.lo2f) {  // near the cut: the reference's rounding decides
+                rd = pair_rd(H, t, pos, pa, s, h);
+                if (!(rd < A.rl2)) continue;
               }
-              near = r2f < near2f;
+              c = row_class_f(r2f, A);
+              if (c < 0) c = row_class(rd >= 0.0 ? rd : pair_rd(H, t, pos, pa, s, h), A);
             } else {
-              const double4 pj = pos[s + dl];
-              const double vx = (pj.x - xi) + shift(t, H, 0, h);
-              const double vy = (pj.y - yi) + shift(t, H, 1, h);
-              const double vz = (pj.z - zi) + shift(t, H, 2, h);
-              const double rd = dist2_rn(vx, vy, vz);
-              if (!(rd < rl2)) continue;
-              near = rd < (double)near2f;
+              const double rd = pair_rd(H, t, pos, pa, s, h);
+              if (!(rd < A.rl2)) continue;
+              c = row_class(rd, A);
             }
-            if (near || nfar == GH_FARMAX) row.put(s);
+            if (c < 3 || nfar == GH_FARMAX) row.put(s);
             else far[(nfar++) * GH_TBLOCK + threadIdx.x] = (unsigned short)s;
           }
         }
     for (int k = 0; k < nfar; ++k) row.put(far[k * GH_TBLOCK + threadIdx.x]);
     row.finish(pad);
-    nn[a] = min(row.m, cap);
-    xold[a] = xi;
-    xold[ncap + a] = yi;
-    xold[2 * ncap + a] = zi;
+    A.nn[a] = min(row.m, cap);
+    A.xold[a] = pa.x;
+    A.xold[A.ncap + a] = pa.y;
+    A.xold[2 * A.ncap + a] = pa.z;
     mmax = max(mmax, row.m);
   }
 }
 
 // Staged build with the rows assembled in LDS (rb: entry k of this thread's
 // row at rb[k * GH_TBLOCK], k = cap is a dump slot): every candidate is
-// written through a computed index, near pairs from the front and the rest
-// from the back, so the candidate loop has no per-pair branch but the rare
-// exact test.  The rows are then stored as whole quads.
+// written through a computed index, so the candidate loop has no per-pair
+// branch but the rare exact test.  A second sweep over the row tags each
+// entry with its class (bits 14-15; staged slots number below 16384), and
+// four more write the row out class by class as whole quads.
 __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
                                                const double4 *__restrict__ pos,
                                                const int *__restrict__ cellof, const float *xf,
                                                const float *yf, const float *zf,
-                                               unsigned short *rowbuf, double rl2,
-                                               float near2f, float lo2f, float hi2f, int cap,
-                                               int pad, int ld, int ncap, int *__restrict__ nn,
-                                               unsigned short *__restrict__ list16,
-                                               double *__restrict__ xold, int &mmax) {
+                                               const unsigned char *scell,
+                                               unsigned short *rowbuf, const BuildArgs &A,
+                                               int pad, int &mmax) {
   unsigned short *rb = rowbuf + threadIdx.x;
-  unsigned long long *lq = reinterpret_cast<unsigned long long *>(list16);
+  const int cap = A.cap;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const int lc = cellof[a] - H.key0;
     const int lx = lc % t.bd[0], ly = (lc / t.bd[0]) % t.bd[1], lz = lc / (t.bd[0] * t.bd[1]);
@@ -299,7 +335,7 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
     const int own = a - H.delta[ha];
     const double4 pa = pos[a];
     const float xfi = xf[own], yfi = yf[own], zfi = zf[own];
-    int mn = 0, mf = 0;
+    int m = 0;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
@@ -308,38 +344,32 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
           for (int s = H.hbase[h]; s < se; ++s) {
             const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
             const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
-            bool keep = s != own && r2f <= hi2f;
-            if (keep && r2f >= lo2f) {  // near the cut: the reference's rounding decides
-              const double4 pj = pos[s + H.delta[h]];
-              const double vx = (pj.x - pa.x) + shift(t, H, 0, h);
-              const double vy = (pj.y - pa.y) + shift(t, H, 1, h);
-              const double vz = (pj.z - pa.z) + shift(t, H, 2, h);
-              keep = dist2_rn(vx, vy, vz) < rl2;
-            }
-            const bool near = r2f < near2f;
-            int k = near ? mn : cap - 1 - mf;
-            if (!keep || (unsigned)k >= (unsigned)cap) k = cap;
-            rb[k * GH_TBLOCK] = (unsigned short)s;
-            mn += (keep && near) ? 1 : 0;
-            mf += (keep && !near) ? 1 : 0;
+            bool keep = s != own && r2f <= A.hi2f;
+            if (keep && r2f >= A.lo2f)  // near the cut: the reference's rounding decides
+              keep = pair_rd(H, t, pos, pa, s, h) < A.rl2;
+            rb[(keep && m < cap ? m : cap) * GH_TBLOCK] = (unsigned short)s;
+            m += keep ? 1 : 0;
           }
         }
-    const int m = mn + mf, mw = min(m, cap);
-    for (int k0 = 0; k0 < mw; k0 += 4) {
-      unsigned long long w = 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = k0 + u;
-        const int sl = k < mw ? (k < mn ? rb[k * GH_TBLOCK] : rb[(cap - 1 - (k - mn)) * GH_TBLOCK])
-                              : pad;
-        w |= (unsigned long long)sl << (16 * u);
-      }
-      lq[(size_t)(k0 >> 2) * ld + a] = w;
+    const int mw = min(m, cap);
+    for (int k = 0; k < mw; ++k) {
+      const int s = rb[k * GH_TBLOCK];
+      const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
+      int c = row_class_f(fmaf(ux, ux, fmaf(uy, uy, uz * uz)), A);
+      if (c < 0) c = row_class(pair_rd(H, t, pos, pa, s, scell[s]), A);
+      rb[k * GH_TBLOCK] = (unsigned short)(s | (c << 14));
     }
-    nn[a] = mw;
-    xold[a] = pa.x;
-    xold[ncap + a] = pa.y;
-    xold[2 * ncap + a] = pa.z;
+    RowWriter row{reinterpret_cast<unsigned long long *>(A.list16), A.ld, a, cap, 0, 0ull};
+    for (int c = 0; c < 4; ++c)
+      for (int k = 0; k < mw; ++k) {
+        const int e = rb[k * GH_TBLOCK];
+        if ((e >> 14) == c) row.put(e & 0x3FFF);
+      }
+    row.finish(pad);
+    A.nn[a] = mw;
+    A.xold[a] = pa.x;
+    A.xold[A.ncap + a] = pa.y;
+    A.xold[2 * A.ncap + a] = pa.z;
     mmax = max(mmax, m);
   }
 }
@@ -549,15 +579,6 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_cell_copyback(CellArgs R) {
   cell_copyback(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
-struct BuildArgs {
-  double rl2;
-  float near2f, lo2f, hi2f;  // inner-branch bound; FP32 prefilter band around rl^2
-  int cap, ld, ncap, lds_slots, lds_rows;
-  int *nn;
-  unsigned short *list16;
-  double *xold;
-};
-
 // Rows of the atoms of brick b (all threads of the workgroup call it; LDS
 // is free again on return).
 __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
@@ -583,13 +604,11 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
         zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
       }
       __syncthreads();
-      build_rows_lds(H, t, pos, cellof, xf, yf, zf, rowbuf, A.rl2, A.near2f, A.lo2f, A.hi2f,
-                     cap, pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
+      build_rows_lds(H, t, pos, cellof, xf, yf, zf, scell, rowbuf, A, pad, mmax);
     } else {
       // halo too large for the slots left beside the rows: unstaged path
       unsigned short *far = rowbuf;  // GH_FARMAX * GH_TBLOCK fits in the row buffer
-      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A.rl2, A.near2f,
-                        A.lo2f, A.hi2f, cap, pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
+      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A, pad, mmax);
     }
   } else {
     unsigned short *far = reinterpret_cast<unsigned short *>(dynf);
@@ -606,11 +625,9 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
         zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
       }
       __syncthreads();
-      build_rows<true>(H, t, pos, cellof, xf, yf, zf, far, A.rl2, A.near2f, A.lo2f, A.hi2f, cap,
-                       pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
+      build_rows<true>(H, t, pos, cellof, xf, yf, zf, far, A, pad, mmax);
     } else {
-      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A.rl2, A.near2f,
-                        A.lo2f, A.hi2f, cap, pad, A.ld, A.ncap, A.nn, A.list16, A.xold, mmax);
+      build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A, pad, mmax);
     }
   }
   mmax = wave_imax(mmax);
@@ -1010,15 +1027,23 @@ size_t gh_tile_static_bytes(int precision, int nt) {
 static size_t build_far_bytes() { return (size_t)GH_FARMAX * GH_TBLOCK * 2; }
 static size_t build_slots() { return (GH_BUILD_LDS - build_far_bytes()) / 13; }
 
-// Squared distance below which a pair takes the inner branch of the pair
-// function for every type pair: min rci^2 (SMA-TB), rc^2 (LJ).
-static double gh_near2(const gh_ctx *c) {
-  if (c->pot == GH_POT_LJ) return c->lj_rc * c->lj_rc;
-  double m = 1e300;
+// Bounds of the pair function's inner branch and of its cutoff over all type
+// pairs: rin = min rci, rout = max rce (SMA-TB); rin = rout = rc (LJ).
+static void inner_outer(const gh_ctx *c, double *rin, double *rout) {
+  if (c->pot == GH_POT_LJ) {
+    *rin = *rout = c->lj_rc;
+    return;
+  }
+  *rin = 1e300;
+  *rout = 0.0;
   for (int a = 0; a < c->nt; ++a)
-    for (int b = 0; b < c->nt; ++b) m = std::min(m, c->tb[a * GH_MAXT + b].rci);
-  return m * m;
+    for (int b = 0; b < c->nt; ++b) {
+      *rin = std::min(*rin, c->tb[a * GH_MAXT + b].rci);
+      *rout = std::max(*rout, c->tb[a * GH_MAXT + b].rce);
+    }
 }
+
+static_assert(GH_BUILD_LDS / 13 < 16384, "staged slots must leave two tag bits in a row entry");
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1042,9 +1067,17 @@ static BuildArgs build_args(const gh_ctx *c, int cap, size_t *lds_out) {
   // FP32 prefilter: |r^2_f32 - r^2| stays below 3e-4 A^2 for local
   // coordinates within 40 A of the brick centre; the margin is ten times that
   const double margin = std::max(3e-3, 1e-4 * A.rl2);
-  A.near2f = (float)gh_near2(c);
   A.lo2f = (float)(A.rl2 - margin);
   A.hi2f = (float)(A.rl2 + margin);
+  // row-order classes: bounds at rin -+ m and rout + m, m = skin / 4
+  double rin, rout;
+  inner_outer(c, &rin, &rout);
+  const double m = 0.25 * c->skin;
+  A.cls2[0] = std::max(rin - m, 0.0) * std::max(rin - m, 0.0);
+  A.cls2[1] = (rin + m) * (rin + m);
+  A.cls2[2] = std::max((rout + m) * (rout + m), A.cls2[1]);
+  for (int k = 0; k < 3; ++k) A.clsf[k] = (float)A.cls2[k];
+  A.clsm = (float)margin;
   A.cap = cap;
   A.ld = c->ncap;
   A.ncap = c->ncap;
