@@ -250,9 +250,11 @@ __device__ __forceinline__ double pair_rd(const Halo &H, const TileGeom &t,
   return dist2_rn(vx, vy, vz);
 }
 
-// Rows through a sequential writer; pairs of classes 0-2 go out in candidate
-// order, class 3 waits in an LDS buffer (GH_FARMAX per thread; any excess
-// joins the first run, which only costs order).
+// Rows through a sequential writer.  Staged: class-0 pairs go out at once and
+// the others wait in an LDS buffer, tagged with their class (GH_FARMAX per
+// thread; any excess joins the first run, which only costs order), to follow
+// class by class.  Unstaged (slot numbers may need all 16 bits): classes 0-2
+// go out in candidate order and class 3 waits.
 template <bool STAGE>
 __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
                                            const double4 *__restrict__ pos,
@@ -299,11 +301,20 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
               if (!(rd < A.rl2)) continue;
               c = row_class(rd, A);
             }
-            if (c < 3 || nfar == GH_FARMAX) row.put(s);
-            else far[(nfar++) * GH_TBLOCK + threadIdx.x] = (unsigned short)s;
+            if (c < (STAGE ? 1 : 3) || nfar == GH_FARMAX) row.put(s);
+            else far[(nfar++) * GH_TBLOCK + threadIdx.x] =
+                (unsigned short)(STAGE ? s | (c << 14) : s);
           }
         }
-    for (int k = 0; k < nfar; ++k) row.put(far[k * GH_TBLOCK + threadIdx.x]);
+    if (STAGE) {
+      for (int c = 1; c < 4; ++c)
+        for (int k = 0; k < nfar; ++k) {
+          const int e = far[k * GH_TBLOCK + threadIdx.x];
+          if ((e >> 14) == c) row.put(e & 0x3FFF);
+        }
+    } else {
+      for (int k = 0; k < nfar; ++k) row.put(far[k * GH_TBLOCK + threadIdx.x]);
+    }
     row.finish(pad);
     A.nn[a] = min(row.m, cap);
     A.xold[a] = pa.x;
@@ -336,21 +347,31 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
     const double4 pa = pos[a];
     const float xfi = xf[own], yfi = yf[own], zfi = zf[own];
     int m = 0;
+    // The cells dx = -1, 0, 1 of a stencil row are consecutive halo cells,
+    // so their slots form one run: nine runs, four candidates per step.
     for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int h = ha + dx + H.hd0 * (dy + H.hd1 * dz);
-          const int se = H.hbase[h + 1];
-          for (int s = H.hbase[h]; s < se; ++s) {
-            const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
-            const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
-            bool keep = s != own && r2f <= A.hi2f;
-            if (keep && r2f >= A.lo2f)  // near the cut: the reference's rounding decides
-              keep = pair_rd(H, t, pos, pa, s, h) < A.rl2;
-            rb[(keep && m < cap ? m : cap) * GH_TBLOCK] = (unsigned short)s;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int h = ha + H.hd0 * (dy + H.hd1 * dz);
+        const int sb = H.hbase[h - 1], se = H.hbase[h + 2];
+        for (int s0 = sb; s0 < se; s0 += 4) {
+          float r2[4];
+          int sl[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            sl[u] = min(s0 + u, se - 1);
+            const float ux = xf[sl[u]] - xfi, uy = yf[sl[u]] - yfi, uz = zf[sl[u]] - zfi;
+            r2[u] = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            bool keep = s0 + u < se && sl[u] != own && r2[u] <= A.hi2f;
+            if (keep && r2[u] >= A.lo2f)  // near the cut: the reference's rounding decides
+              keep = pair_rd(H, t, pos, pa, sl[u], scell[sl[u]]) < A.rl2;
+            rb[(keep && m < cap ? m : cap) * GH_TBLOCK] = (unsigned short)sl[u];
             m += keep ? 1 : 0;
           }
         }
+      }
     const int mw = min(m, cap);
     for (int k = 0; k < mw; ++k) {
       const int s = rb[k * GH_TBLOCK];
@@ -376,9 +397,9 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
 
 // ---------------------------------------------------------------------------
 // Binning (sort_in_cells, Neighbour.F90:927-989, as a counting sort by key =
-// brick * cpb + cell within the brick): wrap, count per key, exclusive scan,
-// scatter, order each cell by caller index (deterministic layout), gather
-// the per-atom state into the new order, copy it back.
+// brick * cpb + cell within the brick): wrap, copy the per-atom state aside
+// and count per key; exclusive scan; scatter; order each cell by caller index
+// (deterministic layout) and gather the state back in the new order.
 // ---------------------------------------------------------------------------
 struct CellArgs {
   GhStatus *st;
@@ -410,27 +431,32 @@ __device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, in
 }
 
 // Wraps periodic coordinates into the box (do_pbc_1, Set_Properties.f90:
-// 444-470), computes each atom's key and counts atoms per key.
+// 444-470), copies the atom's state to the _t arrays (the gather reads it
+// there and writes the new order in place), computes the atom's key and
+// counts atoms per key.
 __device__ void cell_bin(const CellArgs &R, int i0, int stride) {
   const TileGeom &t = R.t;
+  const int ncap = R.ncap;
   for (int i = i0; i < R.n; i += stride) {
     double4 p = R.pos[i];
     double x[3] = {p.x, p.y, p.z};
-    bool moved = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       if (t.pbc[d]) {
-        const double w0 = x[d];
         x[d] = x[d] - t.L[d] * floor(x[d] / t.L[d]);
         if (x[d] >= t.L[d]) x[d] -= t.L[d];
-        moved |= x[d] != w0;
       }
-    if (moved) {
-      p.x = x[0];
-      p.y = x[1];
-      p.z = x[2];
-      R.pos[i] = p;
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    R.pos_t[i] = p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R.vel_t[d * ncap + i] = R.vel[d * ncap + i];
+      R.acc_t[d * ncap + i] = R.acc[d * ncap + i];
     }
+    R.type_t[i] = R.type[i];
+    R.orig_t[i] = R.orig[i];
     int out = 0;
     const int cx = tile_coord(x[0], t, 0, &out);
     const int cy = tile_coord(x[1], t, 1, &out);
@@ -444,13 +470,12 @@ __device__ void cell_bin(const CellArgs &R, int i0, int stride) {
 
 #define GH_SCANBLOCK 1024  // workgroup of the cell-count scan
 
-// Workgroup (GH_SCANBLOCK threads) exclusive scan of four values per thread
-// in thread order; returns this thread's offset and the workgroup total.
-__device__ __forceinline__ int block_scan4(const int (&v)[4], int &total) {
+// Workgroup (GH_SCANBLOCK threads) exclusive scan of one value per thread in
+// thread order; returns this thread's offset and the workgroup total.
+__device__ __forceinline__ int block_scan(int v, int &total) {
   __shared__ int wsum[GH_SCANBLOCK / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tot = v[0] + v[1] + v[2] + v[3];
-  int inc = tot;
+  int inc = v;
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(inc, o, 64);
     if (lane >= o) inc += y;
@@ -465,33 +490,47 @@ __device__ __forceinline__ int block_scan4(const int (&v)[4], int &total) {
     total += x;
   }
   __syncthreads();
-  return woff + inc - tot;
+  return woff + inc - v;
 }
 
-// Exclusive scan of the cell counts by one workgroup, in tiles of four keys
-// per thread (one coalesced 16-B load each); a running carry links the
-// tiles.  start = cursor = the prefix; start[nkey] = the total.
+// Exclusive scan of the cell counts by one workgroup.  Each pass gives every
+// thread GH_SCANRUN consecutive keys, all loaded at once (16-B loads), so a
+// pass costs one memory round trip for the loads and one for the stores; a
+// carry links the passes.  start = cursor = the prefix; start[nkey] = total.
+#define GH_SCANRUN 32
 __device__ void cell_scan(const CellArgs &R) {
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < R.nkey; base += 4 * GH_SCANBLOCK) {
-    const int k0 = base + 4 * threadIdx.x;
-    int v[4];
+  int carry = 0;
+  for (int base = 0; base < R.nkey; base += GH_SCANRUN * GH_SCANBLOCK) {
+    const int k0 = base + GH_SCANRUN * threadIdx.x;
+    int v[GH_SCANRUN];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = k0 + u < R.nkey ? R.count[k0 + u] : 0;
+    for (int u = 0; u < GH_SCANRUN; u += 4) {
+      const int k = k0 + u;
+      if (k + 3 < R.nkey) {
+        const int4 q = *reinterpret_cast<const int4 *>(R.count + k);
+        v[u] = q.x;
+        v[u + 1] = q.y;
+        v[u + 2] = q.z;
+        v[u + 3] = q.w;
+      } else {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) v[u + w] = k + w < R.nkey ? R.count[k + w] : 0;
+      }
+    }
+    int tot = 0;
+#pragma unroll
+    for (int u = 0; u < GH_SCANRUN; ++u) tot += v[u];
     int total;
-    int run = carry + block_scan4(v, total);
+    int run = carry + block_scan(tot, total);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < GH_SCANRUN; ++u) {
       if (k0 + u < R.nkey) {
         R.start[k0 + u] = run;
         R.cursor[k0 + u] = run;
-        run += v[u];
       }
-    __syncthreads();
-    if (threadIdx.x == 0) carry += total;
-    __syncthreads();
+      run += v[u];
+    }
+    carry += total;
   }
   if (threadIdx.x == 0) R.start[R.nkey] = carry;
 }
@@ -503,16 +542,22 @@ __device__ void cell_scatter(const CellArgs &R, int i0, int stride) {
   }
 }
 
-// Orders each cell by caller index (the layout is then independent of the
-// atomic scatter order) and zeroes its count for the next binning.
-__device__ void cell_sort(const CellArgs &R, int c0, int stride) {
-  for (int c = c0; c < R.nkey; c += stride) {
+// One workgroup per GH_TBLOCK keys: each thread orders one cell by caller
+// index (the layout is then independent of the atomic scatter order) and
+// zeroes its count for the next binning; then the workgroup gathers the
+// state of its cells' atoms from the _t copies into the new order.  Cells
+// are contiguous in the new order, so no other workgroup touches the slots
+// written here, and the copies are read-only: no grid-wide barrier needed.
+__device__ void cell_sort_gather(const CellArgs &R) {
+  const int k0 = blockIdx.x * GH_TBLOCK, k1 = min(k0 + GH_TBLOCK, R.nkey);
+  const int c = k0 + threadIdx.x;
+  if (c < k1) {
     const int b = R.start[c], e = R.start[c + 1];
     for (int k = b + 1; k < e; ++k) {
       const int v = R.perm[k];
-      const int key = R.orig[v];
+      const int key = R.orig_t[v];
       int m = k - 1;
-      while (m >= b && R.orig[R.perm[m]] > key) {
+      while (m >= b && R.orig_t[R.perm[m]] > key) {
         R.perm[m + 1] = R.perm[m];
         --m;
       }
@@ -520,35 +565,19 @@ __device__ void cell_sort(const CellArgs &R, int c0, int stride) {
     }
     R.count[c] = 0;
   }
-}
-
-__device__ void cell_gather(const CellArgs &R, int s0, int stride) {
+  __syncthreads();
   const int ncap = R.ncap;
-  for (int s = s0; s < R.n; s += stride) {
+  for (int s = R.start[k0] + threadIdx.x, se = R.start[k1]; s < se; s += GH_TBLOCK) {
     const int o = R.perm[s];
-    R.pos_t[s] = R.pos[o];
+    R.pos[s] = R.pos_t[o];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      R.vel_t[d * ncap + s] = R.vel[d * ncap + o];
-      R.acc_t[d * ncap + s] = R.acc[d * ncap + o];
+      R.vel[d * ncap + s] = R.vel_t[d * ncap + o];
+      R.acc[d * ncap + s] = R.acc_t[d * ncap + o];
     }
-    R.type_t[s] = R.type[o];
-    R.orig_t[s] = R.orig[o];
+    R.type[s] = R.type_t[o];
+    R.orig[s] = R.orig_t[o];
     R.cellof[s] = R.cellid[o];
-  }
-}
-
-__device__ void cell_copyback(const CellArgs &R, int s0, int stride) {
-  const int ncap = R.ncap;
-  for (int s = s0; s < R.n; s += stride) {
-    R.pos[s] = R.pos_t[s];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      R.vel[d * ncap + s] = R.vel_t[d * ncap + s];
-      R.acc[d * ncap + s] = R.acc_t[d * ncap + s];
-    }
-    R.type[s] = R.type_t[s];
-    R.orig[s] = R.orig_t[s];
   }
 }
 
@@ -566,17 +595,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_cell_scatter(CellArgs R) {
   if (!R.st->rebuild) return;
   cell_scatter(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_sort(CellArgs R) {
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_sort_gather(CellArgs R) {
   if (!R.st->rebuild) return;
-  cell_sort(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_gather(CellArgs R) {
-  if (!R.st->rebuild) return;
-  cell_gather(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_copyback(CellArgs R) {
-  if (!R.st->rebuild) return;
-  cell_copyback(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  cell_sort_gather(R);
 }
 
 // Rows of the atoms of brick b (all threads of the workgroup call it; LDS
@@ -1119,13 +1140,11 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   hipStream_t s = c->stream;
   // grid-stride launches of at most 2048 workgroups: cheap when no rebuild is due
   const int ga = std::min((R.n + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
-  const int gk = std::min((R.nkey + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
+  const int gk = (R.nkey + GH_TBLOCK - 1) / GH_TBLOCK;  // one workgroup per GH_TBLOCK keys
   k_cell_bin<<<ga, GH_TBLOCK, 0, s>>>(R);
   k_cell_scan<<<1, GH_SCANBLOCK, 0, s>>>(R);
   k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(R);
-  k_cell_sort<<<gk, GH_TBLOCK, 0, s>>>(R);
-  k_cell_gather<<<ga, GH_TBLOCK, 0, s>>>(R);
-  k_cell_copyback<<<ga, GH_TBLOCK, 0, s>>>(R);
+  k_cell_sort_gather<<<gk, GH_TBLOCK, 0, s>>>(R);
   k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cellof,
                                                             c->cell_start, A);
   GH_HIP(c, hipGetLastError());
