@@ -381,11 +381,6 @@ static int ensure_ready(gh_ctx *c) {
     GH_TRY(dalloc(c, &c->list, (size_t)cap * c->ncap));
   }
   c->cap = cap;
-  if (c->list_mode == 1 && c->tg.nbrick > c->ndispblk) {  // fused force passes: one per brick
-    c->ndispblk = c->tg.nbrick;
-    GH_TRY(dalloc(c, &c->dispblk, (size_t)c->ndispblk));
-    GH_HIP(c, hipMemset(c->dispblk, 0, sizeof(double) * c->ndispblk));
-  }
   h.overflow = 0;
   h.tile_overflow = 0;
   h.nupd = nupd0;
@@ -509,7 +504,7 @@ void gh_destroy(gh_ctx *c) {
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
-                 c->red, c->dispblk, c->list16};
+                 c->red, c->list16};
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->gauss_h) (void)hipHostFree(c->gauss_h);
@@ -655,9 +650,6 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
     GH_TRY(dalloc(c, &c->cellid, ncap));
     GH_TRY(dalloc(c, &c->perm, ncap));
     GH_TRY(dalloc(c, &c->nn, ncap));
-    c->ndispblk = (ncap + 255) / 256;
-    GH_TRY(dalloc(c, &c->dispblk, (size_t)c->ndispblk));
-    GH_HIP(c, hipMemset(c->dispblk, 0, sizeof(double) * c->ndispblk));
     c->nred = std::min(1024, std::max(1, (ncap + 255) / 256));
     GH_TRY(dalloc(c, &c->red, (size_t)c->nred * 5 + 16));
     if (c->gauss) {

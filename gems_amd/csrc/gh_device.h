@@ -9,6 +9,25 @@
 
 namespace ghd {
 
+// Rebuild test on the displacement maximum the integrators fold into
+// GhStatus::maxdisp2_bits (one atomicMax per workgroup).
+__device__ __forceinline__ bool rebuild_due(const GhStatus *st, double skin) {
+  const double d2 = __longlong_as_double((long long)st->maxdisp2_bits);
+  return st->force_rebuild || 2.0 * sqrt(d2) > skin;
+}
+
+// Books the decision in st->rebuild once every kernel that tests
+// rebuild_due has run: counts the rebuild, clears the host request and the
+// row maximum, and restarts the displacement maximum for the next step.
+__device__ __forceinline__ void rebuild_book(GhStatus *st) {
+  if (st->rebuild) {
+    st->nupd += 1;
+    st->force_rebuild = 0;
+    st->maxnn = 0;
+  }
+  st->maxdisp2_bits = 0ull;
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
 // Remapping the workgroup id makes XCD k run a contiguous range of
 // workgroups, i.e. one slab of cell-sorted atoms whose neighbours (and their

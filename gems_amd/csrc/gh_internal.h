@@ -181,8 +181,6 @@ struct gh_ctx {
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
   TbF *tbf = nullptr;                         // [MAXT*MAXT]
   GhStatus *status = nullptr;
-  double *dispblk = nullptr;                  // per-workgroup max displacement^2
-  int ndispblk = 0;
   double *red = nullptr;                      // reduction scratch
   int nred = 0;
   double *red_h = nullptr;                    // pinned result
@@ -219,9 +217,12 @@ struct FuseVV {
   double *vel;
   const double *xold;
   double4 *pos_next;
-  double *dispblk;
+  unsigned long long *maxd2;  // GhStatus::maxdisp2_bits
   int ncap;
 };
+
+// Device address of the displacement maximum the integrators fold into.
+inline unsigned long long *gh_maxd2(gh_ctx *c) { return &c->status->maxdisp2_bits; }
 
 // launchers (gh_kernels.hip)
 int gh_launch_rebuild(gh_ctx *c, bool force);

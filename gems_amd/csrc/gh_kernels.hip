@@ -49,50 +49,15 @@ __global__ void k_fill_int(int *__restrict__ p, int v, int n) {
 // Rebuild decision (test_update, Neighbour.F90:1305-1346).  The reference
 // rebuilds when the two largest displacements since the last build sum past
 // the skin.  Here the displacement is the true (minimum-image) one, and the
-// test uses 2*max, which bounds d1+d2 from above.
+// test uses 2*max, which bounds d1+d2 from above.  Integrators fold each
+// workgroup's max |x - x_old|^2 into GhStatus::maxdisp2_bits with one
+// atomicMax (non-negative doubles order as their bit patterns); the first
+// kernel of the next rebuild sequence reads it (rebuild_due, rebuild_book).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_decide(GhStatus *st, double *__restrict__ dispblk,
-                                                 int nblk, double skin) {
-  __shared__ double part[16];
-  // The status words and eight maxima per thread are fetched together: the
-  // kernel is a chain of memory round trips, so it pays one instead of many.
-  int frb = 0;
-  unsigned long long md = 0ull;
-  if (threadIdx.x == 0) {
-    frb = st->force_rebuild;
-    md = st->maxdisp2_bits;
-  }
-  double m = 0.0;  // maxima are squared displacements, >= 0
-  constexpr int U = 8;
-  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * (int)blockDim.x) {
-    double v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * (int)blockDim.x;
-      v[u] = b < nblk ? dispblk[b] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * (int)blockDim.x;
-      if (b < nblk) dispblk[b] = 0.0;  // consumed: the next integrator step writes fresh maxima
-      m = fmax(m, v[u]);
-    }
-  }
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, part[w]);
-    const double d2 = fmax(m, __longlong_as_double((long long)md));
-    const int rb = frb || (2.0 * sqrt(d2) > skin);
-    st->rebuild = rb;
-    if (rb) {
-      st->nupd += 1;
-      st->force_rebuild = 0;
-      st->maxnn = 0;
-    }
-    st->maxdisp2_bits = 0ull;
-  }
+// Decision of the O(N^2) list mode (one thread).
+__global__ void k_decide(GhStatus *st, double skin) {
+  st->rebuild = rebuild_due(st, skin);
+  rebuild_book(st);
 }
 
 struct GridArgs {
@@ -499,18 +464,16 @@ struct WrapArgs {
 // pair's periodic image, which stays valid while atoms move continuously.
 // They are wrapped into the box at every rebuild (k_wrap_rebuild) and when
 // copied out, which reproduces do_pbc_1 (Set_Properties.f90:444-470).
-// Per-workgroup maximum of |x - x_old|^2, one value per workgroup; k_decide
-// reduces them in its own launch.  (A last-workgroup-done reduction inside
-// the integrator needs an agent-scope release per workgroup, which writes
-// back the XCD's L2 each time: measured 3x slower than the extra launch.)
-__device__ __forceinline__ void disp_reduce(double d2, double *__restrict__ dispblk) {
+// Workgroup maximum of |x - x_old|^2 folded into the device-wide maximum with
+// one atomicMax (no fence: the next kernel reads it).
+__device__ __forceinline__ void disp_reduce(double d2, unsigned long long *__restrict__ maxd2) {
   __shared__ double part[GH_BLOCK / 64];
   d2 = wave_max(d2);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < GH_BLOCK / 64; ++w) d2 = fmax(d2, part[w]);
-    dispblk[blockIdx.x] = d2;
+    atomicMax(maxd2, (unsigned long long)__double_as_longlong(d2));
   }
 }
 
@@ -519,7 +482,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     int n, int ncap, double dt, MassArgs ma,
     const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
     double *__restrict__ acc, const double *__restrict__ frc, const double *__restrict__ xold,
-    double *__restrict__ dispblk) {
+    unsigned long long *__restrict__ maxd2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -542,7 +505,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, dispblk);
+  disp_reduce(d2, maxd2);
 }
 
 // velocity_verlet_b of one step fused with velocity_verlet_a of the next
@@ -553,7 +516,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
     int n, int ncap, double dt, MassArgs ma,
     const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
     const double *__restrict__ frc, const double *__restrict__ xold,
-    double *__restrict__ dispblk) {
+    unsigned long long *__restrict__ maxd2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -575,7 +538,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, dispblk);
+  disp_reduce(d2, maxd2);
 }
 
 // velocity_verlet_b (Integration.F90:1171-1186).
@@ -620,7 +583,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     const int *__restrict__ type,
     const int *__restrict__ orig, const double *__restrict__ gauss, double4 *__restrict__ pos,
     const double *__restrict__ vel, const double *__restrict__ acc,
-    const double *__restrict__ xold, double *__restrict__ dispblk) {
+    const double *__restrict__ xold, unsigned long long *__restrict__ maxd2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -641,7 +604,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, dispblk);
+  disp_reduce(d2, maxd2);
 }
 
 // ermak_b (Integration.F90:1013-1031); vel_v is re-derived from the same
@@ -829,22 +792,17 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
     GH_HIP(c, hipMemcpyAsync(c->status, &h, sizeof(GhStatus), hipMemcpyHostToDevice, s));
     GH_HIP(c, hipStreamSynchronize(s));
   }
-  // dispblk holds the last integrator step's per-workgroup maxima (zero after
-  // a rebuild or at setup)
-  k_decide<<<1, 1024, 0, s>>>(c->status, c->dispblk, c->ndispblk, c->skin);
   const int n = c->n, nb = nblocks(n);
+  if (c->list_mode == 1) return gh_launch_rebuild_cells(c);  // decides in its first kernels
+  k_decide<<<1, 1, 0, s>>>(c->status, c->skin);
   if (c->list_mode == 0 && c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
     k_wrap_rebuild<<<nb, GH_BLOCK, 0, s>>>(c->status, n, make_wrap(c), c->pos);
   const GridArgs g = make_grid(c);
   const double rl = c->rcut + c->skin;
   const double rl2 = rl * rl;
   const int ld = c->ncap;
-  if (c->list_mode == 1) {
-    return gh_launch_rebuild_cells(c);
-  } else {
-    k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
-                                       (unsigned *)c->list, c->xold, c->ncap);
-  }
+  k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
+                                     (unsigned *)c->list, c->xold, c->ncap);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -926,7 +884,7 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 int gh_launch_vv_a(gh_ctx *c) {
   k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                     make_mass(c), c->type, c->pos, c->vel,
-                                                    c->acc, c->frc, c->xold, c->dispblk);
+                                                    c->acc, c->frc, c->xold, gh_maxd2(c));
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -934,7 +892,7 @@ int gh_launch_vv_a(gh_ctx *c) {
 int gh_launch_vv_ba(gh_ctx *c) {
   k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                      make_mass(c), c->type, c->pos, c->vel,
-                                                     c->frc, c->xold, c->dispblk);
+                                                     c->frc, c->xold, gh_maxd2(c));
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -949,7 +907,7 @@ int gh_launch_vv_b(gh_ctx *c) {
 int gh_launch_ermak_a(gh_ctx *c) {
   k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
       c->n, c->ncap, make_ermak(c), make_mass(c), c->type, c->orig,
-      c->gauss, c->pos, c->vel, c->acc, c->xold, c->dispblk);
+      c->gauss, c->pos, c->vel, c->acc, c->xold, gh_maxd2(c));
   GH_HIP(c, hipGetLastError());
   return 0;
 }

@@ -173,14 +173,15 @@ __device__ __forceinline__ double fused_vv(const FuseVV &f, int a, const double4
 }
 
 // Workgroup maximum of d2, stored at dst[b] (all threads call it).
-__device__ __forceinline__ void block_max_store(double d2, double *dst, int b) {
+// Workgroup maximum of |x - x_old|^2 folded into the device-wide maximum.
+__device__ __forceinline__ void block_max_atomic(double d2, unsigned long long *maxd2) {
   __shared__ double part[GH_TBLOCK / 64];
   d2 = wave_max(d2);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < GH_TBLOCK / 64; ++w) d2 = fmax(d2, part[w]);
-    dst[b] = d2;
+    atomicMax(maxd2, (unsigned long long)__double_as_longlong(d2));
   }
 }
 
@@ -404,6 +405,7 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
 struct CellArgs {
   GhStatus *st;
   TileGeom t;
+  double skin;
   int n, ncap, nkey;
   double4 *pos, *pos_t;
   double *vel, *vel_t, *acc, *acc_t;
@@ -581,14 +583,21 @@ __device__ void cell_sort_gather(const CellArgs &R) {
   }
 }
 
-// The binning phases, one launch each (every launch returns at once when no
-// rebuild is due).
+// The binning phases, one launch each; every launch returns at once when no
+// rebuild is due.  k_cell_bin decides (each workgroup evaluates the same
+// test on the unchanged status; workgroup 0 records it), k_cell_scan books
+// the decision: the later kernels read st->rebuild.
 __global__ __launch_bounds__(GH_TBLOCK) void k_cell_bin(CellArgs R) {
-  if (!R.st->rebuild) return;
+  const bool due = rebuild_due(R.st, R.skin);
+  if (blockIdx.x == 0 && threadIdx.x == 0) R.st->rebuild = due ? 1 : 0;
+  if (!due) return;
   cell_bin(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 __global__ __launch_bounds__(GH_SCANBLOCK) void k_cell_scan(CellArgs R) {
-  if (!R.st->rebuild) return;
+  const int due = R.st->rebuild;
+  __syncthreads();  // every thread has read the flag before thread 0 books it
+  if (threadIdx.x == 0) rebuild_book(R.st);
+  if (!due) return;
   cell_scan(R);
 }
 __global__ __launch_bounds__(GH_TBLOCK) void k_cell_scatter(CellArgs R) {
@@ -825,7 +834,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     epa[a] = (ei - sqrt(rho[a])) * ev_ui;
     if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
   }
-  if (fv.on) block_max_store(dmax2, fv.dispblk, b);
+  if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
 
 // ---------------------------------------------------------------------------
@@ -951,7 +960,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
     epa[a] = (ei - sqrt(rho[a])) * ev_ui;
     if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
   }
-  if (fv.on) block_max_store(dmax2, fv.dispblk, b);
+  if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
 
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
@@ -1019,7 +1028,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     epa[a] = ei * 0.5 * ev_ui;
     if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pi, type[a], Fx, Fy, Fz));
   }
-  if (fv.on) block_max_store(dmax2, fv.dispblk, b);
+  if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
 
 }  // namespace
@@ -1116,6 +1125,7 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   CellArgs R;
   R.st = c->status;
   R.t = c->tg;
+  R.skin = c->skin;
   R.n = c->n;
   R.ncap = c->ncap;
   R.nkey = c->tg.nkey;
@@ -1186,7 +1196,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   fv.vel = c->vel;
   fv.xold = c->xold;
   fv.pos_next = c->pos_next;
-  fv.dispblk = c->dispblk;
+  fv.maxd2 = gh_maxd2(c);
   fv.ncap = c->ncap;
   const int grid = xgrid(c->tg.nbrick);
   const int ld = c->ncap;
