@@ -121,22 +121,51 @@ def cpu_baseline(x, L, skin, budget_s=20.0):
 
 
 def nve_drift(hip, precision, x, L, skin, nsteps, tile=None, each=100):
-    """Total-energy drift of velocity-Verlet NVE over nsteps (eV/atom): the
-    energy columns of write_energy every `each` steps."""
+    """Total-energy drift of velocity-Verlet NVE over nsteps (eV/atom), on the
+    energy columns of write_energy every `each` steps, measured as the
+    reference's own trace is: from the first row (step `each`) to the last."""
     c = make_ctx(hip, precision, x, L, skin)
     if tile:
         c.set_tile(tile)
-    ep0, ek0 = c.energies()
-    e0 = ep0 + ek0
     t0 = time.perf_counter()
     en = c.run(nsteps, each=each)
     dt = time.perf_counter() - t0
     c.close()
-    n = len(x)
-    dev = (en[:, 2] - e0) / n
-    return {"steps": nsteps, "drift_ev_per_atom": float(dev[-1]),
-            "max_abs_dev_ev_per_atom": float(np.max(np.abs(dev))),
-            "e0_ev": float(e0), "seconds": dt}
+    r = drift_stats(en, len(x))
+    r.update(steps=nsteps, seconds=dt)
+    return r
+
+
+def drift_stats(en, n):
+    tot = en[:, 2]
+    return {"drift_ev_per_atom": float((tot[-1] - tot[0]) / n),
+            "max_abs_dev_ev_per_atom": float(np.max(np.abs(tot - tot[0])) / n)}
+
+
+def drift_vs_reference(hip):
+    """The 32,000-atom NVE trace the reference itself wrote (10^4 steps, energy
+    every 100; tests/golden/au32k_nve_drift.npz) next to this path's trace of
+    the same run, FP64 and mixed."""
+    path = os.path.join(REPO, "tests", "golden", "au32k_nve_drift.npz")
+    if not os.path.exists(path):
+        return None
+    ref = np.load(path)
+    nx = int(ref["nx"])
+    x = fcc(nx, A_LAT) + np.random.default_rng(int(ref["jitter_seed"])).normal(
+        0.0, 0.02, (4 * nx ** 3, 3))
+    L = nx * A_LAT
+    x = np.mod(x, L)
+    out = {"workload": "Au 32,000 atoms (20^3 fcc cells, 0.02 A jitter), 300 K lcg seed 4242, "
+                       "VV dt 2 fs, skin 1 A, 10^4 steps, energy every 100 steps",
+           "reference": drift_stats(ref["energy"], len(x))}
+    for name, prec in (("fp64", hip.GH_FP64), ("mixed", hip.GH_MIXED)):
+        c = make_ctx(hip, prec, x, L, 1.0, lcg_seed=int(ref["seed"]))
+        en = c.run(int(ref["nsteps"]), each=int(ref["each"]))
+        c.close()
+        out[name] = drift_stats(en, len(x))
+        out[name]["max_abs_diff_vs_reference_ev_per_atom"] = float(
+            np.max(np.abs(en[:, 2] - ref["energy"][:, 2])) / len(x))
+    return out
 
 
 def batch_re(hip, rounds, swap_every=500):
@@ -335,6 +364,7 @@ def main():
             "mixed": nve_drift(hip, hip.GH_MIXED, x, L, args.skin, args.drift_steps, tile_mx[0]),
             "fp64": nve_drift(hip, hip.GH_FP64, x, L, args.skin, args.drift_steps, tile_64[0]),
             "workload": "the same 1,000,188-atom Au NVE run, energy every 100 steps",
+            "vs_reference_32k": drift_vs_reference(hip),
         }
     if not args.no_batch:
         out["batched_re"] = batch_re(hip, args.re_rounds)

@@ -307,7 +307,6 @@ static int alloc_cells(gh_ctx *c) {
   if (c->ncell > c->cell_alloc) {
     GH_TRY(dalloc(c, &c->cell_count, (size_t)c->ncell));
     GH_TRY(dalloc(c, &c->cell_start, (size_t)c->ncell + 1));
-    GH_TRY(dalloc(c, &c->cell_cursor, (size_t)c->ncell));
     c->cell_alloc = c->ncell;
   }
   return 0;
@@ -503,7 +502,7 @@ void gh_destroy(gh_ctx *c) {
   void *dev[] = {c->pos, c->pos_t, c->pos_next, c->vel, c->vel_t, c->acc, c->acc_t, c->frc, c->epa,
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
-                 c->cell_start, c->cell_cursor, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
+                 c->cell_start, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
                  c->red, c->list16};
   for (void *p : dev)
     if (p) (void)hipFree(p);

@@ -173,7 +173,7 @@ struct gh_ctx {
   double *gauss_h = nullptr;                  // pinned host staging
   int *cellof = nullptr;                      // [ncap] cell of sorted atom
   int *cellid = nullptr;                      // [ncap] cell of unsorted atom (rebuild scratch)
-  int *cell_count = nullptr, *cell_start = nullptr, *cell_cursor = nullptr;
+  int *cell_count = nullptr, *cell_start = nullptr;
   int *perm = nullptr;                        // [ncap]
   int *nn = nullptr;                          // [ncap]
   int *list = nullptr;                        // [cap][ncap] ELL (O(N^2) mode)

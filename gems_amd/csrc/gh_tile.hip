@@ -410,7 +410,7 @@ struct CellArgs {
   double4 *pos, *pos_t;
   double *vel, *vel_t, *acc, *acc_t;
   int *type, *type_t, *orig, *orig_t;
-  int *cellid, *cellof, *count, *start, *cursor, *perm;
+  int *cellid, *cellof, *count, *start, *perm;
 };
 
 __device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, int *outside) {
@@ -496,9 +496,9 @@ __device__ __forceinline__ int block_scan(int v, int &total) {
 }
 
 // Exclusive scan of the cell counts by one workgroup.  Each pass gives every
-// thread GH_SCANRUN consecutive keys, all loaded at once (16-B loads), so a
+// thread GH_SCANRUN consecutive keys, loaded and stored 16 B at a time, so a
 // pass costs one memory round trip for the loads and one for the stores; a
-// carry links the passes.  start = cursor = the prefix; start[nkey] = total.
+// carry links the passes.  start = the prefix; start[nkey] = the total.
 #define GH_SCANRUN 32
 __device__ void cell_scan(const CellArgs &R) {
   int carry = 0;
@@ -525,61 +525,61 @@ __device__ void cell_scan(const CellArgs &R) {
     int total;
     int run = carry + block_scan(tot, total);
 #pragma unroll
-    for (int u = 0; u < GH_SCANRUN; ++u) {
-      if (k0 + u < R.nkey) {
-        R.start[k0 + u] = run;
-        R.cursor[k0 + u] = run;
+    for (int u = 0; u < GH_SCANRUN; u += 4) {
+      const int k = k0 + u;
+      int4 q;
+      q.x = run;
+      q.y = q.x + v[u];
+      q.z = q.y + v[u + 1];
+      q.w = q.z + v[u + 2];
+      run = q.w + v[u + 3];
+      if (k + 3 < R.nkey) {
+        *reinterpret_cast<int4 *>(R.start + k) = q;
+      } else {
+        if (k < R.nkey) R.start[k] = q.x;
+        if (k + 1 < R.nkey) R.start[k + 1] = q.y;
+        if (k + 2 < R.nkey) R.start[k + 2] = q.z;
       }
-      run += v[u];
     }
     carry += total;
   }
   if (threadIdx.x == 0) R.start[R.nkey] = carry;
 }
 
+// Places each atom in its cell's range, counting the cell's count down to
+// zero (ready for the next binning); the order within a cell is the atomics'
+// and is fixed by the rank gather.
 __device__ void cell_scatter(const CellArgs &R, int i0, int stride) {
   for (int i = i0; i < R.n; i += stride) {
-    const int slot = atomicAdd(&R.cursor[R.cellid[i]], 1);
-    R.perm[slot] = i;
+    const int key = R.cellid[i];
+    R.perm[R.start[key] + atomicSub(&R.count[key], 1) - 1] = i;
   }
 }
 
-// One workgroup per GH_TBLOCK keys: each thread orders one cell by caller
-// index (the layout is then independent of the atomic scatter order) and
-// zeroes its count for the next binning; then the workgroup gathers the
-// state of its cells' atoms from the _t copies into the new order.  Cells
-// are contiguous in the new order, so no other workgroup touches the slots
-// written here, and the copies are read-only: no grid-wide barrier needed.
-__device__ void cell_sort_gather(const CellArgs &R) {
-  const int k0 = blockIdx.x * GH_TBLOCK, k1 = min(k0 + GH_TBLOCK, R.nkey);
-  const int c = k0 + threadIdx.x;
-  if (c < k1) {
-    const int b = R.start[c], e = R.start[c + 1];
-    for (int k = b + 1; k < e; ++k) {
-      const int v = R.perm[k];
-      const int key = R.orig_t[v];
-      int m = k - 1;
-      while (m >= b && R.orig_t[R.perm[m]] > key) {
-        R.perm[m + 1] = R.perm[m];
-        --m;
-      }
-      R.perm[m + 1] = v;
-    }
-    R.count[c] = 0;
-  }
-  __syncthreads();
+// One thread per slot: the atom placed there by the scatter takes the rank
+// of its caller index among its cell's atoms (the layout is then independent
+// of the atomics' order), and its state moves from the _t copies to that
+// slot.  Every thread reads only the scatter's output and the copies, so
+// no ordering between threads is needed.
+__device__ void cell_rank_gather(const CellArgs &R, int s0, int stride) {
   const int ncap = R.ncap;
-  for (int s = R.start[k0] + threadIdx.x, se = R.start[k1]; s < se; s += GH_TBLOCK) {
-    const int o = R.perm[s];
-    R.pos[s] = R.pos_t[o];
+  for (int s = s0; s < R.n; s += stride) {
+    const int v = R.perm[s];
+    const int c = R.cellid[v];
+    const int b = R.start[c], e = R.start[c + 1];
+    const int key = R.orig_t[v];
+    int rank = 0;
+    for (int k = b; k < e; ++k) rank += R.orig_t[R.perm[k]] < key ? 1 : 0;
+    const int d = b + rank;
+    R.pos[d] = R.pos_t[v];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      R.vel[d * ncap + s] = R.vel_t[d * ncap + o];
-      R.acc[d * ncap + s] = R.acc_t[d * ncap + o];
+    for (int dd = 0; dd < 3; ++dd) {
+      R.vel[dd * ncap + d] = R.vel_t[dd * ncap + v];
+      R.acc[dd * ncap + d] = R.acc_t[dd * ncap + v];
     }
-    R.type[s] = R.type_t[o];
-    R.orig[s] = R.orig_t[o];
-    R.cellof[s] = R.cellid[o];
+    R.type[d] = R.type_t[v];
+    R.orig[d] = key;
+    R.cellof[d] = c;
   }
 }
 
@@ -604,9 +604,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_cell_scatter(CellArgs R) {
   if (!R.st->rebuild) return;
   cell_scatter(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_sort_gather(CellArgs R) {
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_rank_gather(CellArgs R) {
   if (!R.st->rebuild) return;
-  cell_sort_gather(R);
+  cell_rank_gather(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Rows of the atoms of brick b (all threads of the workgroup call it; LDS
@@ -1143,18 +1143,16 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   R.cellof = c->cellof;
   R.count = c->cell_count;
   R.start = c->cell_start;
-  R.cursor = c->cell_cursor;
   R.perm = c->perm;
   size_t lds = 0;
   const BuildArgs A = build_args(c, c->cap, &lds);
   hipStream_t s = c->stream;
   // grid-stride launches of at most 2048 workgroups: cheap when no rebuild is due
   const int ga = std::min((R.n + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
-  const int gk = (R.nkey + GH_TBLOCK - 1) / GH_TBLOCK;  // one workgroup per GH_TBLOCK keys
   k_cell_bin<<<ga, GH_TBLOCK, 0, s>>>(R);
   k_cell_scan<<<1, GH_SCANBLOCK, 0, s>>>(R);
   k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(R);
-  k_cell_sort_gather<<<gk, GH_TBLOCK, 0, s>>>(R);
+  k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(R);
   k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cellof,
                                                             c->cell_start, A);
   GH_HIP(c, hipGetLastError());
