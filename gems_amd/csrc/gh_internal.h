@@ -93,7 +93,9 @@ struct TileGeom {
   double one_cell[3];
 };
 #define GH_MAXHALO 216  // (4 + 2)^3 halo cells at most (brick edge <= 4)
-#define GH_BUILD_LDS 61440  // dynamic LDS the list build may stage positions in
+// Dynamic LDS of the list build: two workgroups (with their static Halo) fit
+// one CU's 160 KB.
+#define GH_BUILD_LDS 77824
 
 struct GhStatus {                     // device-resident counters
   unsigned long long maxdisp2_bits;   // max |dx|^2 since the last build

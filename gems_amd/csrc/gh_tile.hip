@@ -270,11 +270,11 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
     const int ha = (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
     const int own = a - H.delta[ha];
     const double4 pa = pos[a];
-    float xfi = 0.f, yfi = 0.f, zfi = 0.f;
+    float xi = 0.f, yi = 0.f, zi = 0.f;
     if (STAGE) {
-      xfi = xf[own];
-      yfi = yf[own];
-      zfi = zf[own];
+      xi = xf[own];
+      yi = yf[own];
+      zi = zf[own];
     }
     RowWriter row{reinterpret_cast<unsigned long long *>(A.list16), A.ld, a, cap, 0, 0ull};
     int nfar = 0;
@@ -287,7 +287,7 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
             if (s == own) continue;
             int c;
             if (STAGE) {
-              const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
+              const float ux = xf[s] - xi, uy = yf[s] - yi, uz = zf[s] - zi;
               const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
               if (r2f > A.hi2f) continue;
               double rd = -1.0;
@@ -327,10 +327,9 @@ __device__ __forceinline__ void build_rows(const Halo &H, const TileGeom &t,
 
 // Staged build with the rows assembled in LDS (rb: entry k of this thread's
 // row at rb[k * GH_TBLOCK], k = cap is a dump slot): every candidate is
-// written through a computed index, so the candidate loop has no per-pair
-// branch but the rare exact test.  A second sweep over the row tags each
-// entry with its class (bits 14-15; staged slots number below 16384), and
-// four more write the row out class by class as whole quads.
+// written through a computed index, so the candidate loop has no branch.
+// A second sweep makes the exact tests, tags each entry with its class and
+// counts the classes; a third writes every entry to its place in the list.
 __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
                                                const double4 *__restrict__ pos,
                                                const int *__restrict__ cellof, const float *xf,
@@ -346,10 +345,12 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
     const int ha = (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
     const int own = a - H.delta[ha];
     const double4 pa = pos[a];
-    const float xfi = xf[own], yfi = yf[own], zfi = zf[own];
+    const float xi = xf[own], yi = yf[own], zi = zf[own];
     int m = 0;
     // The cells dx = -1, 0, 1 of a stencil row are consecutive halo cells,
-    // so their slots form one run: nine runs, four candidates per step.
+    // so their slots form one run: nine runs, four candidates per step.  The
+    // candidate loop has no branch: a candidate inside the FP32 band keeps a
+    // flag (bit 15) for the exact test of the next sweep.
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy) {
         const int h = ha + H.hd0 * (dy + H.hd1 * dz);
@@ -360,39 +361,60 @@ __device__ __forceinline__ void build_rows_lds(const Halo &H, const TileGeom &t,
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             sl[u] = min(s0 + u, se - 1);
-            const float ux = xf[sl[u]] - xfi, uy = yf[sl[u]] - yfi, uz = zf[sl[u]] - zfi;
+            const float ux = xf[sl[u]] - xi, uy = yf[sl[u]] - yi, uz = zf[sl[u]] - zi;
             r2[u] = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            bool keep = s0 + u < se && sl[u] != own && r2[u] <= A.hi2f;
-            if (keep && r2[u] >= A.lo2f)  // near the cut: the reference's rounding decides
-              keep = pair_rd(H, t, pos, pa, sl[u], scell[sl[u]]) < A.rl2;
-            rb[(keep && m < cap ? m : cap) * GH_TBLOCK] = (unsigned short)sl[u];
+            const bool keep = (s0 + u < se) & (sl[u] != own) & (r2[u] <= A.hi2f);
+            const int e = sl[u] | (r2[u] >= A.lo2f ? 0x8000 : 0);
+            rb[(keep ? min(m, cap) : cap) * GH_TBLOCK] = (unsigned short)e;
             m += keep ? 1 : 0;
           }
         }
       }
-    const int mw = min(m, cap);
-    for (int k = 0; k < mw; ++k) {
-      const int s = rb[k * GH_TBLOCK];
-      const float ux = xf[s] - xfi, uy = yf[s] - yfi, uz = zf[s] - zfi;
-      int c = row_class_f(fmaf(ux, ux, fmaf(uy, uy, uz * uz)), A);
-      if (c < 0) c = row_class(pair_rd(H, t, pos, pa, s, scell[s]), A);
-      rb[k * GH_TBLOCK] = (unsigned short)(s | (c << 14));
-    }
-    RowWriter row{reinterpret_cast<unsigned long long *>(A.list16), A.ld, a, cap, 0, 0ull};
-    for (int c = 0; c < 4; ++c)
-      for (int k = 0; k < mw; ++k) {
-        const int e = rb[k * GH_TBLOCK];
-        if ((e >> 14) == c) row.put(e & 0x3FFF);
+    // Second sweep: exact test of the flagged entries (the reference's
+    // rounding decides; failures are dropped, compacting in place) and the
+    // class of each kept pair in bits 14-15 (staged slots number below 16384).
+    int mw = 0, n0 = 0, n1 = 0, n2 = 0;
+    for (int k = 0, ke = min(m, cap); k < ke; ++k) {
+      const int e = rb[k * GH_TBLOCK];
+      const int s = e & 0x3FFF;
+      int c;
+      if (e & 0x8000) {
+        const double rd = pair_rd(H, t, pos, pa, s, scell[s]);
+        if (!(rd < A.rl2)) continue;
+        c = row_class(rd, A);
+      } else {
+        const float ux = xf[s] - xi, uy = yf[s] - yi, uz = zf[s] - zi;
+        c = row_class_f(fmaf(ux, ux, fmaf(uy, uy, uz * uz)), A);
+        if (c < 0) c = row_class(pair_rd(H, t, pos, pa, s, scell[s]), A);
       }
-    row.finish(pad);
+      rb[(mw++) * GH_TBLOCK] = (unsigned short)(s | (c << 14));
+      n0 += c == 0 ? 1 : 0;
+      n1 += c == 1 ? 1 : 0;
+      n2 += c == 2 ? 1 : 0;
+    }
+    // One sweep writes each entry straight to its place in class order.
+    unsigned short *row = A.list16 + ((size_t)a << 2);
+    const size_t qstride = (size_t)A.ld << 2;
+    int d0 = 0, d1 = n0, d2 = n0 + n1, d3 = n0 + n1 + n2;
+    for (int k = 0; k < mw; ++k) {
+      const int e = rb[k * GH_TBLOCK];
+      const int c = e >> 14;
+      const int d = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
+      d0 += c == 0 ? 1 : 0;
+      d1 += c == 1 ? 1 : 0;
+      d2 += c == 2 ? 1 : 0;
+      d3 += c == 3 ? 1 : 0;
+      row[(d >> 2) * qstride + (d & 3)] = (unsigned short)(e & 0x3FFF);
+    }
+    for (int d = mw; d & 3; ++d) row[(d >> 2) * qstride + (d & 3)] = (unsigned short)pad;
     A.nn[a] = mw;
     A.xold[a] = pa.x;
     A.xold[A.ncap + a] = pa.y;
     A.xold[2 * A.ncap + a] = pa.z;
-    mmax = max(mmax, m);
+    mmax = max(mmax, m);  // before the exact drops: an overflow may have lost entries
   }
 }
 
@@ -609,6 +631,21 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_cell_rank_gather(CellArgs R) {
   cell_rank_gather(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
+// Brick-local FP32 positions of the halo slots as three arrays.
+__device__ __forceinline__ void stage_local(const Halo &H, const TileGeom &t,
+                                            const double4 *__restrict__ pos,
+                                             const unsigned char *scell, float *xf, float *yf,
+                                             float *zf) {
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const double4 p = pos[s + H.delta[h]];
+    xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
+    yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
+    zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
+  }
+  __syncthreads();
+}
+
 // Rows of the atoms of brick b (all threads of the workgroup call it; LDS
 // is free again on return).
 __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
@@ -619,21 +656,14 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
   const int pad = t.slot_cap;  // sentinel slot of the force passes
   const int cap = A.cap, lds_slots = A.lds_slots;
   if (A.lds_rows) {
-    // LDS: rows [(cap + 1) * GH_TBLOCK] u16, then xf, yf, zf, slot map
+    // LDS: rows [(cap + 1) * GH_TBLOCK] u16, then float4 positions, slot map
     unsigned short *rowbuf = reinterpret_cast<unsigned short *>(dynf);
-    float *xf = dynf + ((cap + 1) * GH_TBLOCK + 1) / 2;
+    float *xf = dynf + (cap + 1) * GH_TBLOCK / 2;
     if (H.total <= lds_slots) {
-      float *yf = xf + lds_slots, *zf = xf + 2 * lds_slots;
-      unsigned char *scell = reinterpret_cast<unsigned char *>(xf + 3 * lds_slots);
+      float *yf = xf + lds_slots, *zf = yf + lds_slots;
+      unsigned char *scell = reinterpret_cast<unsigned char *>(zf + lds_slots);
       slot_map(H, scell);
-      for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-        const int h = scell[s];
-        const double4 p = pos[s + H.delta[h]];
-        xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
-        yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
-        zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
-      }
-      __syncthreads();
+      stage_local(H, t, pos, scell, xf, yf, zf);
       build_rows_lds(H, t, pos, cellof, xf, yf, zf, scell, rowbuf, A, pad, mmax);
     } else {
       // halo too large for the slots left beside the rows: unstaged path
@@ -644,17 +674,10 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
     unsigned short *far = reinterpret_cast<unsigned short *>(dynf);
     float *xf = dynf + GH_FARMAX * GH_TBLOCK / 2;
     if (H.total <= lds_slots) {
-      float *yf = xf + lds_slots, *zf = xf + 2 * lds_slots;
-      unsigned char *scell = reinterpret_cast<unsigned char *>(xf + 3 * lds_slots);
+      float *yf = xf + lds_slots, *zf = yf + lds_slots;
+      unsigned char *scell = reinterpret_cast<unsigned char *>(zf + lds_slots);
       slot_map(H, scell);
-      for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
-        const int h = scell[s];
-        const double4 p = pos[s + H.delta[h]];
-        xf[s] = (float)(p.x - (H.org[0] - shift(t, H, 0, h)));
-        yf[s] = (float)(p.y - (H.org[1] - shift(t, H, 1, h)));
-        zf[s] = (float)(p.z - (H.org[2] - shift(t, H, 2, h)));
-      }
-      __syncthreads();
+      stage_local(H, t, pos, scell, xf, yf, zf);
       build_rows<true>(H, t, pos, cellof, xf, yf, zf, far, A, pad, mmax);
     } else {
       build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A, pad, mmax);
@@ -1052,8 +1075,9 @@ size_t gh_tile_static_bytes(int precision, int nt) {
   return sizeof(Halo) + params;
 }
 
-// LDS of the build: the far-entry buffer, then FP32 local coordinates (12 B)
-// and the slot map (1 B) per staged slot.
+// LDS of the build: the far-entry buffer, then FP32 local coordinates (three
+// arrays: 12 B; one float4 array reads with bank conflicts) and the slot map
+// (1 B) per staged slot.
 static size_t build_far_bytes() { return (size_t)GH_FARMAX * GH_TBLOCK * 2; }
 static size_t build_slots() { return (GH_BUILD_LDS - build_far_bytes()) / 13; }
 
@@ -1083,7 +1107,8 @@ static BuildArgs build_args(const gh_ctx *c, int cap, size_t *lds_out) {
   BuildArgs A;
   int lds_slots = (int)std::min<size_t>(build_slots(), (size_t)c->tg.slot_cap);
   size_t lds = build_far_bytes() + 13 * (size_t)lds_slots;
-  // rows in LDS when they and the staged halo fit (the branch-free variant)
+  // rows in LDS when they and the staged halo fit (the branch-free variant;
+  // 13 B per staged slot: three floats and the slot map)
   const size_t rows = ((size_t)(cap + 1) * GH_TBLOCK * 2 + 3) / 4 * 4;
   const size_t slots_left = rows < GH_BUILD_LDS ? (GH_BUILD_LDS - rows) / 13 : 0;
   int lds_rows = 0;
