@@ -734,7 +734,8 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
 
 // ---------------------------------------------------------------------------
 // SMA-TB, mixed precision (fixed-point LDS).  Pass 1: smatb_preinteraction
-// (TB.F90:367-455); LDS slot = {qx, qy, qz, type}.
+// (TB.F90:367-455); LDS per slot: qx, qy, qz in three int arrays (one
+// 16-B array reads with more bank conflicts), the slot map and the type.
 // ---------------------------------------------------------------------------
 template <bool SINGLE>
 __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
@@ -742,7 +743,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbF *__restrict__ tbf,
     double *__restrict__ rho, float *__restrict__ eband32) {
-  extern __shared__ int4 q4[];
+  extern __shared__ int qa[];
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
   for (int k = threadIdx.x; k < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k += blockDim.x) sp[k] = tbf[k];
@@ -750,17 +751,23 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
-  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + ns);
+  int *qxa = qa, *qya = qa + ns, *qza = qa + 2 * ns;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(qa + 3 * ns);
+  unsigned char *ty = scell + ns;
   slot_map(H, scell);
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
     const int src = s + H.delta[h];
     const double4 p = pos[src];
-    q4[s] = make_int4(fxq(p.x - (H.org[0] - shift(t, H, 0, h))),
-                      fxq(p.y - (H.org[1] - shift(t, H, 1, h))),
-                      fxq(p.z - (H.org[2] - shift(t, H, 2, h))), SINGLE ? 0 : type[src]);
+    qxa[s] = fxq(p.x - (H.org[0] - shift(t, H, 0, h)));
+    qya[s] = fxq(p.y - (H.org[1] - shift(t, H, 1, h)));
+    qza[s] = fxq(p.z - (H.org[2] - shift(t, H, 2, h)));
+    if (!SINGLE) ty[s] = (unsigned char)type[src];
   }
-  if (threadIdx.x == 0) q4[t.slot_cap] = make_int4(GH_FXFAR, GH_FXFAR, GH_FXFAR, 0);
+  if (threadIdx.x == 0) {
+    qxa[t.slot_cap] = qya[t.slot_cap] = qza[t.slot_cap] = GH_FXFAR;
+    if (!SINGLE) ty[t.slot_cap] = 0;
+  }
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
@@ -768,27 +775,32 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
               qz = fxq(pa.z - (H.org[2] - 0.0));
     const int ti = SINGLE ? 0 : type[a];
     double acc = 0.0;
-    auto pair = [&](const int4 &q) {
-      const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
+    auto pair = [&](int jx, int jy, int jz, int s) {
+      const double dx = (double)(jx - qx), dy = (double)(jy - qy), dz = (double)(jz - qz);
       float band;
-      if (tb_rho_fx(SINGLE ? sp[0] : sp[ti * GH_MAXT + q.w], fma(dx, dx, fma(dy, dy, dz * dz)),
+      if (tb_rho_fx(SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]], fma(dx, dx, fma(dy, dy, dz * dz)),
                     band))
         acc += (double)band;
     };
     for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
-      const int4 q0 = q4[GH_QSLOT(w, 0)], q1 = q4[GH_QSLOT(w, 1)], q2 = q4[GH_QSLOT(w, 2)],
-                 q3 = q4[GH_QSLOT(w, 3)];
-      pair(q0);
-      pair(q1);
-      pair(q2);
-      pair(q3);
+      const int s0 = GH_QSLOT(w, 0), s1 = GH_QSLOT(w, 1), s2 = GH_QSLOT(w, 2), s3 = GH_QSLOT(w, 3);
+      const int x0 = qxa[s0], y0 = qya[s0], z0 = qza[s0];
+      const int x1 = qxa[s1], y1 = qya[s1], z1 = qza[s1];
+      const int x2 = qxa[s2], y2 = qya[s2], z2 = qza[s2];
+      const int x3 = qxa[s3], y3 = qya[s3], z3 = qza[s3];
+      pair(x0, y0, z0, s0);
+      pair(x1, y1, z1, s1);
+      pair(x2, y2, z2, s2);
+      pair(x3, y3, z3, s3);
     });
     rho[a] = acc;
     eband32[a] = (float)(1.0 / sqrt(acc));
   }
 }
 
-// Pass 2: smatb_interact (TB.F90:457-586); LDS slot = {qx, qy, qz, eband_j}.
+// Pass 2: smatb_interact (TB.F90:457-586); LDS slot = {qx, qy, qz, eband_j}
+// (one 16-B read per neighbour: here fewer instructions outweigh the bank
+// conflicts that make the density pass prefer separate arrays).
 template <bool SINGLE>
 __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
@@ -1062,7 +1074,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
 size_t gh_tile_slot_bytes(int pass, int precision, int nt) {
   const bool multi = nt > 1;
   if (precision == GH_MIXED) {
-    if (pass == 0) return 16 + 1;                 // rho: int4 + slot map
+    if (pass == 0) return 12 + 1 + (multi ? 1 : 0);  // rho: qx, qy, qz + map (+ type)
     return 16 + 1 + (multi ? 1 : 0);              // force: int4 + map + type
   }
   if (pass == 0) return 24 + 1 + (multi ? 4 : 0);  // rho: xyz + map (+ type)
