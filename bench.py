@@ -64,17 +64,25 @@ def make_ctx(hip, precision, x, L, skin, lcg_seed=4242, T=300.0, dt=0.002):
     return c
 
 
-def profiled_traffic(kernel="k_tb_force_fx"):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE, calibrated against a 1 GiB device copy)."""
+def profiled(field, kernel="k_tb_force_fx"):
+    """A per-launch figure of `kernel` from the committed rocprofv3 summary:
+    traffic_bytes_per_launch (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    1 GiB device copy) or valu_insts_per_launch (SQ_INSTS_VALU)."""
     path = os.path.join(REPO, "profiles", "k_tb_force_fx_au1m_counters.md")
     try:
         for ln in open(path):
-            if ln.startswith("traffic_bytes_per_launch %s:" % kernel):
+            if ln.startswith("%s %s:" % (field, kernel)):
                 return int(ln.split(":")[1]), os.path.relpath(path, REPO)
     except OSError:
         pass
     return None, None
+
+
+def fixed_tile(c, spec):
+    """Sets the brick shape 'bx,by,bz'; returns (shape, None) like autotune."""
+    bd = tuple(int(v) for v in spec.split(","))
+    c.set_tile(bd)
+    return bd, None
 
 
 def timed_run(c, steps, warmup, timing=False):
@@ -252,6 +260,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=63)
     ap.add_argument("--skin", type=float, default=1.0)
+    ap.add_argument("--tile", default=None,
+                    help="brick shape 'bx,by,bz' for the mixed run instead of the autotuner "
+                         "(profiling runs fix it so that every run launches the same grids)")
+    ap.add_argument("--tile64", default=None, help="the same for the FP64 run")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -278,7 +290,7 @@ def main():
 
     # mixed precision (the headline)
     c = make_ctx(hip, hip.GH_MIXED, x, L, args.skin)
-    tile_mx = c.autotune(reps=3)
+    tile_mx = fixed_tile(c, args.tile) if args.tile else c.autotune(reps=3)
     F_mx, _, et_mx = c.compute()
     e0 = c.energies()
     dt_mx, _, nupd_mx = timed_run(c, args.steps, args.warmup)
@@ -289,7 +301,7 @@ def main():
     # plain FP64 GPU form, same input
     k64 = args.fp64_steps or max(10, args.steps // 2)
     c = make_ctx(hip, hip.GH_FP64, x, L, args.skin)
-    tile_64 = c.autotune(reps=3)
+    tile_64 = fixed_tile(c, args.tile64) if args.tile64 else c.autotune(reps=3)
     F_64, _, et_64 = c.compute()
     dt_64, _, nupd_64 = timed_run(c, k64, max(2, args.warmup // 2))
     _, tm_64, _ = timed_run(c, k64, 0, timing=True)
@@ -307,7 +319,8 @@ def main():
     t_rho = ms_rho / cnt_rho * 1e-3
     bytes_force = BYTES_PASS2 + BYTES_FUSED_VV
     achieved = bytes_force * n / t_force / 1e9
-    traffic, traffic_src = profiled_traffic()
+    traffic, traffic_src = profiled("traffic_bytes_per_launch")
+    valu_insts, _ = profiled("valu_insts_per_launch")
     out = {
         "metric": METRIC,
         "value": value,
@@ -340,6 +353,12 @@ def main():
             "traffic_source": traffic_src,
             "bytes_per_atom": bytes_force,
             "avg_launch_us": t_force * 1e6,
+            "valu_issue": None if valu_insts is None else {
+                "insts_per_launch": valu_insts,
+                "issue_us": valu_insts * 4 / (1024 * 2.4e3),
+                "frac_of_launch": valu_insts * 4 / (1024 * 2.4e3) / (t_force * 1e6),
+                "note": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz): the pass is bound "
+                        "by FP32/FP64 vector issue of the pair terms, not by HBM"},
             "timed_with": "HIP events around every launch, over a second timed region of "
                           "the same %d steps (%.4f ms/step with the events)"
                           % (args.steps, dt_ev / args.steps * 1e3),
