@@ -176,6 +176,42 @@ def drift_vs_reference(hip):
     return out
 
 
+def lj_bulk(hip, nx=64, steps=100, warmup=10):
+    """Lennard-Jones (lj_interact, Pairs.F90:207-279) on the classic LJ-liquid
+    benchmark geometry: fcc nx^3 cells at rho* = 0.8442 (4 nx^3 atoms), rc =
+    2.5 sigma, skin 0.3 sigma, eps = 1 eV, sigma = 1 A, m = 1 amu, T* = 1.44
+    from the lcg stream, velocity Verlet dt = 0.005 tau; mixed and FP64."""
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    x = fcc(nx, a) + np.random.default_rng(7).normal(0.0, 0.01, (4 * nx ** 3, 3))
+    L = nx * a
+    x = np.mod(x, L)
+    tau = 1e-10 * np.sqrt(1.66053906660e-27 / 1.602176634e-19) * 1e12  # ps
+    T = 1.44 * 1.602176634e-19 / 1.380649e-23
+    out = {"workload": "LJ fcc %d^3 cells = %d atoms, rho* 0.8442, rc 2.5 sigma, skin 0.3 sigma, "
+                       "T* 1.44, VV dt 0.005 tau, periodic" % (nx, len(x))}
+    F = {}
+    for name, prec in (("mixed", hip.GH_MIXED), ("fp64", hip.GH_FP64)):
+        c = hip.Context(0, prec)
+        c.set_box([L] * 3, (1, 1, 1))
+        c.set_atoms(x)
+        c.set_masses([1.0])
+        c.set_lj(1.0, 1.0, 2.5)
+        c.set_skin(0.3)
+        c.prng_lcg(4242)
+        c.set_gdist(T)
+        c.integrator_vv(0.005 * tau)
+        shape, _ = c.autotune(reps=3)
+        F[name] = c.compute()[0]
+        dt, _, nupd = timed_run(c, steps, warmup)
+        c.close()
+        out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "steps": steps, "rebuilds": nupd, "brick_cells": shape}
+    rms = float(np.sqrt(np.mean(F["fp64"] ** 2)))
+    out["force_err_mixed_vs_fp64_rms_over_rms"] = float(
+        np.sqrt(np.mean((F["mixed"] - F["fp64"]) ** 2)) / rms)
+    return out
+
+
 def batch_re(hip, rounds, swap_every=500):
     """Replica-exchange Langevin of 512 Ag-Cu clusters (64 geometries x 8
     temperatures, 300..900 K geometric), friction 1/ps, dt 2 fs, swaps every
@@ -386,6 +422,7 @@ def main():
             "vs_reference_32k": drift_vs_reference(hip),
         }
     if not args.no_batch:
+        out["lj_bulk"] = lj_bulk(hip)
         out["batched_re"] = batch_re(hip, args.re_rounds)
         out["batched_fire"] = batch_fire(hip, args.fire_clusters)
     if not args.no_cpu:

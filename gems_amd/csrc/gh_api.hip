@@ -194,6 +194,7 @@ int gh_upload_tb(gh_ctx *c) {
   GH_HIP(c, hipMemcpy(c->tbd, h, sizeof(h), hipMemcpyHostToDevice));
   GH_TRY(dalloc(c, &c->tbf, GH_MAXT * GH_MAXT));
   GH_HIP(c, hipMemcpy(c->tbf, f, sizeof(f), hipMemcpyHostToDevice));
+  c->tbf0 = f[0];
   return 0;
 }
 

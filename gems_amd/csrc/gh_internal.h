@@ -182,6 +182,7 @@ struct gh_ctx {
   unsigned short *list16 = nullptr;           // [cap][ncap] ELL of LDS slots (cell mode)
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
   TbF *tbf = nullptr;                         // [MAXT*MAXT]
+  TbF tbf0;                                   // host copy of tbf[0] (one-type kernel argument)
   GhStatus *status = nullptr;
   double *red = nullptr;                      // reduction scratch
   int nred = 0;

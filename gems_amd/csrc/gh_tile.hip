@@ -35,7 +35,7 @@ struct Halo {
   double org[3];                  // brick centre: origin of the fixed-point frame
   int delta[GH_MAXHALO];          // atom in slot s of halo cell h: s + delta[h]
   int hbase[GH_MAXHALO + 1];      // first slot of halo cell h
-  int scan[GH_TBLOCK];
+  int wsum[GH_TBLOCK / 64];       // halo_setup's per-wavefront count totals
   signed char shc[3][GH_MAXHALO]; // periodic image shift of halo cell h (box lengths)
 };
 
@@ -92,23 +92,30 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
     H.shc[1][h] = (signed char)sc[1];
     H.shc[2][h] = (signed char)sc[2];
   }
-  // exclusive scan of the halo cell counts (Hillis-Steele over the block)
-  H.scan[threadIdx.x] = cnt;
+  // exclusive scan of the halo cell counts: within each wavefront by
+  // shuffles, then across the four wavefronts
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int inc = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) H.wsum[wv] = inc;
   __syncthreads();
-  for (int o = 1; o < GH_TBLOCK; o <<= 1) {
-    const int v = threadIdx.x >= o ? H.scan[threadIdx.x - o] : 0;
-    __syncthreads();
-    H.scan[threadIdx.x] += v;
-    __syncthreads();
+  int woff = 0, total = 0;
+  for (int w = 0; w < GH_TBLOCK / 64; ++w) {
+    const int x = H.wsum[w];
+    if (w < wv) woff += x;
+    total += x;
   }
   if (h < nh) {
-    const int base = H.scan[h] - cnt;
+    const int base = woff + inc - cnt;
     H.hbase[h] = base;
     H.delta[h] = start - base;
   }
   if (h == 0) {
-    H.hbase[nh] = H.scan[nh - 1];
-    H.total = H.scan[nh - 1];
+    H.hbase[nh] = total;
+    H.total = total;
     H.nh = nh;
     H.hd0 = hd0;
     H.hd1 = hd1;
@@ -742,11 +749,12 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbF *__restrict__ tbf,
-    double *__restrict__ rho, float *__restrict__ eband32) {
+    double *__restrict__ rho, float *__restrict__ eband32, TbF p1) {
   extern __shared__ int qa[];
   __shared__ Halo H;
-  __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
-  for (int k = threadIdx.x; k < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k += blockDim.x) sp[k] = tbf[k];
+  __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
+  if (!SINGLE)
+    for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) sp[k] = tbf[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
@@ -778,7 +786,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     auto pair = [&](int jx, int jy, int jz, int s) {
       const double dx = (double)(jx - qx), dy = (double)(jy - qy), dz = (double)(jz - qz);
       float band;
-      if (tb_rho_fx(SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]], fma(dx, dx, fma(dy, dy, dz * dz)),
+      if (tb_rho_fx(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]], fma(dx, dx, fma(dy, dy, dz * dz)),
                     band))
         acc += (double)band;
     };
@@ -807,11 +815,12 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
     const double *__restrict__ rho, const float *__restrict__ eband32, double ev_ui,
-    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv) {
+    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, TbF p1) {
   extern __shared__ int4 q4[];
   __shared__ Halo H;
-  __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
-  for (int k = threadIdx.x; k < (SINGLE ? 1 : GH_MAXT * GH_MAXT); k += blockDim.x) sp[k] = tbf[k];
+  __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
+  if (!SINGLE)
+    for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) sp[k] = tbf[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
@@ -845,7 +854,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     auto pair = [&](const int4 &q, int s) {
       const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
       float sf, repul;
-      if (tb_force_fx(SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]],
+      if (tb_force_fx(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]],
                       fma(dx, dx, fma(dy, dy, dz * dz)), ebi + __int_as_float(q.w), sf, repul)) {
         const double sfd = (double)sf;
         fx = fma(-sfd, dx, fx);
@@ -1075,7 +1084,7 @@ size_t gh_tile_slot_bytes(int pass, int precision, int nt) {
   const bool multi = nt > 1;
   if (precision == GH_MIXED) {
     if (pass == 0) return 12 + 1 + (multi ? 1 : 0);  // rho: qx, qy, qz + map (+ type)
-    return 16 + 1 + (multi ? 1 : 0);              // force: int4 + map + type
+    return 16 + 1 + (multi ? 1 : 0);                 // force: int4 + map (+ type)
   }
   if (pass == 0) return 24 + 1 + (multi ? 4 : 0);  // rho: xyz + map (+ type)
   return 32 + 1 + (multi ? 4 : 0);                 // force: xyz + eband + map (+ type)
@@ -1246,20 +1255,20 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (single)
         k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                         c->cell_start, c->nn, c->list16, ld,
-                                                        c->tbf, c->rho, c->eband32);
+                                                        c->tbf, c->rho, c->eband32, c->tbf0);
       else
         k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                          c->cell_start, c->nn, c->list16, ld,
-                                                         c->tbf, c->rho, c->eband32);
+                                                         c->tbf, c->rho, c->eband32, c->tbf0);
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
       if (single)
         k_tb_force_fx<true><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv);
+            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0);
       else
         k_tb_force_fx<false><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv);
+            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0);
     } else {
       if (single)
         k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
