@@ -18,7 +18,8 @@
 
 #include "gh_device.h"
 
-#define GH_BBLOCK 512  // largest workgroup of the batch kernels
+#define GH_BBLOCK 512  // largest workgroup of the batch kernels for the ordinary systems
+#define GH_BIGSYS 1024  // systems above this many atoms get 1024-thread workgroups
 
 struct gh_batch {
   int device = 0;
@@ -49,6 +50,11 @@ struct gh_batch {
   TbF *tbf = nullptr;
   int nst = 0;   // LDS staging slots per workgroup (largest system + sentinel)
   int nmax = 0;  // atoms of the largest system
+  // systems above GH_BIGSYS atoms (the first nbig of the largest-first order)
+  // run on a second stream beside the rest; nmax_rest = the largest of the rest
+  int nbig = 0, nmax_rest = 0;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -56,6 +62,7 @@ using namespace ghd;
 
 struct BatchArgs {
   const int *off, *order, *type;
+  int first;  // this launch's workgroup b works on system order[first + b]
   double4 *pos;
   double *vel, *acc, *frc, *e, *rho, *eband, *xold;
   float *eband32;
@@ -98,6 +105,7 @@ __device__ __forceinline__ void bsum(BlockRed<BS> &R, double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
+#pragma unroll 4  // fully unrolled, 16 wavefronts x 8 values would all sit in registers
     for (int w = 0; w < BS / 64; ++w) s += R.v[w][k];
     v[k] = s;
   }
@@ -111,6 +119,7 @@ __device__ __forceinline__ double bmax(BlockRed<BS> &R, double v) {
   if ((threadIdx.x & 63) == 0) R.v[threadIdx.x >> 6][0] = v;
   __syncthreads();
   double m = R.v[0][0];
+#pragma unroll 4
   for (int w = 1; w < BS / 64; ++w) m = fmax(m, R.v[w][0]);
   __syncthreads();
   return m;
@@ -348,7 +357,7 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
     sp[k_] = B.tbd[k_];                                                                    \
     sf[k_] = B.tbf[k_];                                                                    \
   }                                                                                        \
-  const int s = B.order[blockIdx.x];                                                       \
+  const int s = B.order[B.first + blockIdx.x];                                             \
   Sys S;                                                                                   \
   S.a0 = B.off[s];                                                                         \
   S.n = B.off[s + 1] - S.a0;                                                               \
@@ -642,6 +651,7 @@ static BatchArgs bargs(const gh_batch *b) {
   BatchArgs A;
   A.off = b->off;
   A.order = b->order;
+  A.first = 0;
   A.type = b->type;
   A.pos = b->pos;
   A.vel = b->vel;
@@ -735,6 +745,32 @@ static int batch_ready(gh_batch *b) {
 
 static size_t batch_lds(const gh_batch *b) { return 17 * (size_t)b->nst; }
 
+// One batch kernel over all systems.  The systems above GH_BIGSYS atoms bound
+// the run (one workgroup each, largest first), so they get 1024-thread
+// workgroups (four wavefronts per SIMD) on a second stream while the rest
+// run on the batch stream; `big(first, count, stream)` and `rest(first,
+// count, nmax, stream)` launch the two parts.  Variants whose 1024-thread
+// form would spill registers pass big_ok = false and keep 512 threads.
+template <class BIG, class REST>
+static int split_launch(gh_batch *b, bool big_ok, BIG &&big, REST &&rest) {
+  const int nbig = big_ok ? b->nbig : 0;
+  if (nbig > 0) {
+    BH(b, hipEventRecord(b->ev_fork, b->stream));
+    BH(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+    big(0, nbig, b->stream2);
+    BH(b, hipGetLastError());
+  }
+  if (b->nsys > nbig) {
+    rest(nbig, b->nsys - nbig, nbig > 0 ? b->nmax_rest : b->nmax, b->stream);
+    BH(b, hipGetLastError());
+  }
+  if (nbig > 0) {
+    BH(b, hipEventRecord(b->ev_join, b->stream2));
+    BH(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  }
+  return 0;
+}
+
 extern "C" {
 
 int gh_batch_create(gh_batch **out, int device, int precision) {
@@ -748,6 +784,9 @@ int gh_batch_create(gh_batch **out, int device, int precision) {
   b->precision = precision;
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void **)&b->flags, sizeof(int) * 4) != hipSuccess ||
       hipMemset(b->flags, 0, sizeof(int) * 4) != hipSuccess) {
     delete b;
@@ -761,12 +800,16 @@ void gh_batch_destroy(gh_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   void *p[] = {b->off, b->order, b->type, b->nn, b->flags, b->pos, b->vel, b->acc, b->frc,
                b->e, b->rho, b->eband, b->xold, b->T, b->out, b->scale, b->eband32, b->list,
                b->tbd, b->tbf, b->gn};
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   delete b;
 }
 
@@ -799,6 +842,10 @@ int gh_batch_set_systems(gh_batch *b, int nsys, const int *off, const double *x,
   std::stable_sort(b->order_h.begin(), b->order_h.end(), [&](int a, int c) {
     return off[a + 1] - off[a] > off[c + 1] - off[c];
   });
+  b->nbig = 0;
+  while (b->nbig < nsys && off[b->order_h[b->nbig] + 1] - off[b->order_h[b->nbig]] > GH_BIGSYS)
+    ++b->nbig;
+  b->nmax_rest = b->nbig < nsys ? off[b->order_h[b->nbig] + 1] - off[b->order_h[b->nbig]] : 0;
   b->T_h.assign(nsys, 0.0);
   BT(balloc(b, &b->off, nsys + 1));
   BT(balloc(b, &b->order, nsys));
@@ -907,18 +954,29 @@ int gh_batch_compute(gh_batch *b, double *F, double *epot) {
   BT(batch_ready(b));
   const BatchArgs A = bargs(b);
   const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
-#define LAUNCH(MX, SG)                                                                   \
-  if (b->nmax > 512)                                                                     \
-    k_batch_compute<MX, SG, 512><<<b->nsys, 512, batch_lds(b), b->stream>>>(A);        \
-  else                                                                                   \
-    k_batch_compute<MX, SG, 256><<<b->nsys, 256, batch_lds(b), b->stream>>>(A)
+  const size_t lds = batch_lds(b);
+#define LAUNCH(MX, SG)                                                                     \
+  BT(split_launch(                                                                         \
+      b, true,                                                                             \
+      [&](int first, int count, hipStream_t st) {                                          \
+        BatchArgs A1 = A;                                                                  \
+        A1.first = first;                                                                  \
+        k_batch_compute<MX, SG, 1024><<<count, 1024, lds, st>>>(A1);                       \
+      },                                                                                   \
+      [&](int first, int count, int nmax, hipStream_t st) {                                \
+        BatchArgs A1 = A;                                                                  \
+        A1.first = first;                                                                  \
+        if (nmax > 512)                                                                    \
+          k_batch_compute<MX, SG, 512><<<count, 512, lds, st>>>(A1);                       \
+        else                                                                               \
+          k_batch_compute<MX, SG, 256><<<count, 256, lds, st>>>(A1);                       \
+      }))
   if (mixed) {
     if (single) LAUNCH(true, true); else LAUNCH(true, false);
   } else {
     if (single) LAUNCH(false, true); else LAUNCH(false, false);
   }
 #undef LAUNCH
-  BH(b, hipGetLastError());
   BT(check_flags(b));
   b->lists_valid = true;
   b->forces_valid = true;
@@ -955,20 +1013,36 @@ int gh_batch_langevin(gh_batch *b, int nsteps, double dt, double gamma, uint64_t
   k.kB_ui = ghc::kB_ui();
   const BatchArgs A = bargs(b);
   const bool mixed = b->precision == GH_MIXED, single = b->nt == 1;
-#define LAUNCH(MX, SG)                                                                   \
-  if (b->nmax > 512)                                                                     \
-    k_batch_langevin<MX, SG, 512><<<b->nsys, 512, batch_lds(b), b->stream>>>(           \
-        A, nsteps, b->step + 1, k, seed, b->skin);                                       \
-  else                                                                                   \
-    k_batch_langevin<MX, SG, 256><<<b->nsys, 256, batch_lds(b), b->stream>>>(             \
-        A, nsteps, b->step + 1, k, seed, b->skin)
+  const size_t lds = batch_lds(b);
+  const long long step0 = b->step + 1;
+  // one-type variants need more than the 128 VGPRs of a 1024-thread workgroup
+#define LAUNCH(MX, SG)                                                                     \
+  BT(split_launch(                                                                         \
+      b, !SG,                                                                              \
+      [&](int first, int count, hipStream_t st) {                                          \
+        if constexpr (!SG) {                                                               \
+          BatchArgs A1 = A;                                                                \
+          A1.first = first;                                                                \
+          k_batch_langevin<MX, SG, 1024><<<count, 1024, lds, st>>>(A1, nsteps, step0, k,   \
+                                                                   seed, b->skin);         \
+        }                                                                                  \
+      },                                                                                   \
+      [&](int first, int count, int nmax, hipStream_t st) {                                \
+        BatchArgs A1 = A;                                                                  \
+        A1.first = first;                                                                  \
+        if (nmax > 512)                                                                    \
+          k_batch_langevin<MX, SG, 512><<<count, 512, lds, st>>>(A1, nsteps, step0, k,     \
+                                                                 seed, b->skin);           \
+        else                                                                               \
+          k_batch_langevin<MX, SG, 256><<<count, 256, lds, st>>>(A1, nsteps, step0, k,     \
+                                                                 seed, b->skin);           \
+      }))
   if (mixed) {
     if (single) LAUNCH(true, true); else LAUNCH(true, false);
   } else {
     if (single) LAUNCH(false, true); else LAUNCH(false, false);
   }
 #undef LAUNCH
-  BH(b, hipGetLastError());
   b->step += nsteps;
   BT(check_flags(b));
   std::vector<double> o;
@@ -1042,18 +1116,31 @@ int gh_batch_fire(gh_batch *b, int max_iter, double dt_max, double alpha0, doubl
   fc.max_iter = max_iter;
   fc.mixed = b->precision == GH_MIXED;
   const BatchArgs A = bargs(b);
-#define LAUNCH(SG)                                                                       \
-  if (b->nmax > 512)                                                                     \
-    k_batch_fire<SG, 512><<<b->nsys, 512, batch_lds(b), b->stream>>>(A, fc, b->skin);  \
-  else                                                                                   \
-    k_batch_fire<SG, 256><<<b->nsys, 256, batch_lds(b), b->stream>>>(A, fc, b->skin)
+  const size_t lds = batch_lds(b);
+#define LAUNCH(SG)                                                                         \
+  BT(split_launch(                                                                         \
+      b, !SG,                                                                              \
+      [&](int first, int count, hipStream_t st) {                                          \
+        if constexpr (!SG) {                                                               \
+          BatchArgs A1 = A;                                                                \
+          A1.first = first;                                                                \
+          k_batch_fire<SG, 1024><<<count, 1024, lds, st>>>(A1, fc, b->skin);               \
+        }                                                                                  \
+      },                                                                                   \
+      [&](int first, int count, int nmax, hipStream_t st) {                                \
+        BatchArgs A1 = A;                                                                  \
+        A1.first = first;                                                                  \
+        if (nmax > 512)                                                                    \
+          k_batch_fire<SG, 512><<<count, 512, lds, st>>>(A1, fc, b->skin);                 \
+        else                                                                               \
+          k_batch_fire<SG, 256><<<count, 256, lds, st>>>(A1, fc, b->skin);                 \
+      }))
   if (b->nt == 1) {
     LAUNCH(true);
   } else {
     LAUNCH(false);
   }
 #undef LAUNCH
-  BH(b, hipGetLastError());
   BT(check_flags(b));
   b->lists_valid = true;
   b->forces_valid = false;  // the last forces may be FP64 while precision is mixed
