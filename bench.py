@@ -176,19 +176,22 @@ def drift_vs_reference(hip):
     return out
 
 
-def lj_bulk(hip, nx=64, steps=100, warmup=10):
-    """Lennard-Jones (lj_interact, Pairs.F90:207-279) on the classic LJ-liquid
-    benchmark geometry: fcc nx^3 cells at rho* = 0.8442 (4 nx^3 atoms), rc =
-    2.5 sigma, skin 0.3 sigma, eps = 1 eV, sigma = 1 A, m = 1 amu, T* = 1.44
-    from the lcg stream, velocity Verlet dt = 0.005 tau; mixed and FP64."""
+def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72):
+    """Lennard-Jones (lj_interact, Pairs.F90:207-279) on the LJ-liquid geometry
+    of the LJ NVE configuration: fcc nx^3 cells at rho* = 0.8442 (4 nx^3
+    atoms), rc = 2.5 sigma (unshifted), skin 0.3 sigma, eps = 1 eV, sigma =
+    1 A, m = 1 amu, T* from the lcg stream (`set tempgdist`), velocity Verlet
+    dt = 0.005 tau; mixed and FP64, each with its autotuned brick shape."""
     a = (4.0 / 0.8442) ** (1.0 / 3.0)
-    x = fcc(nx, a) + np.random.default_rng(7).normal(0.0, 0.01, (4 * nx ** 3, 3))
+    x = fcc(nx, a)
+    if jitter > 0:
+        x = x + np.random.default_rng(7).normal(0.0, jitter, x.shape)
     L = nx * a
     x = np.mod(x, L)
-    tau = 1e-10 * np.sqrt(1.66053906660e-27 / 1.602176634e-19) * 1e12  # ps
-    T = 1.44 * 1.602176634e-19 / 1.380649e-23
+    tau = 0.01018047  # ps: sqrt(m sigma^2 / eps) in GEMS units
+    T = Tstar / 8.617330337217213e-05  # K (kB in eV/K)
     out = {"workload": "LJ fcc %d^3 cells = %d atoms, rho* 0.8442, rc 2.5 sigma, skin 0.3 sigma, "
-                       "T* 1.44, VV dt 0.005 tau, periodic" % (nx, len(x))}
+                       "T* %.2f, VV dt 0.005 tau, %d steps, periodic" % (nx, len(x), Tstar, steps)}
     F = {}
     for name, prec in (("mixed", hip.GH_MIXED), ("fp64", hip.GH_FP64)):
         c = hip.Context(0, prec)
@@ -205,10 +208,27 @@ def lj_bulk(hip, nx=64, steps=100, warmup=10):
         dt, _, nupd = timed_run(c, steps, warmup)
         c.close()
         out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
-                     "steps": steps, "rebuilds": nupd, "brick_cells": shape}
-    rms = float(np.sqrt(np.mean(F["fp64"] ** 2)))
-    out["force_err_mixed_vs_fp64_rms_over_rms"] = float(
-        np.sqrt(np.mean((F["mixed"] - F["fp64"]) ** 2)) / rms)
+                     "rebuilds": nupd, "brick_cells": shape}
+    if jitter > 0:  # on the perfect lattice the forces vanish and the ratio means nothing
+        rms = float(np.sqrt(np.mean(F["fp64"] ** 2)))
+        out["force_err_mixed_vs_fp64_rms_over_rms"] = float(
+            np.sqrt(np.mean((F["mixed"] - F["fp64"]) ** 2)) / rms)
+    return out
+
+
+def au_32k(hip, steps=2000, warmup=20):
+    """The Au 32k configuration: fcc 20^3 cells (32,000 atoms), 0.02 A jitter,
+    300 K from the lcg stream, VV dt 2 fs, skin 1 A; mixed and FP64."""
+    x, L = workload(20)
+    out = {"workload": "Au SMA-TB fcc 20^3 cells = 32,000 atoms, 0.02 A jitter, 300 K, "
+                       "VV dt 2 fs, skin 1 A, %d steps, periodic" % steps}
+    for name, prec in (("mixed", hip.GH_MIXED), ("fp64", hip.GH_FP64)):
+        c = make_ctx(hip, prec, x, L, 1.0)
+        shape, _ = c.autotune(reps=3)
+        dt, _, nupd = timed_run(c, steps, warmup)
+        c.close()
+        out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "rebuilds": nupd, "brick_cells": shape}
     return out
 
 
@@ -422,7 +442,9 @@ def main():
             "vs_reference_32k": drift_vs_reference(hip),
         }
     if not args.no_batch:
-        out["lj_bulk"] = lj_bulk(hip)
+        out["lj_4000"] = lj_bulk(hip, 10, 1000)
+        out["lj_1m"] = lj_bulk(hip, 64, 100, jitter=0.01, Tstar=1.44)
+        out["au_32k"] = au_32k(hip)
         out["batched_re"] = batch_re(hip, args.re_rounds)
         out["batched_fire"] = batch_fire(hip, args.fire_clusters)
     if not args.no_cpu:
