@@ -5,6 +5,7 @@
 !
 ! Entry points and the GEMS routines they replace:
 !   gh_compute          interact()            src/Interaction.f90:135-241
+!   gh_set_positions    ig%at(i)%o%pos        src/Neighbour.F90:114
 !   gh_set_smatb        smatb_set/readprm     src/TB.F90:106-170, 291-365
 !   gh_set_lj           lj_set                src/Pairs.F90:198-205
 !   gh_set_skin         nb_dcut               src/Neighbour.F90:237
@@ -63,6 +64,12 @@ module gems_hip
       integer(c_int), value :: n
       real(c_double), intent(in) :: x(3, n)
       integer(c_int), intent(in) :: type(n)
+    end function
+
+    integer(c_int) function gh_set_positions(ctx, x) bind(C, name='gh_set_positions')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: x(*)
     end function
 
     integer(c_int) function gh_set_masses(ctx, nt, mass) bind(C, name='gh_set_masses')

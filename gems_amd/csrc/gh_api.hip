@@ -504,7 +504,7 @@ void gh_destroy(gh_ctx *c) {
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
-                 c->red, c->list16};
+                 c->red, c->list16, c->xin};
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->gauss_h) (void)hipHostFree(c->gauss_h);
@@ -656,6 +656,10 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
       GH_HIP(c, hipFree(c->gauss));
       c->gauss = nullptr;
     }
+    if (c->xin) {
+      GH_HIP(c, hipFree(c->xin));
+      c->xin = nullptr;
+    }
     if (c->gauss_h) {
       GH_HIP(c, hipHostFree(c->gauss_h));
       c->gauss_h = nullptr;
@@ -745,6 +749,16 @@ static int put_soa3(gh_ctx *c, double *d, const std::vector<int> &orig, const do
                            c->stream));
   GH_HIP(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int gh_set_positions(gh_ctx *c, const double *x) {
+  if (!c || !x) return gh_fail(c, GH_EINVAL, "gh_set_positions: bad arguments");
+  if (!c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms: call gh_set_atoms first");
+  GH_HIP(c, hipSetDevice(c->device));
+  if (!c->xin) GH_TRY(dalloc(c, &c->xin, 3 * (size_t)c->ncap));
+  GH_HIP(c, hipMemcpyAsync(c->xin, x, sizeof(double) * 3 * (size_t)c->n, hipMemcpyHostToDevice,
+                           c->stream));
+  return gh_launch_set_positions(c, c->xin);
 }
 
 int gh_set_velocities(gh_ctx *c, const double *v) {

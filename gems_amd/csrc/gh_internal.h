@@ -170,6 +170,7 @@ struct gh_ctx {
   int *orig = nullptr, *orig_t = nullptr;     // caller index of sorted atom
   double *xold = nullptr;                     // [3][ncap] positions at build
   double *posv = nullptr;                     // Ermak random displacement [3][ncap]
+  double *xin = nullptr;                      // gh_set_positions upload [3n] (caller order)
   double *velv = nullptr;                     // Ermak random velocity [3][ncap]
   double *gauss = nullptr;                    // host-stream gaussians [6][ncap] (caller order)
   double *gauss_h = nullptr;                  // pinned host staging
@@ -229,6 +230,7 @@ inline unsigned long long *gh_maxd2(gh_ctx *c) { return &c->status->maxdisp2_bit
 
 // launchers (gh_kernels.hip)
 int gh_launch_rebuild(gh_ctx *c, bool force);
+int gh_launch_set_positions(gh_ctx *c, const double *x_dev);
 int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv = false);
 bool gh_can_fuse_vv(const gh_ctx *c);
 int gh_launch_vv_a(gh_ctx *c);

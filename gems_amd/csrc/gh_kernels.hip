@@ -701,6 +701,35 @@ __global__ void k_wrap(int n, WrapArgs wa, double4 *__restrict__ pos) {
   pos[i] = p;
 }
 
+// New positions from the host, in caller order (x: 3n, AoS): each atom's
+// coordinate moves to the periodic image nearest its current one (a host-side
+// wrap is no displacement), and its displacement since the last build joins
+// the rebuild test.
+__global__ __launch_bounds__(GH_BLOCK) void k_set_positions(
+    int n, int ncap, WrapArgs wa, const int *__restrict__ orig, const double *__restrict__ x,
+    double4 *__restrict__ pos, const double *__restrict__ xold,
+    unsigned long long *__restrict__ maxd2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (s < n) {
+    const int o = orig[s];
+    double4 p = pos[s];
+    double v[3] = {x[3 * (size_t)o], x[3 * (size_t)o + 1], x[3 * (size_t)o + 2]};
+    const double old[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (wa.pbc[d]) v[d] += wa.L[d] * rint((old[d] - v[d]) / wa.L[d]);
+      const double dd = v[d] - xold[d * ncap + s];
+      d2 += dd * dd;
+    }
+    p.x = v[0];
+    p.y = v[1];
+    p.z = v[2];
+    pos[s] = p;
+  }
+  disp_reduce(d2, maxd2);
+}
+
 __global__ void k_wrap_rebuild(const GhStatus *st, int n, WrapArgs wa,
                                double4 *__restrict__ pos) {
   if (!st->rebuild) return;
@@ -937,6 +966,13 @@ int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui) {
   GH_HIP(c, hipStreamSynchronize(s));
   *epot_ui = c->red_h[0];
   *ekin_ui = c->red_h[8] * 0.5;
+  return 0;
+}
+
+int gh_launch_set_positions(gh_ctx *c, const double *x_dev) {
+  k_set_positions<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
+      c->n, c->ncap, make_wrap(c), c->orig, x_dev, c->pos, c->xold, gh_maxd2(c));
+  GH_HIP(c, hipGetLastError());
   return 0;
 }
 

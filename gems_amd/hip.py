@@ -19,7 +19,8 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 # every symbol declared in include/gems_hip.h
 SYMBOLS = [
     "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box",
-    "gh_set_atoms", "gh_set_masses", "gh_set_velocities", "gh_set_smatb", "gh_set_lj",
+    "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
+    "gh_set_lj",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
     "gh_get_stats", "gh_set_timing", "gh_get_timing", "gh_set_tile", "gh_autotune",
@@ -66,6 +67,7 @@ def load():
     L.gh_set_atoms.argtypes = [vp, C.c_int, _dp, _ip]
     L.gh_set_masses.argtypes = [vp, C.c_int, _dp]
     L.gh_set_velocities.argtypes = [vp, _dp]
+    L.gh_set_positions.argtypes = [vp, _dp]
     L.gh_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
     L.gh_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.gh_set_skin.argtypes = [vp, C.c_double]
@@ -168,6 +170,12 @@ class Context:
     def set_velocities(self, v=None):
         vv = None if v is None else np.ascontiguousarray(v, np.float64)
         self._ck(self.L.gh_set_velocities(self.h, _p(vv)))
+
+    def set_positions(self, x):
+        """New positions (n x 3, caller order) for the next compute; lists kept."""
+        xx = np.ascontiguousarray(x, np.float64)
+        assert xx.shape == (self.n, 3)
+        self._ck(self.L.gh_set_positions(self.h, _p(xx)))
 
     def set_smatb(self, nt, A, xi, p, q, r0, rci, rce):
         arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (nt, nt)))

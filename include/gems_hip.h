@@ -106,6 +106,15 @@ int gh_set_tile(gh_ctx *ctx, const int bd[3]);
  * counterpart in the reference. */
 int gh_autotune(gh_ctx *ctx, int reps, int bd_out[3], double *us_out);
 
+/* New positions (3n, A, caller order) for the next gh_compute when the
+ * host integrates: what an interaction plugin reads from ig%at(i)%o%pos
+ * (Neighbour.F90:114) at each interact() (Interaction.f90:135-241).  The
+ * neighbour lists and cell order are kept; each coordinate is taken at the
+ * periodic image nearest the atom's current one (so the host's do_pbc wrap,
+ * Set_Properties.f90:444-470, is no displacement) and the skin test of the
+ * next gh_compute sees the displacement since the last build. */
+int gh_set_positions(gh_ctx *ctx, const double *x);
+
 /* One force evaluation == interact() (Interaction.f90:135-241): skin test
  * and neighbour rebuild (test_update/update, Neighbour.F90:1242-1346), then
  * smatb_interact (TB.F90:457-586) or lj_interact (Pairs.F90:207-279).

@@ -340,3 +340,39 @@ def test_overflow_and_errors_are_reported():
     with pytest.raises(hip.GhError):
         c.set_skin(-1.0)
     c.close()
+
+
+@pytest.mark.parametrize("name", ["au864_pbc", "agcu201_free", "lj4000_pbc"])
+@pytest.mark.parametrize("prec", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_host_positions_keep_the_lists(name, prec):
+    """gh_set_positions (the host integrates, the plugin computes forces):
+    small moves keep the list, a host-side wrap is no displacement, a move
+    past half the skin rebuilds; forces match the oracle each time."""
+    f, c, s, box, skin = _setup(name, precision=prec)
+    lj = name.startswith("lj")
+    tol = 1e-10 if prec == hip.GH_FP64 else 1e-5
+    c.compute()
+    nupd0 = c.stats()["nupd"]
+    rng = np.random.default_rng(11)
+    x = f["x"] + rng.normal(0.0, 0.05 * skin, f["x"].shape)  # max |dx| stays below skin/2
+    g = dict(f)
+    g["x"] = x
+    for step in range(3):
+        xs = x.copy()
+        if step == 1 and "L" in f and np.any(f["pbc"]):  # the host wraps some atoms
+            L = np.asarray(f["L"], float)
+            xs[::7] = np.where(np.asarray(f["pbc"], bool), xs[::7] + L, xs[::7])
+        c.set_positions(xs)
+        F, e, et = c.compute()
+        Fo, eo, epo, _, _ = _oracle_forces(g, s, box, skin, lj=lj)
+        assert np.sqrt(np.mean((F - Fo) ** 2)) / np.sqrt(np.mean(Fo ** 2)) <= tol, step
+        assert c.stats()["nupd"] == nupd0, step  # no rebuild: |dx| < skin/2
+    x = x.copy()
+    x[0] += 0.6 * skin  # past half the skin: the next force call rebuilds
+    g["x"] = x
+    c.set_positions(x)
+    F, e, et = c.compute()
+    Fo, eo, epo, _, _ = _oracle_forces(g, s, box, skin, lj=lj)
+    assert np.sqrt(np.mean((F - Fo) ** 2)) / np.sqrt(np.mean(Fo ** 2)) <= tol
+    assert c.stats()["nupd"] == nupd0 + 1
+    c.close()
