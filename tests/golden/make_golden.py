@@ -151,29 +151,38 @@ def forces_fixture(name, syms, x, L, pbc, pot_block, types, mass=None, extra="")
         shutil.rmtree(wd)
 
 
+def md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each, mass=None,
+              dt=None):
+    """Writes the inputs of an MD fixture into wd and returns the GEMS script:
+    `dinamica nsteps` with the energy columns every `each` steps (E.dat) and
+    the final positions (X.xyz)."""
+    write_xyz(os.path.join(wd, "x.xyz"), syms, x)
+    pot_block = pot_lines(wd, pot_block)
+    s = head(seed)
+    s += "out float 25.16\n"
+    s += ">< read x.xyz\nsys add\n"
+    if L is not None:
+        s += "box size %s %s %s\n" % tuple(fmt(v) for v in L)
+        s += "> sys\nset pbc %s %s %s\n" % tuple("T" if p else "F" for p in pbc)
+    s += "> sys\n"
+    if mass is not None:
+        s += "set mass %s\n" % fmt(mass)
+    s += "group 1 add\n" + pot_block
+    if dt is not None:
+        s += "time step %s\n" % fmt(dt)
+    s += setup
+    s += ("outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each %d\n"
+          "outfile :x name X.xyz\noutfile :x pos 1\noutfile :x each %d\n" % (each, nsteps))
+    s += "> sys\n" + evolve + "dinamica %d\n" % nsteps
+    return s
+
+
 def md_fixture(name, syms, x, L, pbc, pot_block, types, seed, setup, evolve, nsteps, each,
                mass=None, dt=None):
     """Reference energy trace of `dinamica nsteps` (16 digits) + final state."""
     wd = tempfile.mkdtemp(prefix="gold_")
     try:
-        write_xyz(os.path.join(wd, "x.xyz"), syms, x)
-        pot_block = pot_lines(wd, pot_block)
-        s = head(seed)
-        s += "out float 25.16\n"
-        s += ">< read x.xyz\nsys add\n"
-        if L is not None:
-            s += "box size %s %s %s\n" % tuple(fmt(v) for v in L)
-            s += "> sys\nset pbc %s %s %s\n" % tuple("T" if p else "F" for p in pbc)
-        s += "> sys\n"
-        if mass is not None:
-            s += "set mass %s\n" % fmt(mass)
-        s += "group 1 add\n" + pot_block
-        if dt is not None:
-            s += "time step %s\n" % fmt(dt)
-        s += setup
-        s += ("outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each %d\n"
-              "outfile :x name X.xyz\noutfile :x pos 1\noutfile :x each %d\n" % (each, nsteps))
-        s += "> sys\n" + evolve + "dinamica %d\n" % nsteps
+        s = md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each, mass, dt)
         log, nupd = run(wd, s)
         E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3)
         X = read_frames(os.path.join(wd, "X.xyz"))[-1]
