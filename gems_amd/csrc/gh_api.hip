@@ -372,7 +372,7 @@ static int ensure_ready(gh_ctx *c) {
       GH_HIP(c, hipFree(c->list));
       c->list = nullptr;
     }
-    GH_TRY(dalloc(c, &c->list16, (size_t)cap * c->ncap));
+    GH_TRY(dalloc(c, &c->list16, (size_t)(cap + 4 * GH_LIST_PAD_QUADS) * c->ncap));
   } else {
     if (c->list16) {
       GH_HIP(c, hipFree(c->list16));

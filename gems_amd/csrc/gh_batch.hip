@@ -420,16 +420,28 @@ __global__ __launch_bounds__(BS) void k_batch_langevin(BatchArgs B, int nsteps,
                                                               long long step0, ErmakC k,
                                                               unsigned long long seed,
                                                               double skin) {
+  // The step constants live in LDS and are read where they are used: held in
+  // scalar registers for the whole kernel they were spilled, and the ROCm 7.2
+  // compiler has mis-allocated such spills here (a later kernel-argument load
+  // overwrote the registers before they were saved).
+  __shared__ ErmakC kc;
+  __shared__ unsigned long long kseed;
+  __shared__ double kskin;
+  if (threadIdx.x == 0) {
+    kc = k;
+    kseed = seed;
+    kskin = skin;
+  }
   SYS_SETUP(SINGLE)
   const int a0 = S.a0, a1 = S.a0 + S.n;
-  const double skt = sqrt(k.kB_ui * B.T[s]);
+  const double skt = sqrt(kc.kB_ui * B.T[s]);
   int nreb = 0;
   for (int st = 0; st < nsteps; ++st) {
     const long long step = step0 + st;
     double d2 = 0.0;
     for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
       double g[6];
-      philox_normals(seed, step, i, g);
+      philox_normals(kseed, step, i, g);
 #pragma unroll
       for (int q = 0; q < 6; ++q) B.gn[q * B.ld + i] = (float)g[q];  // exact: FP32 normals
       const double f_lan = skt * B.osm[B.type[i]];
@@ -437,8 +449,8 @@ __global__ __launch_bounds__(BS) void k_batch_langevin(BatchArgs B, int nsteps,
       double x[3] = {p.x, p.y, p.z};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const double pos_v = f_lan * k.sdr * g[2 * d];
-        x[d] = x[d] + k.c1 * B.vel[d * B.ld + i] + k.c2 * k.dt * B.acc[d * B.ld + i] + pos_v;
+        const double pos_v = f_lan * kc.sdr * g[2 * d];
+        x[d] = x[d] + kc.c1 * B.vel[d * B.ld + i] + kc.c2 * kc.dt * B.acc[d * B.ld + i] + pos_v;
         const double dd = x[d] - B.xold[d * B.ld + i];
         d2 += dd * dd;
       }
@@ -449,7 +461,7 @@ __global__ __launch_bounds__(BS) void k_batch_langevin(BatchArgs B, int nsteps,
     }
     const double dmax2 = bmax(R, d2);
     if (MIXED) sys_stage(B, S, &sflag, !SINGLE);
-    if (2.0 * sqrt(dmax2) > skin) {
+    if (2.0 * sqrt(dmax2) > kskin) {
       if (!MIXED) sys_stage(B, S, &sflag, !SINGLE);
       sys_build(B, S);
       ++nreb;
@@ -464,10 +476,10 @@ __global__ __launch_bounds__(BS) void k_batch_langevin(BatchArgs B, int nsteps,
       const double om = B.om[t];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const double vel_v = f_lan * k.sdv * (k.crv1 * g[2 * d] + k.crv2 * g[2 * d + 1]);
+        const double vel_v = f_lan * kc.sdv * (kc.crv1 * g[2 * d] + kc.crv2 * g[2 * d + 1]);
         const double f = B.frc[d * B.ld + i];
-        B.vel[d * B.ld + i] = k.c0 * B.vel[d * B.ld + i] + (k.c1 - k.c2) * B.acc[d * B.ld + i] +
-                              k.c2 * f * om + vel_v;
+        B.vel[d * B.ld + i] = kc.c0 * B.vel[d * B.ld + i] + (kc.c1 - kc.c2) * B.acc[d * B.ld + i] +
+                              kc.c2 * f * om + vel_v;
         B.acc[d * B.ld + i] = f * om;
       }
     }
@@ -513,7 +525,14 @@ __device__ double sys_fmax(const BatchArgs &B, BlockRed<BS> &R, int a0, int a1) 
 // v and resets a.  Mixed precision switches to FP64 pair terms once
 // max|F| < fswitch; convergence is max|F| < ftol in FP64.
 template <bool SINGLE, int BS>
-__global__ __launch_bounds__(BS) void k_batch_fire(BatchArgs B, FireC fc, double skin) {
+__global__ __launch_bounds__(BS) void k_batch_fire(BatchArgs B, FireC fc_arg, double skin_arg) {
+  // constants in LDS, read where used (see k_batch_langevin)
+  __shared__ FireC fc;
+  __shared__ double skin;
+  if (threadIdx.x == 0) {
+    fc = fc_arg;
+    skin = skin_arg;
+  }
   SYS_SETUP(SINGLE)
   const int a0 = S.a0, a1 = S.a0 + S.n;
   bool mixed = fc.mixed != 0;
@@ -731,7 +750,7 @@ static int batch_ready(gh_batch *b) {
     int est = (int)(4.18879 * rl * rl * rl * dens) + 16;
     b->cap = std::max(8, std::min(nmax, est));
     b->cap = (b->cap + 7) / 8 * 8;
-    BT(balloc(b, &b->list, (size_t)b->cap * b->ld));
+    BT(balloc(b, &b->list, (size_t)(b->cap + 4 * GH_LIST_PAD_QUADS) * b->ld));
     // LDS staging of the largest system (int4 + type byte per atom, and the
     // sentinel) beside the static tables; larger systems use global memory
     const size_t stat = (b->nt > 1 ? GH_MAXT * GH_MAXT : 1) * (sizeof(TbDev) + sizeof(TbF)) +
@@ -788,7 +807,8 @@ int gh_batch_create(gh_batch **out, int device, int precision) {
       hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void **)&b->flags, sizeof(int) * 4) != hipSuccess ||
-      hipMemset(b->flags, 0, sizeof(int) * 4) != hipSuccess) {
+      hipMemsetAsync(b->flags, 0, sizeof(int) * 4, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess) {
     delete b;
     return GH_EHIP;
   }
@@ -876,15 +896,20 @@ int gh_batch_set_systems(gh_batch *b, int nsys, const int *off, const double *x,
     ht[i] = type ? type[i] : 0;
     if (ht[i] < 0 || ht[i] >= GH_MAXT) return bfail(b, GH_EINVAL, "type out of range");
   }
-  BH(b, hipMemcpy(b->pos, hp.data(), sizeof(double4) * ld, hipMemcpyHostToDevice));
-  BH(b, hipMemcpy(b->type, ht.data(), sizeof(int) * ld, hipMemcpyHostToDevice));
-  BH(b, hipMemcpy(b->off, off, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice));
-  BH(b, hipMemcpy(b->order, b->order_h.data(), sizeof(int) * nsys, hipMemcpyHostToDevice));
-  BH(b, hipMemset(b->vel, 0, sizeof(double) * 3 * ld));
-  BH(b, hipMemset(b->acc, 0, sizeof(double) * 3 * ld));
-  BH(b, hipMemset(b->frc, 0, sizeof(double) * 3 * ld));
-  BH(b, hipMemset(b->out, 0, sizeof(double) * nsys * OUTW));
-  BH(b, hipMemset(b->T, 0, sizeof(double) * nsys));
+  // every upload and clear is ordered on the batch stream (a non-blocking
+  // stream does not wait for work on the null stream)
+  hipStream_t st = b->stream;
+  BH(b, hipMemcpyAsync(b->pos, hp.data(), sizeof(double4) * ld, hipMemcpyHostToDevice, st));
+  BH(b, hipMemcpyAsync(b->type, ht.data(), sizeof(int) * ld, hipMemcpyHostToDevice, st));
+  BH(b, hipMemcpyAsync(b->off, off, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
+  BH(b, hipMemcpyAsync(b->order, b->order_h.data(), sizeof(int) * nsys, hipMemcpyHostToDevice,
+                       st));
+  BH(b, hipMemsetAsync(b->vel, 0, sizeof(double) * 3 * ld, st));
+  BH(b, hipMemsetAsync(b->acc, 0, sizeof(double) * 3 * ld, st));
+  BH(b, hipMemsetAsync(b->frc, 0, sizeof(double) * 3 * ld, st));
+  BH(b, hipMemsetAsync(b->out, 0, sizeof(double) * nsys * OUTW, st));
+  BH(b, hipMemsetAsync(b->T, 0, sizeof(double) * nsys, st));
+  BH(b, hipStreamSynchronize(st));
   b->lists_valid = false;
   b->forces_valid = false;
   return 0;
@@ -937,14 +962,17 @@ int gh_batch_set_velocities(gh_batch *b, const double *v) {
   if (v)
     for (int i = 0; i < b->ntot; ++i)
       for (int d = 0; d < 3; ++d) h[(size_t)d * b->ld + i] = v[3 * (size_t)i + d];
-  BH(b, hipMemcpy(b->vel, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  BH(b, hipMemcpyAsync(b->vel, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                       b->stream));
+  BH(b, hipStreamSynchronize(b->stream));
   return 0;
 }
 
 int gh_batch_set_temperatures(gh_batch *b, const double *T) {
   if (!b || !b->nsys || !T) return bfail(b, GH_ESTATE, "no systems");
   b->T_h.assign(T, T + b->nsys);
-  BH(b, hipMemcpy(b->T, T, sizeof(double) * b->nsys, hipMemcpyHostToDevice));
+  BH(b, hipMemcpyAsync(b->T, T, sizeof(double) * b->nsys, hipMemcpyHostToDevice, b->stream));
+  BH(b, hipStreamSynchronize(b->stream));
   return 0;
 }
 
@@ -989,7 +1017,9 @@ int gh_batch_compute(gh_batch *b, double *F, double *epot) {
     for (int s = 0; s < b->nsys; ++s) epot[s] = b->epot_h[s];
   if (F) {
     std::vector<double> h(3 * (size_t)b->ld);
-    BH(b, hipMemcpy(h.data(), b->frc, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    BH(b, hipMemcpyAsync(h.data(), b->frc, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                         b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
     for (int i = 0; i < b->ntot; ++i)
       for (int d = 0; d < 3; ++d) F[3 * (size_t)i + d] = h[(size_t)d * b->ld + i];
   }
@@ -1162,7 +1192,9 @@ int gh_batch_get_state(gh_batch *b, double *x, double *v) {
   if (!b || !b->nsys) return bfail(b, GH_ESTATE, "no systems");
   if (x) {
     std::vector<double4> h(b->ld);
-    BH(b, hipMemcpy(h.data(), b->pos, sizeof(double4) * b->ld, hipMemcpyDeviceToHost));
+    BH(b, hipMemcpyAsync(h.data(), b->pos, sizeof(double4) * b->ld, hipMemcpyDeviceToHost,
+                         b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
     for (int i = 0; i < b->ntot; ++i) {
       x[3 * (size_t)i] = h[i].x;
       x[3 * (size_t)i + 1] = h[i].y;
@@ -1171,7 +1203,9 @@ int gh_batch_get_state(gh_batch *b, double *x, double *v) {
   }
   if (v) {
     std::vector<double> h(3 * (size_t)b->ld);
-    BH(b, hipMemcpy(h.data(), b->vel, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    BH(b, hipMemcpyAsync(h.data(), b->vel, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                         b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
     for (int i = 0; i < b->ntot; ++i)
       for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = h[(size_t)d * b->ld + i];
   }

@@ -166,8 +166,11 @@ struct RowWriter {  // quad-packed row of one atom
 // Runs f(slot) over an atom's list, one 8-byte load per four entries; the
 // row's last quad is padded with the sentinel slot, so every entry of every
 // quad is processed without a bounds test.  The loads run two quads ahead of
-// use and are unconditional (the index is clamped to the row's last quad), so
-// no wait on the list stream sits between a quad's load and its use.
+// use, unconditionally and through a running pointer: the list allocation
+// holds GH_LIST_PAD_QUADS rows past its capacity, so the last prefetches
+// stay in bounds (their values are never used).  No wait on the list stream
+// sits between a quad's load and its use, and no 64-bit multiply or clamp
+// sits in the loop.
 template <class F>
 __device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__ list16, int ld,
                                               int a, int m, F &&f) {
@@ -176,10 +179,12 @@ __device__ __forceinline__ void for_each_slot(const unsigned short *__restrict__
   const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
   const size_t sld = (size_t)ld;
   unsigned long long w0 = ld_quad(lp);
-  unsigned long long w1 = ld_quad(lp + (size_t)min(1, ng - 1) * sld);
+  unsigned long long w1 = ld_quad(lp + sld);
+  const unsigned long long *pn = lp + 2 * sld;
   for (int g = 0; g < ng; g += 2) {
-    const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
-    const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
+    const unsigned long long n0 = ld_quad(pn);
+    const unsigned long long n1 = ld_quad(pn + sld);
+    pn += 2 * sld;
 #pragma unroll
     for (int u = 0; u < 4; ++u) f((int)((w0 >> (16 * u)) & 0xFFFFull));
     if (g + 1 < ng) {
@@ -202,10 +207,12 @@ __device__ __forceinline__ void for_each_quad(const unsigned short *__restrict__
   const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
   const size_t sld = (size_t)ld;
   unsigned long long w0 = ld_quad(lp);
-  unsigned long long w1 = ld_quad(lp + (size_t)min(1, ng - 1) * sld);
+  unsigned long long w1 = ld_quad(lp + sld);
+  const unsigned long long *pn = lp + 2 * sld;
   for (int g = 0; g < ng; g += 2) {
-    const unsigned long long n0 = ld_quad(lp + (size_t)min(g + 2, ng - 1) * sld);
-    const unsigned long long n1 = ld_quad(lp + (size_t)min(g + 3, ng - 1) * sld);
+    const unsigned long long n0 = ld_quad(pn);
+    const unsigned long long n1 = ld_quad(pn + sld);
+    pn += 2 * sld;
     f(w0);
     if (g + 1 < ng) f(w1);
     w0 = n0;

@@ -96,6 +96,9 @@ struct TileGeom {
 // Dynamic LDS of the list build: two workgroups (with their static Halo) fit
 // one CU's 160 KB.
 #define GH_BUILD_LDS 77824
+// Quad rows every neighbour-list allocation carries past its capacity: the
+// list traversals prefetch two quads ahead without a bounds clamp.
+#define GH_LIST_PAD_QUADS 2
 
 struct GhStatus {                     // device-resident counters
   unsigned long long maxdisp2_bits;   // max |dx|^2 since the last build
