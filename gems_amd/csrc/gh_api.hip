@@ -504,9 +504,10 @@ void gh_destroy(gh_ctx *c) {
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
-                 c->red, c->list16, c->xin};
+                 c->red, c->list16, c->xin, c->chain_dev};
   for (void *p : dev)
     if (p) (void)hipFree(p);
+  if (c->chain_pin) (void)hipHostFree(c->chain_pin);
   if (c->gauss_h) (void)hipHostFree(c->gauss_h);
   if (c->red_h) (void)hipHostFree(c->red_h);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);

@@ -111,7 +111,7 @@ struct GhStatus {                     // device-resident counters
   int tile_overflow;                  // a brick's halo exceeded slot_cap (sticky)
   int max_halo;                       // largest halo (atoms) at the last build
   int max_brick;                      // largest brick (atoms) at the last build
-  int pad;
+  int ticket;                         // binning workgroups done (last one scans)
 };
 
 struct gh_ctx {
@@ -184,6 +184,9 @@ struct gh_ctx {
   int *nn = nullptr;                          // [ncap]
   int *list = nullptr;                        // [cap][ncap] ELL (O(N^2) mode)
   unsigned short *list16 = nullptr;           // [cap][ncap] ELL of LDS slots (cell mode)
+  void *chain_dev = nullptr;                  // rebuild-chain arguments (device copy)
+  void *chain_pin = nullptr;                  // their pinned staging copy
+  std::vector<unsigned char> chain_last;      // bytes last uploaded
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
   TbF *tbf = nullptr;                         // [MAXT*MAXT]
   TbF tbf0;                                   // host copy of tbf[0] (one-type kernel argument)

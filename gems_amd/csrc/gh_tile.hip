@@ -22,6 +22,7 @@
 // exact in FP64 for every pair within 7 A, so the only rounding of a pair's
 // geometry is the 2^-24 quantisation of each position (6e-8 A).
 #include <math.h>
+#include <cstring>
 
 #include "gh_device.h"
 
@@ -465,11 +466,11 @@ __device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, in
 // 444-470), copies the atom's state to the _t arrays (the gather reads it
 // there and writes the new order in place), computes the atom's key and
 // counts atoms per key.
-__device__ void cell_bin(const CellArgs &R, int i0, int stride) {
+__device__ void cell_bin(const CellArgs &R, const double4 *__restrict__ pos, int i0, int stride) {
   const TileGeom &t = R.t;
   const int ncap = R.ncap;
   for (int i = i0; i < R.n; i += stride) {
-    double4 p = R.pos[i];
+    double4 p = pos[i];
     double x[3] = {p.x, p.y, p.z};
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -499,12 +500,11 @@ __device__ void cell_bin(const CellArgs &R, int i0, int stride) {
   }
 }
 
-#define GH_SCANBLOCK 1024  // workgroup of the cell-count scan
-
-// Workgroup (GH_SCANBLOCK threads) exclusive scan of one value per thread in
-// thread order; returns this thread's offset and the workgroup total.
+// Workgroup (BS threads) exclusive scan of one value per thread in thread
+// order; returns this thread's offset and the workgroup total.
+template <int BS>
 __device__ __forceinline__ int block_scan(int v, int &total) {
-  __shared__ int wsum[GH_SCANBLOCK / 64];
+  __shared__ int wsum[BS / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int inc = v;
   for (int o = 1; o < 64; o <<= 1) {
@@ -515,7 +515,7 @@ __device__ __forceinline__ int block_scan(int v, int &total) {
   __syncthreads();
   int woff = 0;
   total = 0;
-  for (int w = 0; w < GH_SCANBLOCK / 64; ++w) {
+  for (int w = 0; w < BS / 64; ++w) {
     const int x = wsum[w];
     if (w < wv) woff += x;
     total += x;
@@ -524,14 +524,16 @@ __device__ __forceinline__ int block_scan(int v, int &total) {
   return woff + inc - v;
 }
 
-// Exclusive scan of the cell counts by one workgroup.  Each pass gives every
-// thread GH_SCANRUN consecutive keys, loaded and stored 16 B at a time, so a
-// pass costs one memory round trip for the loads and one for the stores; a
-// carry links the passes.  start = the prefix; start[nkey] = the total.
+// Exclusive scan of the cell counts by one workgroup of BS threads.  Each
+// pass gives every thread GH_SCANRUN consecutive keys, loaded and stored 16 B
+// at a time, so a pass costs one memory round trip for the loads and one for
+// the stores; a carry links the passes.  start = the prefix; start[nkey] =
+// the total.
 #define GH_SCANRUN 32
+template <int BS>
 __device__ void cell_scan(const CellArgs &R) {
   int carry = 0;
-  for (int base = 0; base < R.nkey; base += GH_SCANRUN * GH_SCANBLOCK) {
+  for (int base = 0; base < R.nkey; base += GH_SCANRUN * BS) {
     const int k0 = base + GH_SCANRUN * threadIdx.x;
     int v[GH_SCANRUN];
 #pragma unroll
@@ -552,7 +554,7 @@ __device__ void cell_scan(const CellArgs &R) {
 #pragma unroll
     for (int u = 0; u < GH_SCANRUN; ++u) tot += v[u];
     int total;
-    int run = carry + block_scan(tot, total);
+    int run = carry + block_scan<BS>(tot, total);
 #pragma unroll
     for (int u = 0; u < GH_SCANRUN; u += 4) {
       const int k = k0 + u;
@@ -590,7 +592,8 @@ __device__ void cell_scatter(const CellArgs &R, int i0, int stride) {
 // of the atomics' order), and its state moves from the _t copies to that
 // slot.  Every thread reads only the scatter's output and the copies, so
 // no ordering between threads is needed.
-__device__ void cell_rank_gather(const CellArgs &R, int s0, int stride) {
+__device__ void cell_rank_gather(const CellArgs &R, double4 *__restrict__ pos, int s0,
+                                 int stride) {
   const int ncap = R.ncap;
   for (int s = s0; s < R.n; s += stride) {
     const int v = R.perm[s];
@@ -600,7 +603,7 @@ __device__ void cell_rank_gather(const CellArgs &R, int s0, int stride) {
     int rank = 0;
     for (int k = b; k < e; ++k) rank += R.orig_t[R.perm[k]] < key ? 1 : 0;
     const int d = b + rank;
-    R.pos[d] = R.pos_t[v];
+    pos[d] = R.pos_t[v];
 #pragma unroll
     for (int dd = 0; dd < 3; ++dd) {
       R.vel[dd * ncap + d] = R.vel_t[dd * ncap + v];
@@ -612,30 +615,53 @@ __device__ void cell_rank_gather(const CellArgs &R, int s0, int stride) {
   }
 }
 
-// The binning phases, one launch each; every launch returns at once when no
-// rebuild is due.  k_cell_bin decides (each workgroup evaluates the same
-// test on the unchanged status; workgroup 0 records it), k_cell_scan books
-// the decision: the later kernels read st->rebuild.
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_bin(CellArgs R) {
-  const bool due = rebuild_due(R.st, R.skin);
-  if (blockIdx.x == 0 && threadIdx.x == 0) R.st->rebuild = due ? 1 : 0;
+// Arguments of the rebuild chain, resident in device memory: the kernels take
+// a pointer (small kernel arguments launch faster, and most launches only
+// read the rebuild flag and return).  The positions buffer alternates from
+// step to step, so it comes as a kernel argument (R.pos is unused).
+struct ChainArgs {
+  CellArgs R;
+  BuildArgs A;
+};
+
+// The binning phases; every launch returns at once when no rebuild is due.
+// k_cell_bin decides (each workgroup evaluates the same test on the
+// unchanged status; workgroup 0 records it), bins, and its last workgroup to
+// finish scans the counts; k_cell_scatter books the decision (every kernel
+// that tests rebuild_due has finished by then) and scatters; the later
+// kernels read st->rebuild.
+#define GH_BINBLOCK 1024  // binning workgroup: its last workgroup scans the counts
+__global__ __launch_bounds__(GH_BINBLOCK) void k_cell_bin(GhStatus *st,
+                                                        const ChainArgs *__restrict__ C,
+                                                        double4 *pos, double skin) {
+  const bool due = rebuild_due(st, skin);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rebuild = due ? 1 : 0;
   if (!due) return;
-  cell_bin(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  const CellArgs &R = C->R;
+  cell_bin(R, pos, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  __shared__ int last;
+  __syncthreads();  // this workgroup's counts are issued
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&st->ticket, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every workgroup's counts are visible
+  cell_scan<GH_BINBLOCK>(R);
+  if (threadIdx.x == 0) st->ticket = 0;
 }
-__global__ __launch_bounds__(GH_SCANBLOCK) void k_cell_scan(CellArgs R) {
-  const int due = R.st->rebuild;
-  __syncthreads();  // every thread has read the flag before thread 0 books it
-  if (threadIdx.x == 0) rebuild_book(R.st);
-  if (!due) return;
-  cell_scan(R);
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_scatter(GhStatus *st,
+                                                            const ChainArgs *__restrict__ C) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) rebuild_book(st);
+  if (!st->rebuild) return;
+  cell_scatter(C->R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_scatter(CellArgs R) {
-  if (!R.st->rebuild) return;
-  cell_scatter(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-__global__ __launch_bounds__(GH_TBLOCK) void k_cell_rank_gather(CellArgs R) {
-  if (!R.st->rebuild) return;
-  cell_rank_gather(R, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+__global__ __launch_bounds__(GH_TBLOCK) void k_cell_rank_gather(GhStatus *st,
+                                                                const ChainArgs *__restrict__ C,
+                                                                double4 *pos) {
+  if (!st->rebuild) return;
+  cell_rank_gather(C->R, pos, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Brick-local FP32 positions of the halo slots as three arrays.
@@ -698,15 +724,16 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
   __syncthreads();
 }
 
-__global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(
-    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cellof,
-    const int *__restrict__ cell_start, BuildArgs A) {
+__global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(GhStatus *st,
+                                                           const ChainArgs *__restrict__ C,
+                                                           const double4 *__restrict__ pos) {
   if (!st->rebuild) return;
   extern __shared__ float dynf[];
   __shared__ Halo H;
+  const CellArgs &R = C->R;
   const int b = xcd_block();
-  if (b >= t.nbrick) return;
-  build_brick(st, t, pos, cellof, cell_start, A, dynf, H, b);
+  if (b >= R.t.nbrick) return;
+  build_brick(st, R.t, pos, R.cellof, R.start, C->A, dynf, H, b);
 }
 
 // Largest halo and brick (atoms) over all bricks; sizes slot_cap.
@@ -1168,14 +1195,16 @@ static BuildArgs build_args(const gh_ctx *c, int cap, size_t *lds_out) {
 
 
 int gh_launch_rebuild_cells(gh_ctx *c) {
-  CellArgs R;
+  ChainArgs h;
+  std::memset(&h, 0, sizeof(h));  // padding too: the bytes are compared below
+  CellArgs &R = h.R;
   R.st = c->status;
   R.t = c->tg;
   R.skin = c->skin;
   R.n = c->n;
   R.ncap = c->ncap;
   R.nkey = c->tg.nkey;
-  R.pos = c->pos;
+  R.pos = nullptr;  // alternates: passed to the kernels
   R.pos_t = c->pos_t;
   R.vel = c->vel;
   R.vel_t = c->vel_t;
@@ -1191,16 +1220,29 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   R.start = c->cell_start;
   R.perm = c->perm;
   size_t lds = 0;
-  const BuildArgs A = build_args(c, c->cap, &lds);
+  h.A = build_args(c, c->cap, &lds);
   hipStream_t s = c->stream;
+  // upload on change only, ordered on the stream; the pinned staging copy is
+  // rewritten only after the stream has drained
+  if (c->chain_last.size() != sizeof(h) || std::memcmp(c->chain_last.data(), &h, sizeof(h)) != 0) {
+    if (!c->chain_dev) {
+      GH_HIP(c, hipMalloc(&c->chain_dev, sizeof(h)));
+      GH_HIP(c, hipHostMalloc(&c->chain_pin, sizeof(h), hipHostMallocDefault));
+    }
+    GH_HIP(c, hipStreamSynchronize(s));
+    std::memcpy(c->chain_pin, &h, sizeof(h));
+    GH_HIP(c, hipMemcpyAsync(c->chain_dev, c->chain_pin, sizeof(h), hipMemcpyHostToDevice, s));
+    c->chain_last.assign(reinterpret_cast<const unsigned char *>(&h),
+                         reinterpret_cast<const unsigned char *>(&h) + sizeof(h));
+  }
+  const ChainArgs *C = static_cast<const ChainArgs *>(c->chain_dev);
   // grid-stride launches of at most 2048 workgroups: cheap when no rebuild is due
   const int ga = std::min((R.n + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
-  k_cell_bin<<<ga, GH_TBLOCK, 0, s>>>(R);
-  k_cell_scan<<<1, GH_SCANBLOCK, 0, s>>>(R);
-  k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(R);
-  k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(R);
-  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cellof,
-                                                            c->cell_start, A);
+  const int gb = std::min((R.n + GH_BINBLOCK - 1) / GH_BINBLOCK, 1024);
+  k_cell_bin<<<gb, GH_BINBLOCK, 0, s>>>(c->status, C, c->pos, c->skin);
+  k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(c->status, C);
+  k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(c->status, C, c->pos);
+  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
