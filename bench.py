@@ -426,6 +426,16 @@ def main():
                  "rho_pass_us": tm_64["rho"][0] / tm_64["rho"][1] * 1e3,
                  "rebuilds": nupd_64},
         "mixed_vs_fp64": value / value64,
+        "mixed_vs_fp64_force": {
+            "force_evaluation": (None if tile_mx[1] is None or tile_64[1] is None
+                                 else tile_64[1] / tile_mx[1]),
+            "passes_in_md_step": ((tm_64["rho"][0] / tm_64["rho"][1]
+                                   + tm_64["force"][0] / tm_64["force"][1])
+                                  / (ms_rho / cnt_rho + ms_force / cnt_force)),
+            "note": "force_evaluation: one gh_compute (both SMA-TB passes) at each form's "
+                    "tuned brick shape; passes_in_md_step: the two passes of an MD step "
+                    "(velocity Verlet fused into the second), HIP-event timed; the whole "
+                    "step (mixed_vs_fp64) also holds the list upkeep both forms share"},
         "force_err_vs_fp64": {"rms_over_rms": err_rms, "max_over_rms": err_max},
         "epot_rel_err_vs_fp64": abs(et_mx / et_64 - 1.0),
         "energy_change_ev_per_atom": (e1[0] + e1[1] - e0[0] - e0[1]) / n,
