@@ -828,7 +828,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
       pair(x2, y2, z2, s2);
       pair(x3, y3, z3, s3);
     });
-    rho[a] = acc;
+    if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
     eband32[a] = (float)(1.0 / sqrt(acc));
   }
 }
@@ -899,11 +899,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
       pair(q3, s3);
     });
     const double Fx = fx * fscale, Fy = fy * fscale, Fz = fz * fscale;
-    frc[a] = Fx;
-    frc[ncap + a] = Fy;
-    frc[2 * ncap + a] = Fz;
-    epa[a] = (ei - sqrt(rho[a])) * ev_ui;
-    if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
+    if (fv.on) {
+      dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
+    } else {
+      frc[a] = Fx;
+      frc[ncap + a] = Fy;
+      frc[2 * ncap + a] = Fz;
+      epa[a] = (ei - sqrt(rho[a])) * ev_ui;
+    }
   }
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
@@ -958,7 +961,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
       const double dr = sqrt(dist2_rn(vx, vy, vz));
       if (dr <= P.rce) acc += tb_band64(P, dr);
     });
-    rho[a] = acc;
+    if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
     const double eb = 1.0 / sqrt(acc);
     eband[a] = eb;
     eband32[a] = (float)eb;
@@ -1025,11 +1028,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
       }
     });
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
-    frc[a] = Fx;
-    frc[ncap + a] = Fy;
-    frc[2 * ncap + a] = Fz;
-    epa[a] = (ei - sqrt(rho[a])) * ev_ui;
-    if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
+    if (fv.on) {
+      dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
+    } else {
+      frc[a] = Fx;
+      frc[ncap + a] = Fy;
+      frc[2 * ncap + a] = Fz;
+      epa[a] = (ei - sqrt(rho[a])) * ev_ui;
+    }
   }
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
@@ -1093,11 +1099,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
       }
     });
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
-    frc[a] = Fx;
-    frc[ncap + a] = Fy;
-    frc[2 * ncap + a] = Fz;
-    epa[a] = ei * 0.5 * ev_ui;
-    if (fv.on) dmax2 = fmax(dmax2, fused_vv(fv, a, pi, type[a], Fx, Fy, Fz));
+    if (fv.on) {
+      dmax2 = fmax(dmax2, fused_vv(fv, a, pi, type[a], Fx, Fy, Fz));
+    } else {
+      frc[a] = Fx;
+      frc[ncap + a] = Fy;
+      frc[2 * ncap + a] = Fz;
+      epa[a] = ei * 0.5 * ev_ui;
+    }
   }
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
@@ -1290,6 +1299,9 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   const bool single = c->nt == 1;
   const bool mixed = c->precision == GH_MIXED;
   const size_t slots = (size_t)c->tg.slot_cap + 1;  // + the sentinel slot
+  // a fused step stores no forces, per-atom energies or densities: the
+  // steps whose state is observed run unfused and store them
+  double *rho1 = fuse_vv ? nullptr : c->rho;
   if (c->pot == GH_POT_SMATB) {
     const size_t lds1 = gh_tile_slot_bytes(0, c->precision, c->nt) * slots;
     const size_t lds2 = gh_tile_slot_bytes(1, c->precision, c->nt) * slots;
@@ -1297,11 +1309,11 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (single)
         k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                         c->cell_start, c->nn, c->list16, ld,
-                                                        c->tbf, c->rho, c->eband32, c->tbf0);
+                                                        c->tbf, rho1, c->eband32, c->tbf0);
       else
         k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                          c->cell_start, c->nn, c->list16, ld,
-                                                         c->tbf, c->rho, c->eband32, c->tbf0);
+                                                         c->tbf, rho1, c->eband32, c->tbf0);
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
       if (single)
         k_tb_force_fx<true><<<grid, GH_TBLOCK, lds2, s>>>(
@@ -1315,11 +1327,11 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (single)
         k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, c->rho, c->eband, c->eband32);
+            c->nt, rho1, c->eband, c->eband32);
       else
         k_tb_rho64_tiled<false><<<grid, GH_TBLOCK, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, c->rho, c->eband, c->eband32);
+            c->nt, rho1, c->eband, c->eband32);
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
       if (single)
         k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
