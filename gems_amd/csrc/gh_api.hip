@@ -191,9 +191,12 @@ int gh_upload_tb(gh_ctx *c) {
     o.rho_bq = d.rho_b * 1.1920928955078125e-07f;
   }
   GH_TRY(dalloc(c, &c->tbd, GH_MAXT * GH_MAXT));
-  GH_HIP(c, hipMemcpy(c->tbd, h, sizeof(h), hipMemcpyHostToDevice));
   GH_TRY(dalloc(c, &c->tbf, GH_MAXT * GH_MAXT));
-  GH_HIP(c, hipMemcpy(c->tbf, f, sizeof(f), hipMemcpyHostToDevice));
+  // ordered on the context's stream (a non-blocking stream does not wait for
+  // null-stream copies); h and f live until the stream has drained
+  GH_HIP(c, hipMemcpyAsync(c->tbd, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipMemcpyAsync(c->tbf, f, sizeof(f), hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
   c->tbf0 = f[0];
   return 0;
 }
@@ -487,7 +490,9 @@ int gh_create(gh_ctx **out, int device, int precision) {
   }
   GhStatus h;
   memset(&h, 0, sizeof(h));
-  if (hipMemcpy(c->status, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpyAsync(c->status, &h, sizeof(h), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
     delete c;
     return GH_EHIP;
   }
@@ -698,15 +703,18 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
     }
     ho[i] = i;
   }
-  GH_HIP(c, hipMemcpy(c->pos, hp.data(), sizeof(double4) * ncap, hipMemcpyHostToDevice));
-  GH_HIP(c, hipMemcpy(c->type, ht.data(), sizeof(int) * ncap, hipMemcpyHostToDevice));
-  GH_HIP(c, hipMemcpy(c->orig, ho.data(), sizeof(int) * ncap, hipMemcpyHostToDevice));
-  std::vector<double> z(3 * (size_t)ncap, 0.0);
-  GH_HIP(c, hipMemcpy(c->vel, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
-  GH_HIP(c, hipMemcpy(c->acc, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
-  GH_HIP(c, hipMemcpy(c->frc, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
-  GH_HIP(c, hipMemcpy(c->xold, z.data(), sizeof(double) * 3 * ncap, hipMemcpyHostToDevice));
-  GH_HIP(c, hipMemcpy(c->epa, z.data(), sizeof(double) * ncap, hipMemcpyHostToDevice));
+  // uploads and clears ordered on the context's stream, then drained (the host
+  // vectors die on return)
+  hipStream_t st = c->stream;
+  GH_HIP(c, hipMemcpyAsync(c->pos, hp.data(), sizeof(double4) * ncap, hipMemcpyHostToDevice, st));
+  GH_HIP(c, hipMemcpyAsync(c->type, ht.data(), sizeof(int) * ncap, hipMemcpyHostToDevice, st));
+  GH_HIP(c, hipMemcpyAsync(c->orig, ho.data(), sizeof(int) * ncap, hipMemcpyHostToDevice, st));
+  GH_HIP(c, hipMemsetAsync(c->vel, 0, sizeof(double) * 3 * ncap, st));
+  GH_HIP(c, hipMemsetAsync(c->acc, 0, sizeof(double) * 3 * ncap, st));
+  GH_HIP(c, hipMemsetAsync(c->frc, 0, sizeof(double) * 3 * ncap, st));
+  GH_HIP(c, hipMemsetAsync(c->xold, 0, sizeof(double) * 3 * ncap, st));
+  GH_HIP(c, hipMemsetAsync(c->epa, 0, sizeof(double) * ncap, st));
+  GH_HIP(c, hipStreamSynchronize(st));
   c->have_atoms = true;
   c->ready = false;
   c->pos_dirty = true;
