@@ -8,6 +8,7 @@
 !   gh_set_positions    ig%at(i)%o%pos        src/Neighbour.F90:114
 !   gh_set_smatb        smatb_set/readprm     src/TB.F90:106-170, 291-365
 !   gh_set_lj           lj_set                src/Pairs.F90:198-205
+!   gh_set_plj          plj_set               src/Pairs.F90:282-291
 !   gh_set_skin         nb_dcut               src/Neighbour.F90:237
 !   gh_set_box          box_setvars/set_pbc   src/Program_Types.F90:126-141
 !   gh_neighbors        ig%nn, ig%list        src/Neighbour.F90:120-121
@@ -98,6 +99,14 @@ module gems_hip
       import :: c_ptr, c_double, c_int
       type(c_ptr), value :: ctx
       real(c_double), value :: eps, sigma, rc
+    end function
+
+    integer(c_int) function gh_set_plj(ctx, nt, eps, sigma, rc) bind(C, name='gh_set_plj')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: eps(nt, nt), sigma(nt, nt)
+      real(c_double), value :: rc
     end function
 
     integer(c_int) function gh_set_skin(ctx, skin) bind(C, name='gh_set_skin')

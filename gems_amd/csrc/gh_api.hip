@@ -216,10 +216,14 @@ static const int GH_LDS_BYTES = 65536;
 // the static tables, over the larger per-slot need of the two passes.
 size_t gh_tile_slot_bytes(int pass, int precision, int nt);
 size_t gh_tile_static_bytes(int precision, int nt);
+size_t gh_pair_slot_bytes(int pot);
 static int max_slots(const gh_ctx *c) {
-  const size_t per = std::max(gh_tile_slot_bytes(0, c->precision, c->nt),
-                              gh_tile_slot_bytes(1, c->precision, c->nt));
-  const size_t stat = gh_tile_static_bytes(c->precision, c->nt);
+  const bool pair = c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ;
+  const size_t per = pair ? gh_pair_slot_bytes(c->pot)
+                          : std::max(gh_tile_slot_bytes(0, c->precision, c->nt),
+                                     gh_tile_slot_bytes(1, c->precision, c->nt));
+  const size_t stat = pair ? gh_tile_static_bytes(GH_FP64, 1) + sizeof(PljDev) * GH_MAXT * GH_MAXT
+                           : gh_tile_static_bytes(c->precision, c->nt);
   return (int)std::min<size_t>((GH_LDS_BYTES - stat) / per - 1, 65534);  // - the sentinel
 }
 
@@ -509,7 +513,7 @@ void gh_destroy(gh_ctx *c) {
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
-                 c->red, c->list16, c->xin, c->chain_dev};
+                 c->red, c->list16, c->xin, c->chain_dev, c->pljd};
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->chain_pin) (void)hipHostFree(c->chain_pin);
@@ -812,6 +816,41 @@ int gh_set_lj(gh_ctx *c, double eps, double sigma, double rc) {
   c->lj_sigma = sigma;
   c->lj_rc = rc;
   c->pot = GH_POT_LJ;
+  c->rcut = rc;
+  c->ready = false;
+  return 0;
+}
+
+int gh_set_plj(gh_ctx *c, int nt, const double *eps, const double *sigma, double rc) {
+  if (!c || nt <= 0 || nt > GH_MAXT || !eps || !sigma || !(rc > 0))
+    return gh_fail(c, GH_EINVAL, "gh_set_plj: bad arguments");
+  PljDev h[GH_MAXT * GH_MAXT];
+  memset(h, 0, sizeof(h));
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const double e = eps[a * nt + b], sg = sigma[a * nt + b];
+      if (!(sg > 0) || e != eps[b * nt + a] || sg != sigma[b * nt + a])
+        return gh_fail(c, GH_EINVAL, "gh_set_plj: eps and sigma must be symmetric, sigma > 0");
+      // plj_set (Pairs.F90:282-291)
+      PljDev &d = h[a * GH_MAXT + b];
+      d.e = e;
+      d.s = sg;
+      d.prc = 4.0 * e * (pow(sg / rc, 12) - pow(sg / rc, 6));
+      d.frc = 24.0 * e * (2.0 * (pow(sg, 12) / pow(rc, 13)) - (pow(sg, 6) / pow(rc, 7)));
+      d.ef = (float)e;
+      d.s2f = (float)(sg * sg);
+      d.prcf = (float)d.prc;
+      d.frcf = (float)d.frc;
+    }
+  GH_HIP(c, hipSetDevice(c->device));
+  GH_TRY(dalloc(c, &c->pljd, GH_MAXT * GH_MAXT));
+  GH_HIP(c, hipMemcpyAsync(c->pljd, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  c->nt = nt;
+  c->lj_eps = 0.0;
+  c->lj_sigma = 0.0;
+  c->lj_rc = rc;
+  c->pot = GH_POT_PLJ;
   c->rcut = rc;
   c->ready = false;
   return 0;

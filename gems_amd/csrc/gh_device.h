@@ -264,6 +264,27 @@ __device__ __forceinline__ bool tb_force_fx(const TbF &P, double r2q, float ebs,
   return true;
 }
 
+// Shifted Lennard-Jones pair (plj_interact, Pairs.F90:293-370): energy p and
+// f/r with f = -dp/dr, for r^2 within the cut; FP64 as the reference's
+// arithmetic, FP32 with the hardware reciprocal square root.
+__device__ __forceinline__ void plj_pair64(const PljDev &P, double rc, double r2, double &p,
+                                           double &f_r) {
+  const double dr = sqrt(r2);
+  const double t = P.s / dr;
+  const double t2 = t * t, t4 = t2 * t2, s6 = t4 * t2;  // (s/dr)**6
+  p = 4.0 * P.e * (s6 * s6 - s6) - P.prc - P.frc * (dr - rc);
+  f_r = (P.e * (48.0 * s6 - 24.0) * s6 / dr + P.frc) / dr;
+}
+__device__ __forceinline__ void plj_pair32(const PljDev &P, float rcf, float r2f, float &p,
+                                           float &f_r) {
+  const float ir = __builtin_amdgcn_rsqf(r2f);
+  const float dr = r2f * ir, ir2 = ir * ir;
+  const float s2 = P.s2f * ir2;
+  const float s6 = s2 * s2 * s2;
+  p = 4.0f * P.ef * (s6 * s6 - s6) - P.prcf - P.frcf * (dr - rcf);
+  f_r = fmaf(P.ef * fmaf(48.0f, s6, -24.0f), s6, P.frcf * dr) * ir2;
+}
+
 // Philox4x32-10 (Salmon et al., SC'11): counter-based stream.
 __device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
 #pragma unroll

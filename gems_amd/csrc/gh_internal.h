@@ -73,6 +73,14 @@ struct TbF {
   // r0q, rciq, rceq = r0f, rci_f, rce_f * 2^23; rep_bq, rho_bq = rep_b, rho_b * 2^-23
   float r0q, rciq, rceq, rep_bq, rho_bq, pad;
 };
+// Shifted Lennard-Jones (plj) per type pair: prc and frc are the value and
+// the slope of the plain 12-6 form at the cut (plj_set, Pairs.F90:282-291);
+// FP64 and FP32 forms.
+struct PljDev {
+  double e, s, prc, frc;
+  float ef, s2f, prcf, frcf;
+};
+
 #define GH_FXS 8388608.0         // fixed-point steps per Angstrom (2^23)
 #define GH_FXS2 70368744177664.0 // GH_FXS^2 (2^46)
 
@@ -190,6 +198,7 @@ struct gh_ctx {
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
   TbF *tbf = nullptr;                         // [MAXT*MAXT]
   TbF tbf0;                                   // host copy of tbf[0] (one-type kernel argument)
+  PljDev *pljd = nullptr;                     // [MAXT*MAXT] shifted LJ per type pair
   GhStatus *status = nullptr;
   double *red = nullptr;                      // reduction scratch
   int nred = 0;

@@ -408,6 +408,58 @@ __global__ __launch_bounds__(GH_BLOCK) void k_lj64(
   epa[i] = ei * 0.5 * ev_ui;
 }
 
+// Shifted Lennard-Jones (plj_interact, Pairs.F90:293-370) on the O(N^2)-mode
+// list, newton off: per type pair p = 4e(s6^2 - s6) - prc - frc (r - rc),
+// f = e(48 s6 - 24) s6 / r + frc; e_i += p/2; F_i -= f vd / r.
+template <bool MIXED>
+__global__ __launch_bounds__(GH_BLOCK) void k_plj(
+    const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
+    const unsigned *__restrict__ list, int ld, int n, int ncap, const PljDev *__restrict__ pljd,
+    double rc2, double L0, double L1, double L2, double ev_ui, double *__restrict__ frc,
+    double *__restrict__ epa) {
+  __shared__ ShiftLds sh;
+  __shared__ PljDev pp[GH_MAXT * GH_MAXT];
+  load_shift(sh, L0, L1, L2);
+  for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) pp[k] = pljd[k];
+  __syncthreads();
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int ti = type[i], m = nn[i];
+  const double rc = sqrt(rc2);
+  const float rcf = (float)rc;
+  double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
+    const int j = (int)(e & GH_JMASK);
+    const int w = (int)(e >> 27);
+    const double4 pj = pos[j];
+    const double vx = (pj.x - pi.x) + sh.x[w];
+    const double vy = (pj.y - pi.y) + sh.y[w];
+    const double vz = (pj.z - pi.z) + sh.z[w];
+    const double r2 = MIXED ? fma(vx, vx, fma(vy, vy, vz * vz)) : dist2_rn(vx, vy, vz);
+    if (r2 > rc2) continue;
+    const PljDev &P = pp[ti * GH_MAXT + type[j]];
+    double p, sf;
+    if (MIXED) {
+      float pf, fr;
+      plj_pair32(P, rcf, (float)r2, pf, fr);
+      p = (double)pf;
+      sf = (double)fr;
+    } else {
+      plj_pair64(P, rc, r2, p, sf);
+    }
+    ei += p;
+    fx -= sf * vx;
+    fy -= sf * vy;
+    fz -= sf * vz;
+  }
+  frc[i] = fx * ev_ui;
+  frc[ncap + i] = fy * ev_ui;
+  frc[2 * ncap + i] = fz * ev_ui;
+  epa[i] = ei * 0.5 * ev_ui;
+}
+
 __global__ __launch_bounds__(GH_BLOCK) void k_lj_mx(
     const double4 *__restrict__ pos, const int *__restrict__ nn, const unsigned *__restrict__ list,
     int ld, int n, int ncap, float eps, float sigma2, double rc2, double L0, double L1,
@@ -838,7 +890,7 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
 
 bool gh_can_fuse_vv(const gh_ctx *c) {
   return c->list_mode == 1 && c->integ == GH_INT_VV &&
-         (c->pot == GH_POT_SMATB || c->pot == GH_POT_LJ);
+         (c->pot == GH_POT_SMATB || c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ);
 }
 
 int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
@@ -903,6 +955,16 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     else
       k_lj64<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
                                      c->lj_sigma, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
+  } else if (c->pot == GH_POT_PLJ) {
+    if (mid) GH_HIP(c, hipEventRecord(mid, s));
+    const double rc2 = c->lj_rc * c->lj_rc;
+    if (mixed)
+      k_plj<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->ncap,
+                                                 c->pljd, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
+    else
+      k_plj<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->ncap,
+                                                  c->pljd, rc2, L0, L1, L2, ev_ui, c->frc,
+                                                  c->epa);
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");
   }

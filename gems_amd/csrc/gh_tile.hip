@@ -1041,45 +1041,66 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
 }
 
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
-template <bool MIXED>
+template <bool MIXED, bool PLJ>
 __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
     double eps, double sigma, double rc2, double ev_ui, double *__restrict__ frc,
-    double *__restrict__ epa, const int *__restrict__ type, FuseVV fv) {
+    double *__restrict__ epa, const int *__restrict__ type, FuseVV fv,
+    const PljDev *__restrict__ pljd) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
+  __shared__ PljDev pp[PLJ ? GH_MAXT * GH_MAXT : 1];
+  if (PLJ)
+    for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) pp[k] = pljd[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
   unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * cap);
+  unsigned char *ty = scell + cap;  // plj: the slots' types
   slot_map(H, scell);
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
-    const double4 p = pos[s + H.delta[h]];
+    const int src = s + H.delta[h];
+    const double4 p = pos[src];
     px[s] = p.x + shift(t, H, 0, h);
     py[s] = p.y + shift(t, H, 1, h);
     pz[s] = p.z + shift(t, H, 2, h);
+    if (PLJ) ty[s] = (unsigned char)type[src];
   }
-  if (threadIdx.x == 0) px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
+  if (threadIdx.x == 0) {
+    px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
+    if (PLJ) ty[t.slot_cap] = 0;
+  }
   __syncthreads();
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
+  const double rc = sqrt(rc2);
+  const float rcf = (float)rc;
   double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
+    const int ti = type[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
     for_each_slot(list16, ld, a, nn[a], [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
       if (MIXED) {
         const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
         if (r2 <= rc2) {
-          const float ir2 = __builtin_amdgcn_rcpf((float)r2);
-          const float s2 = sig2f * ir2;
-          const float s6 = s2 * s2 * s2;
-          ei += (double)(4.0f * epsf * (s6 * s6 - s6));
-          const double sf = (double)(epsf * fmaf(48.0f, s6, -24.0f) * s6 * ir2);
+          double sf;
+          if (PLJ) {
+            float p, f_r;
+            plj_pair32(pp[ti * GH_MAXT + ty[s]], rcf, (float)r2, p, f_r);
+            ei += (double)p;
+            sf = (double)f_r;
+          } else {
+            const float ir2 = __builtin_amdgcn_rcpf((float)r2);
+            const float s2 = sig2f * ir2;
+            const float s6 = s2 * s2 * s2;
+            ei += (double)(4.0f * epsf * (s6 * s6 - s6));
+            sf = (double)(epsf * fmaf(48.0f, s6, -24.0f) * s6 * ir2);
+          }
           fx = fma(-sf, vx, fx);
           fy = fma(-sf, vy, fy);
           fz = fma(-sf, vz, fz);
@@ -1087,11 +1108,18 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
       } else {
         const double r2 = dist2_rn(vx, vy, vz);
         if (r2 <= rc2) {
-          const double dr = sqrt(r2);
-          const double tt = sigma / dr;
-          const double t2 = tt * tt, t4 = t2 * t2, s6 = t4 * t2;
-          ei += 4.0 * eps * (s6 * s6 - s6);
-          const double sf = eps * (48.0 * s6 - 24.0) * s6 / dr / dr;
+          double sf;
+          if (PLJ) {
+            double p;
+            plj_pair64(pp[ti * GH_MAXT + ty[s]], rc, r2, p, sf);
+            ei += p;
+          } else {
+            const double dr = sqrt(r2);
+            const double tt = sigma / dr;
+            const double t2 = tt * tt, t4 = t2 * t2, s6 = t4 * t2;
+            ei += 4.0 * eps * (s6 * s6 - s6);
+            sf = eps * (48.0 * s6 - 24.0) * s6 / dr / dr;
+          }
           fx -= sf * vx;
           fy -= sf * vy;
           fz -= sf * vz;
@@ -1100,7 +1128,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     });
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
     if (fv.on) {
-      dmax2 = fmax(dmax2, fused_vv(fv, a, pi, type[a], Fx, Fy, Fz));
+      dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
     } else {
       frc[a] = Fx;
       frc[ncap + a] = Fy;
@@ -1125,6 +1153,8 @@ size_t gh_tile_slot_bytes(int pass, int precision, int nt) {
   if (pass == 0) return 24 + 1 + (multi ? 4 : 0);  // rho: xyz + map (+ type)
   return 32 + 1 + (multi ? 4 : 0);                 // force: xyz + eband + map (+ type)
 }
+// Pair potentials (LJ, plj): FP64 xyz + map per slot (+ the type for plj).
+size_t gh_pair_slot_bytes(int pot) { return 25 + (pot == GH_POT_PLJ ? 1 : 0); }
 size_t gh_tile_static_bytes(int precision, int nt) {
   const size_t params = nt > 1 ? (precision == GH_MIXED ? sizeof(TbF) : sizeof(TbDev)) *
                                      GH_MAXT * GH_MAXT
@@ -1141,7 +1171,7 @@ static size_t build_slots() { return (GH_BUILD_LDS - build_far_bytes()) / 13; }
 // Bounds of the pair function's inner branch and of its cutoff over all type
 // pairs: rin = min rci, rout = max rce (SMA-TB); rin = rout = rc (LJ).
 static void inner_outer(const gh_ctx *c, double *rin, double *rout) {
-  if (c->pot == GH_POT_LJ) {
+  if (c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ) {
     *rin = *rout = c->lj_rc;
     return;
   }
@@ -1342,20 +1372,22 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
             c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv);
     }
-  } else if (c->pot == GH_POT_LJ) {
+  } else if (c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
-    const size_t lds = 25 * slots;
-    if (mixed)
-      k_lj_tiled<true><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
-                                                    c->nn, c->list16, ld, c->ncap, c->lj_eps,
-                                                    c->lj_sigma, rc2, ev_ui, c->frc, c->epa,
-                                                    c->type, fv);
-    else
-      k_lj_tiled<false><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
-                                                     c->nn, c->list16, ld, c->ncap, c->lj_eps,
-                                                     c->lj_sigma, rc2, ev_ui, c->frc, c->epa,
-                                                    c->type, fv);
+    const bool plj = c->pot == GH_POT_PLJ;
+    const size_t lds = gh_pair_slot_bytes(c->pot) * slots;
+#define LJ_LAUNCH(MX, PL)                                                                   \
+  k_lj_tiled<MX, PL><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,  \
+                                                  c->nn, c->list16, ld, c->ncap, c->lj_eps, \
+                                                  c->lj_sigma, rc2, ev_ui, c->frc, c->epa,  \
+                                                  c->type, fv, c->pljd)
+    if (mixed) {
+      if (plj) LJ_LAUNCH(true, true); else LJ_LAUNCH(true, false);
+    } else {
+      if (plj) LJ_LAUNCH(false, true); else LJ_LAUNCH(false, false);
+    }
+#undef LJ_LAUNCH
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");
   }

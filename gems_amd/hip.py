@@ -20,7 +20,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 SYMBOLS = [
     "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box",
     "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
-    "gh_set_lj",
+    "gh_set_lj", "gh_set_plj",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
     "gh_get_stats", "gh_set_timing", "gh_get_timing", "gh_set_tile", "gh_autotune",
@@ -70,6 +70,7 @@ def load():
     L.gh_set_positions.argtypes = [vp, _dp]
     L.gh_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
     L.gh_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+    L.gh_set_plj.argtypes = [vp, C.c_int, _dp, _dp, C.c_double]
     L.gh_set_skin.argtypes = [vp, C.c_double]
     L.gh_set_tile.argtypes = [vp, _ip]
     L.gh_autotune.argtypes = [vp, C.c_int, _ip, _dp]
@@ -184,6 +185,15 @@ class Context:
 
     def set_lj(self, eps, sigma, rc):
         self._ck(self.L.gh_set_lj(self.h, eps, sigma, rc))
+
+    def set_plj(self, eps, sigma, rc=10.0):
+        """Shifted Lennard-Jones (plj, Pairs.F90:282-370): eps (eV) and sigma
+        (A) are nt x nt symmetric matrices over the atom types; rc in A."""
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        nt = e.shape[0]
+        self._ck(self.L.gh_set_plj(self.h, nt, e.ctypes.data_as(_dp),
+                                   sg.ctypes.data_as(_dp), rc))
 
     def set_skin(self, skin):
         self._ck(self.L.gh_set_skin(self.h, skin))

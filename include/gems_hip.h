@@ -47,6 +47,7 @@ typedef struct gh_ctx gh_ctx;
 #define GH_POT_NONE 0
 #define GH_POT_SMATB 1
 #define GH_POT_LJ 2
+#define GH_POT_PLJ 3
 
 /* integrators */
 #define GH_INT_VV 0     /* velocity Verlet   (Integration.F90:1150-1186) */
@@ -90,6 +91,16 @@ int gh_set_smatb(gh_ctx *ctx, int nt, const double *A, const double *xi,
 /* Lennard-Jones 12-6, unshifted: `interact ... under lj e s rc`
  * (lj_set, Pairs.F90:198-205).  eps in eV, sigma and rc in A. */
 int gh_set_lj(gh_ctx *ctx, double eps, double sigma, double rc);
+
+/* Lennard-Jones with its value and slope shifted to zero at the cut:
+ * `interact A with B plj e s` (plj_set / plj_interact, Pairs.F90:282-370;
+ * the script form fixes rc = 10 A, Pairs.F90:148-151).  One context holds
+ * every pair of types: eps (eV) and sigma (A) are nt x nt, row-major and
+ * symmetric (entry [ti][tj] is the reference's plj group for types ti, tj),
+ * rc (A) is the cut of all of them; types come from gh_set_atoms.  Each
+ * directed pair adds half its energy to atom i, as the full-list build
+ * does. */
+int gh_set_plj(gh_ctx *ctx, int nt, const double *eps, const double *sigma, double rc);
 
 /* Verlet skin `nb_dcut` (Neighbour.F90:237), A. */
 int gh_set_skin(gh_ctx *ctx, double skin);

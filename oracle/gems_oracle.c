@@ -479,6 +479,63 @@ void or_lj_force(const double *x, int n, const or_box *b, double eps,
 }
 
 /* ------------------------------------------------------------------ */
+/* shifted Lennard-Jones: Pairs.F90:282-370 (plj_set, plj_interact)      */
+/* ------------------------------------------------------------------ */
+void or_plj_init(or_plj *p, int nt, double rc) {
+  memset(p, 0, sizeof(*p));
+  p->nt = nt;
+  p->rc = rc;
+}
+
+void or_plj_set(or_plj *p, int i, int j, double e, double s) {
+  double rc = p->rc;
+  double prc = 4.0 * e * (pow(s / rc, 12) - pow(s / rc, 6));
+  double frc = 24.0 * e * (2.0 * (pow(s, 12) / pow(rc, 13)) - (pow(s, 6) / pow(rc, 7)));
+  int k[2] = {i * OR_MAXT + j, j * OR_MAXT + i};
+  for (int u = 0; u < 2; ++u) {
+    p->e[k[u]] = e;
+    p->s[k[u]] = s;
+    p->prc[k[u]] = prc;
+    p->frc[k[u]] = frc;
+  }
+}
+
+void or_plj_force(const double *x, const int *type, int n, const or_box *b, const or_plj *pl,
+                  const int *nn, const int *list, int maxnb, double *f, double *e,
+                  double *epot, int nthreads) {
+  double ev_ui = 1.0 / c_ui_ev();
+  double rcut = pl->rc, rcut2 = rcut * rcut;
+  (void)nthreads;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int i = 0; i < n; ++i) {
+    double fi[3] = {0.0, 0.0, 0.0}, ei = 0.0;
+    for (int l = 0; l < nn[i]; ++l) {
+      int j = list[(size_t)i * maxnb + l];
+      int k = type[i] * OR_MAXT + type[j];
+      double vd[3];
+      or_vdistance(b, x + 3 * j, x + 3 * i, vd);
+      double dr = dot3(vd);
+      if (dr > rcut2) continue;
+      dr = sqrt(dr);
+      double t = pl->s[k] / dr;
+      double t2 = t * t, t4 = t2 * t2;
+      double s_dr = t4 * t2; /* (s/dr)**6 */
+      double p = 4.0 * pl->e[k] * (s_dr * s_dr - s_dr) - pl->prc[k] - pl->frc[k] * (dr - rcut);
+      double fr = pl->e[k] * (48.0 * s_dr - 24.0) * s_dr / dr + pl->frc[k];
+      p = p * ev_ui;
+      fr = fr * ev_ui;
+      ei = ei + p * 0.5;
+      for (int d = 0; d < 3; ++d) fi[d] = fi[d] - fr * vd[d] / dr;
+    }
+    for (int d = 0; d < 3; ++d) f[3 * i + d] = fi[d];
+    e[i] = ei;
+  }
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) acc += e[i];
+  if (epot) *epot = acc;
+}
+
+/* ------------------------------------------------------------------ */
 /* integrators: Integration.F90:909-1031 (Ermak), 1150-1186 (VV)         */
 /* ------------------------------------------------------------------ */
 void or_vv_a(int n, double dt, double *x, double *v, double *a,
@@ -626,7 +683,7 @@ void or_set_gdist(or_rng *r, int n, double T, double *v, const double *mass,
 /* Neighbour.F90:1242-1346 (update/test_update)                          */
 /* ------------------------------------------------------------------ */
 static double md_rcut(const or_md *md) {
-  return md->pot == 0 ? or_smatb_rcut(md->tb) : md->lj_rc;
+  return md->pot == 0 ? or_smatb_rcut(md->tb) : md->pot == 2 ? md->plj->rc : md->lj_rc;
 }
 
 static int md_update(or_md *md) {
@@ -665,6 +722,9 @@ int or_md_interact(or_md *md) {
     or_smatb_force(md->x, md->type, md->n, &md->box, md->tb, md->nn, md->list,
                    md->maxnb, md->f, md->e, NULL, NULL, &md->epot,
                    md->nthreads);
+  else if (md->pot == 2)
+    or_plj_force(md->x, md->type, md->n, &md->box, md->plj, md->nn, md->list, md->maxnb,
+                 md->f, md->e, &md->epot, md->nthreads);
   else
     or_lj_force(md->x, md->n, &md->box, md->lj_eps, md->lj_sigma, md->lj_rc,
                 md->nn, md->list, md->maxnb, md->f, md->e, &md->epot,

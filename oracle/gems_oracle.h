@@ -95,6 +95,23 @@ void or_lj_force(const double *x, int n, const or_box *b, double eps,
                  double sigma, double rc, const int *nn, const int *list,
                  int maxnb, double *f, double *e, double *epot, int nthreads);
 
+/* ---- shifted Lennard-Jones "plj" (src/Pairs.F90:282-370) ----
+ * One table over all type pairs (the reference has one plj intergroup per
+ * group pair); one cut rc for all (the script form fixes 10 A). */
+typedef struct {
+  int nt;
+  double rc;
+  double e[OR_MAXT * OR_MAXT], s[OR_MAXT * OR_MAXT], prc[OR_MAXT * OR_MAXT],
+      frc[OR_MAXT * OR_MAXT];
+} or_plj;
+void or_plj_init(or_plj *p, int nt, double rc);
+/* plj_set (Pairs.F90:282-291) for types i, j (0-based), both orders */
+void or_plj_set(or_plj *p, int i, int j, double e, double s);
+/* plj_interact (Pairs.F90:293-370), full list, newton off; f, e overwritten */
+void or_plj_force(const double *x, const int *type, int n, const or_box *b, const or_plj *p,
+                  const int *nn, const int *list, int maxnb, double *f, double *e,
+                  double *epot, int nthreads);
+
 /* ---- integrators (src/Integration.F90) ---- */
 void or_vv_a(int n, double dt, double *x, double *v, double *a,
              const double *f, const double *one_mass);      /* :1150-1169 */
@@ -129,10 +146,11 @@ typedef struct {
   const int *type;             /* [n] internal types (SMA-TB), 0-based */
   const double *mass;          /* [n] */
   or_box box;
-  /* potential: 0 = smatb, 1 = lj */
+  /* potential: 0 = smatb, 1 = lj, 2 = plj */
   int pot;
   const or_smatb *tb;
   double lj_eps, lj_sigma, lj_rc;
+  const or_plj *plj;
   double skin;                 /* nb_dcut */
   /* integrator: 0 = velocity Verlet, 1 = Ermak */
   int integ;

@@ -64,12 +64,18 @@ class Ermak(C.Structure):
                 ("c0", "c1", "c2", "sdr", "sdv", "crv1", "crv2", "skt", "dt", "gamma")]
 
 
+class Plj(C.Structure):
+    _fields_ = [("nt", C.c_int), ("rc", C.c_double)] + [
+        (nm, C.c_double * _MM) for nm in ("e", "s", "prc", "frc")]
+
+
 class MD(C.Structure):
     _fields_ = [
         ("n", C.c_int), ("x", _dp), ("v", _dp), ("a", _dp), ("f", _dp), ("e", _dp),
         ("type", _ip), ("mass", _dp), ("box", Box),
         ("pot", C.c_int), ("tb", C.POINTER(Smatb)),
         ("lj_eps", C.c_double), ("lj_sigma", C.c_double), ("lj_rc", C.c_double),
+        ("plj", C.c_void_p),
         ("skin", C.c_double),
         ("integ", C.c_int), ("dt", C.c_double), ("T", C.c_double), ("gamma", C.c_double),
         ("rng", C.POINTER(Rng)),
@@ -103,6 +109,10 @@ def _setup(L):
                                  _ip, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
     L.or_lj_force.argtypes = [_dp, C.c_int, C.POINTER(Box), C.c_double, C.c_double,
                               C.c_double, _ip, _ip, C.c_int, _dp, _dp, _dp, C.c_int]
+    L.or_plj_init.argtypes = [C.POINTER(Plj), C.c_int, C.c_double]
+    L.or_plj_set.argtypes = [C.POINTER(Plj), C.c_int, C.c_int, C.c_double, C.c_double]
+    L.or_plj_force.argtypes = [_dp, _ip, C.c_int, C.POINTER(Box), C.POINTER(Plj), _ip, _ip,
+                               C.c_int, _dp, _dp, _dp, C.c_int]
     L.or_ermak_set.argtypes = [C.POINTER(Ermak), C.c_double, C.c_double, C.c_double]
     L.or_ekin.restype = C.c_double
     L.or_ekin.argtypes = [C.c_int, _dp, _dp]
@@ -231,6 +241,32 @@ def lj_force(x, box, eps, sigma, rc, nn, lst, nthreads=1):
     return f, e, ep.value
 
 
+def plj_params(eps, sigma, rc=10.0):
+    """plj_set (Pairs.F90:282-291) over nt x nt symmetric eps (eV), sigma (A)."""
+    eps = np.asarray(eps, dtype=np.float64)
+    sigma = np.asarray(sigma, dtype=np.float64)
+    nt = eps.shape[0]
+    p = Plj()
+    lib().or_plj_init(C.byref(p), nt, rc)
+    for i in range(nt):
+        for j in range(i, nt):
+            lib().or_plj_set(C.byref(p), i, j, eps[i, j], sigma[i, j])
+    return p
+
+
+def plj_force(x, types, box, p, nn, lst, nthreads=1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[0]
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    f = np.zeros((n, 3))
+    e = np.zeros(n)
+    ep = C.c_double(0)
+    lst = np.ascontiguousarray(lst, dtype=np.int32)
+    lib().or_plj_force(_p(x), _p(t, _ip), n, C.byref(box), C.byref(p), _p(nn, _ip),
+                       _p(lst, _ip), lst.shape[1], _p(f), _p(e), C.byref(ep), nthreads)
+    return f, e, ep.value
+
+
 def ekin(v, mass):
     v = np.ascontiguousarray(v, dtype=np.float64)
     m = np.ascontiguousarray(mass, dtype=np.float64)
@@ -255,7 +291,7 @@ def ermak_coeffs(dt, gamma, T):
 class MDRun:
     """The reference's ``dinamica`` loop on the CPU (Programs.f90:114-167)."""
 
-    def __init__(self, x, v, mass, box, pot="smatb", tb=None, types=None, lj=None,
+    def __init__(self, x, v, mass, box, pot="smatb", tb=None, types=None, lj=None, plj=None,
                  skin=1.0, integ="vv", dt=0.002, T=0.0, gamma=1.0, rng=None,
                  nthreads=1, maxnb=128):
         n = x.shape[0]
@@ -280,11 +316,14 @@ class MDRun:
         md = MD()
         md.n = n
         md.box = box
-        md.pot = 0 if pot == "smatb" else 1
+        md.pot = {"smatb": 0, "lj": 1, "plj": 2}[pot]
         if tb is not None:
             md.tb = C.pointer(tb)
         if lj is not None:
             md.lj_eps, md.lj_sigma, md.lj_rc = lj
+        self.plj = plj
+        if plj is not None:
+            md.plj = C.cast(C.pointer(plj), C.c_void_p)
         md.skin = skin
         md.integ = 0 if integ == "vv" else 1
         md.dt, md.T, md.gamma = dt, T, gamma

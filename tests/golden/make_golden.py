@@ -4,6 +4,7 @@ oracle/ref.mk) on seeded inputs.  Run in the build container only:
 
     make -f oracle/ref.mk REF=<path to the GEMS sources>
     python tests/golden/make_golden.py
+    python tests/golden/make_golden.py --plj <path to the GEMS sources>
 
 Inputs are written with literal 17-digit numbers (no `{}` expressions or
 `$var$` expansion), so the reference reads exactly the values stored here.
@@ -330,8 +331,68 @@ def drift_main(nsteps=10000, each=100):
         shutil.rmtree(wd)
 
 
+def nve_plj_script(ref_dir, nsteps, outputs):
+    """The reference's examples/NVE/v_verlet.gms (100 He + 100 Ne placed by
+    `fill` from the lcg stream, shifted LJ for the three group pairs, VV) with
+    the pair parameters as literals (the example's `{}` expressions evaluated
+    here) and `outputs` in place of its energy file."""
+    kc = 2.61144768e19 / 6.0221417930e23 * 1000.0  # $kcalmol_ev$ of the example
+    e = {"He": 0.021270 * kc, "Ne": 0.086000 * kc, "HeNe": np.sqrt(0.086000 * 0.021270) * kc}
+    sg = {"He": 1.4800, "Ne": 1.5300, "HeNe": 0.5 * (1.5300 + 1.4800)}
+    s = ("dimension 3\nbox size 50 50 50\nprng lcg\nprng seed 123456\n"
+         ">< fill 100 2.6\nsys add He\n>< fill 100 2.6\nsys add Ne\n"
+         "> sys\nset pbc T T T\ngroup 1 add\n> element He\ngroup 2 add\n"
+         "> element Ne\ngroup 3 add\nbox move\ntime step 0.002d0\n")
+    s += "interact 2 with 2 plj %s %s\n" % (fmt(e["He"]), fmt(sg["He"]))
+    s += "interact 3 with 3 plj %s %s\n" % (fmt(e["Ne"]), fmt(sg["Ne"]))
+    s += "interact 2 with 3 plj %s %s\n" % (fmt(e["HeNe"]), fmt(sg["HeNe"]))
+    s += "out float 25.16\n" + outputs
+    s += "> sys\nset tempgdist 300\nevolve v_verlet\ndinamica %d\n" % nsteps
+    return s, e, sg
+
+
+def nve_plj_main(ref_dir):
+    """nve_plj.npz: the state after step 1 of the NVE example (positions,
+    velocities, 16 digits) and the 16-digit energy columns every 100 steps to
+    step 5000 of the same run; ref_Energy.v_verlet.dat: the example's own
+    known-answer file (7 digits), copied."""
+    wd = tempfile.mkdtemp(prefix="gold_")
+    try:
+        outs = ("outfile :x name X.xyz\noutfile :x pos 1\noutfile :x each 1\n"
+                "outfile :v name V.xyz\noutfile :v vel 1\noutfile :v each 1\n")
+        sc, e, sg = nve_plj_script(ref_dir, 1, outs)
+        run(wd, sc)
+        syms = [ln.split()[0] for ln in open(os.path.join(wd, "X.xyz")).read().splitlines()[2:]
+                if ln.strip()]
+        x1 = read_frames(os.path.join(wd, "X.xyz"))[-1]
+        v1 = read_frames(os.path.join(wd, "V.xyz"))[-1]
+        sc, _, _ = nve_plj_script(ref_dir, 5000, "outfile :e name E.dat\noutfile :e cols energy 1\n"
+                                                  "outfile :e each 100\n")
+        log, nupd = run(wd, sc)
+        E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3)
+        kat = os.path.join(ref_dir, "examples", "NVE", "ref", "Energy.v_verlet.dat")
+        shutil.copy(kat, os.path.join(HERE, "ref_Energy.v_verlet.dat"))
+        K = np.loadtxt(kat)
+        ulp = 10.0 ** (np.floor(np.log10(np.abs(K))) - 6)
+        types = np.array([0 if t == "He" else 1 for t in syms], np.int32)
+        eps = np.array([[e["He"], e["HeNe"]], [e["HeNe"], e["Ne"]]])
+        sig = np.array([[sg["He"], sg["HeNe"]], [sg["HeNe"], sg["Ne"]]])
+        np.savez_compressed(os.path.join(HERE, "nve_plj.npz"), x1=x1, v1=v1, types=types,
+                            # Elements.F90:65,73: default-real (FP32) literals
+                            mass=np.array([np.float32(4.0032), np.float32(20.183)], np.float64),
+                            L=50.0, eps=eps, sigma=sig,
+                            rc=10.0, skin=1.0, dt=0.002, energy=E, each=100, nsteps=5000,
+                            nupd=nupd)
+        print("nve_plj n=%d rows=%d nupd=%d; vs the example's own file: max %.2f units of its "
+              "last digit" % (len(x1), len(E), nupd, np.max(np.abs(E - K) / ulp)))
+    finally:
+        shutil.rmtree(wd)
+
+
 if __name__ == "__main__":
     if "--drift" in sys.argv:
         drift_main()
+    elif "--plj" in sys.argv:
+        nve_plj_main(sys.argv[sys.argv.index("--plj") + 1])
     else:
         main()

@@ -1,0 +1,114 @@
+"""Shifted Lennard-Jones (`plj`, plj_set / plj_interact, src/Pairs.F90:282-370)
+on the device path, against the oracle and the reference's own NVE example.
+
+The workload is examples/NVE/v_verlet.gms: 100 He + 100 Ne placed by the
+reference's `fill` in a 50 A periodic box, one plj table over the three type
+pairs (rc = 10 A, the script form's cut), velocity Verlet dt = 2 fs, skin
+1 A.  tests/golden/nve_plj.npz holds the reference binary's state after
+step 1 and its 16-digit energy columns every 100 steps to step 5000; the
+example's own ref/Energy.v_verlet.dat (7 digits) is
+tests/golden/ref_Energy.v_verlet.dat.  The dilute gas is chaotic (the
+16-digit rounding of the step-1 state reaches the 7th digit of epot after
+step 1800), so rows are compared in the window that rounding leaves and the
+total energy over all 50 rows.
+
+Bars: forces against the oracle, max|dF|/rms|F| <= 1e-10 (FP64) / 1e-5
+(mixed).  The energy trace, FP64: epot within 1e-6 and ekin, total within
+1e-9 relative of the reference's 16-digit columns to step 2000, every column
+within 0.6 of the example file's last digit to step 1800; mixed: the total
+within 1e-6 relative of the example file and epot within 1e-5 eV to step
+1800.  Both: no drift, the median total of the last ten rows within 2e-5
+relative of the first (a single row can fall inside a close collision, where
+velocity Verlet's total swings by +-2.5 meV in either precision and in the
+reference's own per-step trace, so rows past the chaos window are compared
+through the median).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import common as cm
+
+pytestmark = pytest.mark.gpu
+
+hip = pytest.importorskip("gems_amd.hip")
+
+
+def _fixture():
+    return np.load(cm.GOLDEN + "/nve_plj.npz")
+
+
+def _ctx(precision, f, x, box=True):
+    c = hip.Context(0, precision)
+    if box:
+        c.set_box([float(f["L"])] * 3, (1, 1, 1))
+    else:
+        c.set_box(None)
+    c.set_atoms(x, f["types"][:len(x)])
+    c.set_masses(f["mass"])
+    c.set_plj(f["eps"], f["sigma"], float(f["rc"]))
+    c.set_skin(float(f["skin"]))
+    return c
+
+
+def _oracle_forces(f, x, box):
+    p = O.plj_params(f["eps"], f["sigma"], float(f["rc"]))
+    nn, lst, _ = O.neighbours(x, box, float(f["rc"]) + float(f["skin"]))
+    return O.plj_force(x, f["types"][:len(x)], box, p, nn, lst)
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_plj_forces_match_oracle_tiled(precision):
+    f = _fixture()
+    Fo, eo, epo = _oracle_forces(f, f["x1"], O.make_box([float(f["L"])] * 3, (1, 1, 1)))
+    with _ctx(precision, f, f["x1"]) as c:
+        F, e, et = c.compute()
+        assert c.stats()["list_mode"] == 1
+    tol = 1e-10 if precision == hip.GH_FP64 else 1e-5
+    assert cm.force_err(F, Fo) <= tol
+    assert et == pytest.approx(epo, rel=1e-10 if precision == hip.GH_FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_plj_forces_match_oracle_free_cluster(precision):
+    # the atoms of a 24 A corner of the box as a free cluster: the O(N^2) list
+    f = _fixture()
+    x = f["x1"]
+    sel = np.all(x < 24.0, axis=1)
+    order = np.argsort(~sel, kind="stable")  # the selected atoms first, types kept aligned
+    g = dict(f)
+    g["types"] = f["types"][order]
+    xs = x[order][:int(sel.sum())]
+    Fo, eo, epo = _oracle_forces(g, xs, O.make_box(None))
+    with _ctx(precision, g, xs, box=False) as c:
+        F, e, et = c.compute()
+        assert c.stats()["list_mode"] == 0
+    tol = 1e-10 if precision == hip.GH_FP64 else 1e-5
+    assert cm.force_err(F, Fo) <= tol
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_plj_nve_known_answer(precision):
+    f = _fixture()
+    with _ctx(precision, f, f["x1"]) as c:
+        c.set_velocities(f["v1"])
+        c.integrator_vv(float(f["dt"]))
+        E = np.vstack([c.run(99, each=99), c.run(4900, each=100)])
+    ref = f["energy"]
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.v_verlet.dat")
+    unit = 10.0 ** (np.floor(np.log10(np.abs(kat))) - 6)
+    print("%s: epot to step 2000 max rel %.2e; last-digit units to step 1800 max %.2f; "
+          "total over 50 rows max rel %.2e"
+          % ("fp64" if precision == hip.GH_FP64 else "mixed",
+             np.max(np.abs(E[:20, 0] - ref[:20, 0]) / np.abs(ref[:20, 0])),
+             np.max(np.abs(E[:18] - kat[:18]) / unit[:18]),
+             np.max(np.abs(E[:, 2] - kat[:, 2]) / np.abs(kat[:, 2]))))
+    print("total minus the example file's, ueV:", np.round((E[:, 2] - kat[:, 2]) * 1e6, 1))
+    if precision == hip.GH_FP64:
+        assert np.all(np.abs(E[:20, 0] - ref[:20, 0]) <= 1e-6 * np.abs(ref[:20, 0]))
+        assert np.all(np.abs(E[:20, 1:] - ref[:20, 1:]) <= 1e-9 * np.abs(ref[:20, 1:]))
+        assert np.all(np.abs(E[:18] - kat[:18]) <= 0.6 * unit[:18])
+    else:
+        assert np.all(np.abs(E[:18, 2] - kat[:18, 2]) <= 1e-6 * np.abs(kat[:18, 2]))
+        assert np.all(np.abs(E[:18, 0] - kat[:18, 0]) <= 1e-5)
+    assert abs(np.median(E[-10:, 2]) - kat[0, 2]) <= 2e-5 * abs(kat[0, 2])

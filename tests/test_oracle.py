@@ -344,3 +344,97 @@ def test_fire_reaches_the_reference_lbfgs_minimum():
     F, _, ep, _ = O.smatb_force(r["x"], np.zeros(13), O.make_box(None), s, nn, lst)
     assert np.sqrt((F ** 2).sum(axis=1)).max() * cm.UI_EV < 1e-5
     assert ep * cm.UI_EV == pytest.approx(r["epot"], rel=1e-13)
+
+
+# --------------------------------------------------------------------------
+# shifted Lennard-Jones (plj, Pairs.F90:282-370)
+# --------------------------------------------------------------------------
+def _plj_energy(x, types, box, p):
+    nn, lst, _ = O.neighbours(x, box, p.rc + 1.0)
+    F, _, ep = O.plj_force(x, types, box, p, nn, lst)
+    return F, ep
+
+
+def test_plj_pair_closed_form():
+    # two atoms: p(r) = 4e((s/r)^12 - (s/r)^6) - prc - frc (r - rc) with
+    # prc = u(rc) and frc = 24e(2 s^12/rc^13 - s^6/rc^7) = -u'(rc)
+    # (Pairs.F90:282-291, 313-314), i.e. p = u(r) - u(rc) + u'(rc)(r - rc):
+    # the energy vanishes at the cut, the force there is -2 u'(rc) (the
+    # linear term's sign, reproduced as written)
+    e, s, rc = 0.0037, 2.6, 10.0
+    p = O.plj_params([[e]], [[s]], rc)
+    box = O.make_box(None)
+    types = np.zeros(2, np.int32)
+
+    def u(r):
+        return 4 * e * ((s / r) ** 12 - (s / r) ** 6)
+
+    def du(r):
+        return 4 * e * (-12 * s ** 12 / r ** 13 + 6 * s ** 6 / r ** 7)
+
+    for r in (2.4, 3.0, 5.5, 9.999):
+        x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        F, ep = _plj_energy(x, types, box, p)
+        want = u(r) - u(rc) + du(rc) * (r - rc)
+        assert ep * cm.UI_EV == pytest.approx(want, rel=1e-10, abs=1e-17)
+        assert F[1, 0] * cm.UI_EV == pytest.approx(-(du(r) + du(rc)), rel=1e-10, abs=1e-15)
+        assert F[0, 0] == pytest.approx(-F[1, 0], rel=1e-14)
+    x = np.array([[0.0, 0.0, 0.0], [rc * (1 - 1e-9), 0.0, 0.0]])
+    F, ep = _plj_energy(x, types, box, p)
+    assert abs(ep) * cm.UI_EV < 3 * abs(du(rc)) * rc * 1e-9
+    assert F[1, 0] * cm.UI_EV == pytest.approx(-2 * du(rc), rel=1e-6)
+
+
+def test_plj_force_is_minus_gradient():
+    # two types with their own (e, s); F = -dE/dx by central differences
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0, 12.0, (40, 3))
+    keep = [0]
+    for i in range(1, len(x)):  # no pair closer than 2.3 A
+        if np.min(np.linalg.norm(x[keep] - x[i], axis=1)) > 2.3:
+            keep.append(i)
+    x = x[keep]
+    types = (np.arange(len(x)) % 2).astype(np.int32)
+    p = O.plj_params([[0.002, 0.004], [0.004, 0.007]], [[2.2, 2.4], [2.4, 2.6]], 10.0)
+    box = O.make_box(None)
+    F, _ = _plj_energy(x, types, box, p)
+    h = 1e-6
+    for (i, d) in [(0, 0), (3, 1), (len(x) // 2, 2), (len(x) - 1, 0)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        g = (_plj_energy(xp, types, box, p)[1] - _plj_energy(xm, types, box, p)[1]) / (2 * h)
+        assert -g == pytest.approx(F[i, d], rel=1e-6, abs=1e-9)
+    assert np.abs(F.sum(axis=0)).max() < 1e-10 * np.abs(F).max() * len(x)
+
+
+def plj_nve_run(f, md_factory):
+    """Energy rows at steps 100..5000 of the NVE example from its step-1
+    state: 99 steps to the first row, then 4900 with a row every 100."""
+    md = md_factory()
+    return np.vstack([md.run(99, each=99), md.run(4900, each=100)])
+
+
+def test_reference_kat_nve_plj():
+    """examples/NVE/v_verlet.gms (100 He + 100 Ne, shifted LJ, VV) from the
+    reference's state after step 1 (tests/golden/nve_plj.npz): the oracle
+    against the reference's 16-digit columns of the same run and against the
+    example's own ref/Energy.v_verlet.dat (7 digits).  The dilute gas is
+    chaotic: the 16-digit rounding of the step-1 state grows to 1e-8 in epot
+    by step 2000 and past the example's 7th digit after step 1800, so rows
+    are compared in the window that rounding leaves; the total energy over
+    all 50 rows."""
+    f = np.load(cm.GOLDEN + "/nve_plj.npz")
+    t = f["types"]
+    box = O.make_box([float(f["L"])] * 3, (1, 1, 1))
+    p = O.plj_params(f["eps"], f["sigma"], float(f["rc"]))
+    E = plj_nve_run(f, lambda: O.MDRun(f["x1"], f["v1"], f["mass"][t], box, pot="plj", plj=p,
+                                       types=t, skin=float(f["skin"]), integ="vv",
+                                       dt=float(f["dt"])))
+    ref = f["energy"]
+    assert np.all(np.abs(E[:20, 0] - ref[:20, 0]) <= 1e-6 * np.abs(ref[:20, 0]))
+    assert np.all(np.abs(E[:20, 1:] - ref[:20, 1:]) <= 1e-9 * np.abs(ref[:20, 1:]))
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.v_verlet.dat")
+    unit = 10.0 ** (np.floor(np.log10(np.abs(kat))) - 6)  # last printed digit
+    assert np.all(np.abs(E[:18] - kat[:18]) <= 0.6 * unit[:18])
+    assert np.all(np.abs(E[:, 2] - kat[:, 2]) <= 1e-5 * np.abs(kat[:, 2]))
