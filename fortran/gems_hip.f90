@@ -16,6 +16,7 @@
 !   gh_integrator_ermak evolve ermak T g      src/Integration.F90:909-1031
 !   gh_run              dinamica n            src/Programs.f90:114-167
 !   gh_prng_lcg         prng lcg / seed       src/Random.F90:137-172
+!   gh_get/set_lcg_state lcg chp record       src/Random.F90:178-190
 !   gh_set_gdist        set tempgdist T       src/Set_Properties.f90:301-377
 module gems_hip
   use iso_c_binding
@@ -147,6 +148,18 @@ module gems_hip
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
       integer(c_int), value :: seed
+    end function
+
+    integer(c_int) function gh_get_lcg_state(ctx, state) bind(C, name='gh_get_lcg_state')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: state(3)
+    end function
+
+    integer(c_int) function gh_set_lcg_state(ctx, state) bind(C, name='gh_set_lcg_state')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: state(3)
     end function
 
     integer(c_int) function gh_set_gdist(ctx, T) bind(C, name='gh_set_gdist')

@@ -946,6 +946,23 @@ int gh_prng_lcg(gh_ctx *c, int seed) {
   return 0;
 }
 
+int gh_get_lcg_state(gh_ctx *c, int state[3]) {
+  if (!c || !state) return GH_EINVAL;
+  state[0] = c->lcg.idum;
+  state[1] = c->lcg.iy;
+  state[2] = c->lcg.ix;
+  return 0;
+}
+
+int gh_set_lcg_state(gh_ctx *c, const int state[3]) {
+  if (!c || !state) return GH_EINVAL;
+  c->lcg.idum = state[0];
+  c->lcg.iy = state[1];
+  c->lcg.ix = state[2];
+  c->lcg.stored = false;
+  return 0;
+}
+
 // set_gdist + set_scal_vel + CM removal (Set_Properties.f90:301-377).
 int gh_set_gdist(gh_ctx *c, double T) {
   if (!c || !c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms");

@@ -20,7 +20,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 SYMBOLS = [
     "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box",
     "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
-    "gh_set_lj", "gh_set_plj",
+    "gh_set_lj", "gh_set_plj", "gh_get_lcg_state", "gh_set_lcg_state",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
     "gh_get_stats", "gh_set_timing", "gh_get_timing", "gh_set_tile", "gh_autotune",
@@ -77,6 +77,8 @@ def load():
     L.gh_compute.argtypes = [vp, _dp, _dp, _dp]
     L.gh_neighbors.argtypes = [vp, _ip, _ip, C.c_int]
     L.gh_prng_lcg.argtypes = [vp, C.c_int]
+    L.gh_get_lcg_state.argtypes = [vp, _ip]
+    L.gh_set_lcg_state.argtypes = [vp, _ip]
     L.gh_set_gdist.argtypes = [vp, C.c_double]
     L.gh_integrator_vv.argtypes = [vp, C.c_double]
     L.gh_integrator_ermak.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -227,6 +229,16 @@ class Context:
 
     def prng_lcg(self, seed):
         self._ck(self.L.gh_prng_lcg(self.h, seed))
+
+    def lcg_state(self, state=None):
+        """The lcg stream's (idum, iy, ix) as a checkpoint stores it; with
+        `state`, sets it first."""
+        st = np.zeros(3, np.int32)
+        if state is not None:
+            st[:] = state
+            self._ck(self.L.gh_set_lcg_state(self.h, _p(st, _ip)))
+        self._ck(self.L.gh_get_lcg_state(self.h, _p(st, _ip)))
+        return tuple(int(v) for v in st)
 
     def set_gdist(self, T):
         self._ck(self.L.gh_set_gdist(self.h, T))

@@ -141,6 +141,13 @@ int gh_neighbors(gh_ctx *ctx, int *nn, int *idx, int cap);
 
 /* Random stream: `prng lcg` + `prng seed s` (Random.F90:137-172). */
 int gh_prng_lcg(gh_ctx *ctx, int seed);
+
+/* The `prng lcg` stream's state as a checkpoint stores it: (idum, iy, ix),
+ * the record lcg_write_chpseed writes and lcg_read_chpseed reads
+ * (Random.F90:178-190).  Setting it resumes the reference's stream where a
+ * checkpoint left it (draws come in pairs, so no cached normal is pending). */
+int gh_get_lcg_state(gh_ctx *ctx, int state[3]);
+int gh_set_lcg_state(gh_ctx *ctx, const int state[3]);
 /* `set tempgdist T` on all atoms (set_gdist + set_scal_vel,
  * Set_Properties.f90:301-377), drawing from the lcg stream. */
 int gh_set_gdist(gh_ctx *ctx, double T);

@@ -389,10 +389,36 @@ def nve_plj_main(ref_dir):
         shutil.rmtree(wd)
 
 
+def nve_plj_checkpoints(ref_dir):
+    """chp_plj_s100.chp and chp_plj_s200.chp: the reference's checkpoints
+    (`checkpoint write`) of the NVE example at steps 100 and 200 of one run;
+    xyz_plj_s100.xyz and energy_plj.dat: its position frame at step 100 and
+    its energy columns (default format) every 50 steps, for the format
+    readers and writers."""
+    wd = tempfile.mkdtemp(prefix="gold_")
+    try:
+        outs = ("outfile :x name X.xyz\noutfile :x pos 1\noutfile :x each 100\n"
+                "outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each 50\n")
+        sc, _, _ = nve_plj_script(ref_dir, 100, outs)
+        sc = sc.replace("out float 25.16\n", "")
+        sc += "checkpoint write s100.chp\ndinamica 100\ncheckpoint write s200.chp\n"
+        run(wd, sc)
+        shutil.copy(os.path.join(wd, "s100.chp"), os.path.join(HERE, "chp_plj_s100.chp"))
+        shutil.copy(os.path.join(wd, "s200.chp"), os.path.join(HERE, "chp_plj_s200.chp"))
+        frames = open(os.path.join(wd, "X.xyz")).read().splitlines()
+        with open(os.path.join(HERE, "xyz_plj_s100.xyz"), "w") as f:
+            f.write("\n".join(frames[:202]) + "\n")
+        shutil.copy(os.path.join(wd, "E.dat"), os.path.join(HERE, "energy_plj.dat"))
+        print("nve_plj checkpoints written")
+    finally:
+        shutil.rmtree(wd)
+
+
 if __name__ == "__main__":
     if "--drift" in sys.argv:
         drift_main()
     elif "--plj" in sys.argv:
         nve_plj_main(sys.argv[sys.argv.index("--plj") + 1])
+        nve_plj_checkpoints(sys.argv[sys.argv.index("--plj") + 1])
     else:
         main()

@@ -26,6 +26,7 @@ through the median).
 import numpy as np
 import pytest
 
+from gems_amd import formats as FM
 from oracle import oracle as O
 from tests import common as cm
 
@@ -112,3 +113,42 @@ def test_plj_nve_known_answer(precision):
         assert np.all(np.abs(E[:18, 2] - kat[:18, 2]) <= 1e-6 * np.abs(kat[:18, 2]))
         assert np.all(np.abs(E[:18, 0] - kat[:18, 0]) <= 1e-5)
     assert abs(np.median(E[-10:, 2]) - kat[0, 2]) <= 2e-5 * abs(kat[0, 2])
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_restart_from_the_reference_checkpoint(precision):
+    """The reference's `checkpoint write` at step 100 of the NVE example
+    (tests/golden/chp_plj_s100.chp) loaded into a context (positions,
+    velocities, box, the lcg stream's state): 100 more steps reach the
+    reference's own checkpoint at step 200 of the same run (positions within
+    1e-9 A FP64 / 1e-4 A mixed, minimum image) and the energy row at step 200
+    of its 16-digit trace; the context's checkpoint reads back as written."""
+    f = _fixture()
+    ck100 = FM.read_chp(cm.GOLDEN + "/chp_plj_s100.chp")
+    ck200 = FM.read_chp(cm.GOLDEN + "/chp_plj_s200.chp")
+    L = float(f["L"])
+    with hip.Context(0, precision) as c:
+        FM.load_checkpoint(c, ck100, f["types"])
+        assert c.lcg_state() == ck100["lcg"]
+        c.set_masses(f["mass"])
+        c.set_plj(f["eps"], f["sigma"], float(f["rc"]))
+        c.set_skin(float(f["skin"]))
+        c.integrator_vv(float(f["dt"]))
+        en = c.run(100, each=100)
+        x, v, _, _ = c.state()
+        ck = FM.checkpoint_of(c, [L] * 3, f["mass"][f["types"]], time=ck200["time"])
+    d = x - ck200["pos"]
+    d -= L * np.round(d / L)
+    xtol, etol = (1e-9, 1e-9) if precision == hip.GH_FP64 else (1e-4, 1e-6)
+    print("restart: max |dx| %.2e A, max |dv| %.2e A/ps, total energy rel diff %.2e"
+          % (np.abs(d).max(), np.abs(v - ck200["vel"]).max(),
+             abs(en[0, 2] / f["energy"][1, 2] - 1)))
+    assert np.abs(d).max() <= xtol
+    assert np.abs(v - ck200["vel"]).max() <= xtol * 100
+    assert abs(en[0, 2] / f["energy"][1, 2] - 1) <= etol
+    with __import__("tempfile").TemporaryDirectory() as tmp:
+        path = tmp + "/mine.chp"
+        FM.write_chp(path, ck)
+        back = FM.read_chp(path)
+    assert back["nlocal"] == 200 and np.array_equal(back["pos"], x)
+    assert back["lcg"] == ck100["lcg"]  # velocity Verlet draws nothing
