@@ -9,6 +9,8 @@
 !   gh_set_smatb        smatb_set/readprm     src/TB.F90:106-170, 291-365
 !   gh_set_lj           lj_set                src/Pairs.F90:198-205
 !   gh_set_plj          plj_set               src/Pairs.F90:282-291
+!   gh_set_slj          slj_set               src/Pairs.F90:374-401
+!   gh_set_wca          wca_set               src/Pairs.F90:495-506
 !   gh_set_skin         nb_dcut               src/Neighbour.F90:237
 !   gh_set_box          box_setvars/set_pbc   src/Program_Types.F90:126-141
 !   gh_neighbors        ig%nn, ig%list        src/Neighbour.F90:120-121
@@ -108,6 +110,20 @@ module gems_hip
       integer(c_int), value :: nt
       real(c_double), intent(in) :: eps(nt, nt), sigma(nt, nt)
       real(c_double), value :: rc
+    end function
+
+    integer(c_int) function gh_set_slj(ctx, nt, eps, sigma, rs) bind(C, name='gh_set_slj')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: eps(nt, nt), sigma(nt, nt), rs(nt, nt)
+    end function
+
+    integer(c_int) function gh_set_wca(ctx, nt, eps, sigma) bind(C, name='gh_set_wca')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: eps(nt, nt), sigma(nt, nt)
     end function
 
     integer(c_int) function gh_set_skin(ctx, skin) bind(C, name='gh_set_skin')

@@ -218,7 +218,7 @@ size_t gh_tile_slot_bytes(int pass, int precision, int nt);
 size_t gh_tile_static_bytes(int precision, int nt);
 size_t gh_pair_slot_bytes(int pot);
 static int max_slots(const gh_ctx *c) {
-  const bool pair = c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ;
+  const bool pair = c->pot == GH_POT_LJ || gh_is_table_pot(c->pot);
   const size_t per = pair ? gh_pair_slot_bytes(c->pot)
                           : std::max(gh_tile_slot_bytes(0, c->precision, c->nt),
                                      gh_tile_slot_bytes(1, c->precision, c->nt));
@@ -470,6 +470,56 @@ static int rec(gh_ctx *c, int k) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// The nt x nt table of a tabulated Lennard-Jones form: validates symmetry,
+// fills each pair with fill(entry, a, b), uploads, and selects the form.
+template <class FILL>
+static int set_pair_table(gh_ctx *c, int pot, int nt, const double *eps, const double *sigma,
+                          const char *who, FILL &&fill) {
+  if (!c || nt <= 0 || nt > GH_MAXT || !eps || !sigma)
+    return gh_fail(c, GH_EINVAL, std::string(who) + ": bad arguments");
+  PljDev h[GH_MAXT * GH_MAXT];
+  memset(h, 0, sizeof(h));
+  double rmax = 0.0;
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const double e = eps[a * nt + b], sg = sigma[a * nt + b];
+      if (!(sg > 0) || e != eps[b * nt + a] || sg != sigma[b * nt + a])
+        return gh_fail(c, GH_EINVAL,
+                       std::string(who) + ": eps and sigma must be symmetric, sigma > 0");
+      PljDev &d = h[a * GH_MAXT + b];
+      d.e = e;
+      d.s = sg;
+      if (!fill(d, a, b)) return gh_fail(c, GH_EINVAL, std::string(who) + ": bad arguments");
+      d.rc2 = d.rc * d.rc;
+      d.ef = (float)e;
+      d.s2f = (float)(sg * sg);
+      d.prcf = (float)d.prc;
+      d.frcf = (float)d.frc;
+      d.rsf = (float)d.rs;
+      d.sa4f = (float)d.sa4;
+      d.sa8f = (float)d.sa8;
+      d.rcf = (float)d.rc;
+      rmax = std::max(rmax, d.rc);
+    }
+  GH_HIP(c, hipSetDevice(c->device));
+  GH_TRY(dalloc(c, &c->pljd, GH_MAXT * GH_MAXT));
+  GH_HIP(c, hipMemcpyAsync(c->pljd, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  c->nt = nt;
+  c->lj_eps = 0.0;
+  c->lj_sigma = 0.0;
+  c->lj_rc = rmax;
+  c->pot = pot;
+  c->rcut = rmax;
+  c->ready = false;
+  return 0;
+}
+
+static double pow6(double x) {  // x**6 as the reference's compiled integer power
+  const double x2 = x * x, x4 = x2 * x2;
+  return x4 * x2;
+}
+
 extern "C" {
 
 int gh_create(gh_ctx **out, int device, int precision) {
@@ -822,38 +872,40 @@ int gh_set_lj(gh_ctx *c, double eps, double sigma, double rc) {
 }
 
 int gh_set_plj(gh_ctx *c, int nt, const double *eps, const double *sigma, double rc) {
-  if (!c || nt <= 0 || nt > GH_MAXT || !eps || !sigma || !(rc > 0))
-    return gh_fail(c, GH_EINVAL, "gh_set_plj: bad arguments");
-  PljDev h[GH_MAXT * GH_MAXT];
-  memset(h, 0, sizeof(h));
-  for (int a = 0; a < nt; ++a)
-    for (int b = 0; b < nt; ++b) {
-      const double e = eps[a * nt + b], sg = sigma[a * nt + b];
-      if (!(sg > 0) || e != eps[b * nt + a] || sg != sigma[b * nt + a])
-        return gh_fail(c, GH_EINVAL, "gh_set_plj: eps and sigma must be symmetric, sigma > 0");
-      // plj_set (Pairs.F90:282-291)
-      PljDev &d = h[a * GH_MAXT + b];
-      d.e = e;
-      d.s = sg;
-      d.prc = 4.0 * e * (pow(sg / rc, 12) - pow(sg / rc, 6));
-      d.frc = 24.0 * e * (2.0 * (pow(sg, 12) / pow(rc, 13)) - (pow(sg, 6) / pow(rc, 7)));
-      d.ef = (float)e;
-      d.s2f = (float)(sg * sg);
-      d.prcf = (float)d.prc;
-      d.frcf = (float)d.frc;
-    }
-  GH_HIP(c, hipSetDevice(c->device));
-  GH_TRY(dalloc(c, &c->pljd, GH_MAXT * GH_MAXT));
-  GH_HIP(c, hipMemcpyAsync(c->pljd, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-  GH_HIP(c, hipStreamSynchronize(c->stream));
-  c->nt = nt;
-  c->lj_eps = 0.0;
-  c->lj_sigma = 0.0;
-  c->lj_rc = rc;
-  c->pot = GH_POT_PLJ;
-  c->rcut = rc;
-  c->ready = false;
-  return 0;
+  if (!(rc > 0)) return gh_fail(c, GH_EINVAL, "gh_set_plj: bad arguments");
+  return set_pair_table(c, GH_POT_PLJ, nt, eps, sigma, "gh_set_plj", [&](PljDev &d, int, int) {
+    // plj_set (Pairs.F90:282-291)
+    d.prc = 4.0 * d.e * (pow(d.s / rc, 12) - pow(d.s / rc, 6));
+    d.frc = 24.0 * d.e * (2.0 * (pow(d.s, 12) / pow(rc, 13)) - (pow(d.s, 6) / pow(rc, 7)));
+    d.rc = rc;
+    return true;
+  });
+}
+
+int gh_set_slj(gh_ctx *c, int nt, const double *eps, const double *sigma, const double *rs) {
+  if (!rs) return gh_fail(c, GH_EINVAL, "gh_set_slj: bad arguments");
+  return set_pair_table(c, GH_POT_SLJ, nt, eps, sigma, "gh_set_slj", [&](PljDev &d, int a, int b) {
+    // slj_set (Pairs.F90:374-401)
+    const double r = rs[a * nt + b];
+    if (!(r > 0) || r != rs[b * nt + a]) return false;
+    const double s_dr = pow6(d.s / r);
+    const double urs = 4.0 * d.e * (s_dr * s_dr - s_dr);
+    const double dudrs = -d.e * (48.0 * s_dr - 24.0) * s_dr / r;
+    d.rs = r;
+    d.rc = r - 32.0 * urs / (11.0 * dudrs);
+    const double dd = d.rc - r, d2 = dd * dd, d4 = d2 * d2, d8 = d4 * d4;
+    d.sa4 = (8.0 * urs + dd * dudrs) / (4 * d4);
+    d.sa8 = -(4.0 * urs + dd * dudrs) / (4 * d8);
+    return d.rc > r;
+  });
+}
+
+int gh_set_wca(gh_ctx *c, int nt, const double *eps, const double *sigma) {
+  return set_pair_table(c, GH_POT_WCA, nt, eps, sigma, "gh_set_wca", [&](PljDev &d, int, int) {
+    // wca_set (Pairs.F90:495-506): 2._dp**(1./6.), the exponent a default-real literal
+    d.rc = pow(2.0, (double)(1.0f / 6.0f)) * d.s;
+    return true;
+  });
 }
 
 int gh_set_skin(gh_ctx *c, double skin) {

@@ -264,25 +264,46 @@ __device__ __forceinline__ bool tb_force_fx(const TbF &P, double r2q, float ebs,
   return true;
 }
 
-// Shifted Lennard-Jones pair (plj_interact, Pairs.F90:293-370): energy p and
-// f/r with f = -dp/dr, for r^2 within the cut; FP64 as the reference's
-// arithmetic, FP32 with the hardware reciprocal square root.
-__device__ __forceinline__ void plj_pair64(const PljDev &P, double rc, double r2, double &p,
-                                           double &f_r) {
+// Tabulated Lennard-Jones pairs: energy p and f/r with f = -dp/dr, for r^2
+// within the pair's cut.  POT: GH_POT_PLJ (plj_interact, Pairs.F90:293-370),
+// GH_POT_SLJ (slj_interact, :403-493), GH_POT_WCA (wca_interact, :508-560).
+// FP64 follows the reference's arithmetic; FP32 uses the hardware
+// reciprocal square root.
+template <int POT>
+__device__ __forceinline__ void table_pair64(const PljDev &P, double r2, double &p,
+                                             double &f_r) {
   const double dr = sqrt(r2);
-  const double t = P.s / dr;
-  const double t2 = t * t, t4 = t2 * t2, s6 = t4 * t2;  // (s/dr)**6
-  p = 4.0 * P.e * (s6 * s6 - s6) - P.prc - P.frc * (dr - rc);
-  f_r = (P.e * (48.0 * s6 - 24.0) * s6 / dr + P.frc) / dr;
+  double f;
+  if (POT == 4 && !(dr < P.rs)) {
+    const double d = dr - P.rc, d2 = d * d, s4 = d2 * d2;  // (dr - rcut)**4
+    p = P.sa4 * s4 + P.sa8 * s4 * s4;
+    f = -(4.0 * P.sa4 * s4 + 8.0 * P.sa8 * s4 * s4) / (dr - P.rc);
+  } else {
+    const double t = P.s / dr;
+    const double t2 = t * t, t4 = t2 * t2, s6 = t4 * t2;  // (s/dr)**6
+    if (POT == 3) p = 4.0 * P.e * (s6 * s6 - s6) - P.prc - P.frc * (dr - P.rc);
+    else if (POT == 4) p = 4.0 * P.e * (s6 - 1.0) * s6;
+    else p = 4.0 * P.e * (s6 * s6 - s6) + P.e;
+    f = P.e * (48.0 * s6 - 24.0) * s6 / dr + (POT == 3 ? P.frc : 0.0);
+  }
+  f_r = f / dr;
 }
-__device__ __forceinline__ void plj_pair32(const PljDev &P, float rcf, float r2f, float &p,
-                                           float &f_r) {
+template <int POT>
+__device__ __forceinline__ void table_pair32(const PljDev &P, float r2f, float &p, float &f_r) {
   const float ir = __builtin_amdgcn_rsqf(r2f);
   const float dr = r2f * ir, ir2 = ir * ir;
-  const float s2 = P.s2f * ir2;
-  const float s6 = s2 * s2 * s2;
-  p = 4.0f * P.ef * (s6 * s6 - s6) - P.prcf - P.frcf * (dr - rcf);
-  f_r = fmaf(P.ef * fmaf(48.0f, s6, -24.0f), s6, P.frcf * dr) * ir2;
+  if (POT == 4 && !(dr < P.rsf)) {
+    const float d = dr - P.rcf, d2 = d * d, s4 = d2 * d2;
+    p = fmaf(P.sa8f * s4, s4, P.sa4f * s4);
+    f_r = -fmaf(8.0f * P.sa8f * s4, s4, 4.0f * P.sa4f * s4) / d * ir;
+  } else {
+    const float s2 = P.s2f * ir2;
+    const float s6 = s2 * s2 * s2;
+    if (POT == 3) p = 4.0f * P.ef * (s6 * s6 - s6) - P.prcf - P.frcf * (dr - P.rcf);
+    else if (POT == 4) p = 4.0f * P.ef * (s6 - 1.0f) * s6;
+    else p = 4.0f * P.ef * (s6 * s6 - s6) + P.ef;
+    f_r = fmaf(P.ef * fmaf(48.0f, s6, -24.0f), s6, POT == 3 ? P.frcf * dr : 0.0f) * ir2;
+  }
 }
 
 // Philox4x32-10 (Salmon et al., SC'11): counter-based stream.

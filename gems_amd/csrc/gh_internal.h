@@ -73,13 +73,15 @@ struct TbF {
   // r0q, rciq, rceq = r0f, rci_f, rce_f * 2^23; rep_bq, rho_bq = rep_b, rho_b * 2^-23
   float r0q, rciq, rceq, rep_bq, rho_bq, pad;
 };
-// Shifted Lennard-Jones (plj) per type pair: prc and frc are the value and
-// the slope of the plain 12-6 form at the cut (plj_set, Pairs.F90:282-291);
-// FP64 and FP32 forms.
+// Tabulated Lennard-Jones forms per type pair (plj, slj, wca; the context's
+// pot says which): prc and frc are plj's value and slope terms at the cut
+// (plj_set, Pairs.F90:282-291); rs, sa4, sa8 slj's spline (slj_set,
+// Pairs.F90:374-401); rc this pair's cut.  FP64 and FP32 forms.
 struct PljDev {
-  double e, s, prc, frc;
-  float ef, s2f, prcf, frcf;
+  double e, s, prc, frc, rs, sa4, sa8, rc, rc2;
+  float ef, s2f, prcf, frcf, rsf, sa4f, sa8f, rcf;
 };
+inline bool gh_is_table_pot(int pot) { return pot >= 3 && pot <= 5; }
 
 #define GH_FXS 8388608.0         // fixed-point steps per Angstrom (2^23)
 #define GH_FXS2 70368744177664.0 // GH_FXS^2 (2^46)

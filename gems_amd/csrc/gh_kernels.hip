@@ -408,10 +408,10 @@ __global__ __launch_bounds__(GH_BLOCK) void k_lj64(
   epa[i] = ei * 0.5 * ev_ui;
 }
 
-// Shifted Lennard-Jones (plj_interact, Pairs.F90:293-370) on the O(N^2)-mode
-// list, newton off: per type pair p = 4e(s6^2 - s6) - prc - frc (r - rc),
-// f = e(48 s6 - 24) s6 / r + frc; e_i += p/2; F_i -= f vd / r.
-template <bool MIXED>
+// Tabulated Lennard-Jones forms (plj_interact, Pairs.F90:293-370;
+// slj_interact, :403-493; wca_interact, :508-560) on the O(N^2)-mode list,
+// newton off, per type pair: e_i += p/2; F_i -= f vd / r.
+template <bool MIXED, int POT>
 __global__ __launch_bounds__(GH_BLOCK) void k_plj(
     const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
     const unsigned *__restrict__ list, int ld, int n, int ncap, const PljDev *__restrict__ pljd,
@@ -426,8 +426,6 @@ __global__ __launch_bounds__(GH_BLOCK) void k_plj(
   if (i >= n) return;
   const double4 pi = pos[i];
   const int ti = type[i], m = nn[i];
-  const double rc = sqrt(rc2);
-  const float rcf = (float)rc;
   double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
   for (int k = 0; k < m; ++k) {
     const unsigned e = ld_stream(&list[(size_t)k * ld + i]);
@@ -440,14 +438,15 @@ __global__ __launch_bounds__(GH_BLOCK) void k_plj(
     const double r2 = MIXED ? fma(vx, vx, fma(vy, vy, vz * vz)) : dist2_rn(vx, vy, vz);
     if (r2 > rc2) continue;
     const PljDev &P = pp[ti * GH_MAXT + type[j]];
+    if (r2 > P.rc2) continue;  // this pair's cut
     double p, sf;
     if (MIXED) {
       float pf, fr;
-      plj_pair32(P, rcf, (float)r2, pf, fr);
+      table_pair32<POT>(P, (float)r2, pf, fr);
       p = (double)pf;
       sf = (double)fr;
     } else {
-      plj_pair64(P, rc, r2, p, sf);
+      table_pair64<POT>(P, r2, p, sf);
     }
     ei += p;
     fx -= sf * vx;
@@ -890,7 +889,7 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
 
 bool gh_can_fuse_vv(const gh_ctx *c) {
   return c->list_mode == 1 && c->integ == GH_INT_VV &&
-         (c->pot == GH_POT_SMATB || c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ);
+         (c->pot == GH_POT_SMATB || c->pot == GH_POT_LJ || gh_is_table_pot(c->pot));
 }
 
 int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
@@ -955,16 +954,25 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     else
       k_lj64<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
                                      c->lj_sigma, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
-  } else if (c->pot == GH_POT_PLJ) {
+  } else if (gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
-    if (mixed)
-      k_plj<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->ncap,
-                                                 c->pljd, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
-    else
-      k_plj<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->ncap,
-                                                  c->pljd, rc2, L0, L1, L2, ev_ui, c->frc,
-                                                  c->epa);
+#define PLJ_LAUNCH(MX, PT)                                                                  \
+  k_plj<MX, PT><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->ncap, \
+                                               c->pljd, rc2, L0, L1, L2, ev_ui, c->frc, c->epa)
+#define PLJ_POTS(MX)                                                                        \
+  switch (c->pot) {                                                                         \
+    case GH_POT_SLJ: PLJ_LAUNCH(MX, GH_POT_SLJ); break;                                     \
+    case GH_POT_WCA: PLJ_LAUNCH(MX, GH_POT_WCA); break;                                     \
+    default: PLJ_LAUNCH(MX, GH_POT_PLJ);                                                    \
+  }
+    if (mixed) {
+      PLJ_POTS(true)
+    } else {
+      PLJ_POTS(false)
+    }
+#undef PLJ_POTS
+#undef PLJ_LAUNCH
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");
   }

@@ -1041,7 +1041,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
 }
 
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
-template <bool MIXED, bool PLJ>
+// POT: GH_POT_LJ (eps, sigma as arguments) or a tabulated form (plj, slj,
+// wca: per type pair, types staged in LDS).
+template <bool MIXED, int POT>
 __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
@@ -1050,8 +1052,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     const PljDev *__restrict__ pljd) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
-  __shared__ PljDev pp[PLJ ? GH_MAXT * GH_MAXT : 1];
-  if (PLJ)
+  constexpr bool TAB = POT != GH_POT_LJ;
+  __shared__ PljDev pp[TAB ? GH_MAXT * GH_MAXT : 1];
+  if (TAB)
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) pp[k] = pljd[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
@@ -1059,7 +1062,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
   const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
   unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * cap);
-  unsigned char *ty = scell + cap;  // plj: the slots' types
+  unsigned char *ty = scell + cap;  // tabulated forms: the slots' types
   slot_map(H, scell);
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
@@ -1068,16 +1071,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     px[s] = p.x + shift(t, H, 0, h);
     py[s] = p.y + shift(t, H, 1, h);
     pz[s] = p.z + shift(t, H, 2, h);
-    if (PLJ) ty[s] = (unsigned char)type[src];
+    if (TAB) ty[s] = (unsigned char)type[src];
   }
   if (threadIdx.x == 0) {
     px[t.slot_cap] = py[t.slot_cap] = pz[t.slot_cap] = GH_FAR;
-    if (PLJ) ty[t.slot_cap] = 0;
+    if (TAB) ty[t.slot_cap] = 0;
   }
   __syncthreads();
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
-  const double rc = sqrt(rc2);
-  const float rcf = (float)rc;
   double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pi = pos[a];
@@ -1089,9 +1090,11 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
         const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
         if (r2 <= rc2) {
           double sf;
-          if (PLJ) {
+          if (TAB) {
+            const PljDev &P = pp[ti * GH_MAXT + ty[s]];
+            if (r2 > P.rc2) return;  // this pair's cut
             float p, f_r;
-            plj_pair32(pp[ti * GH_MAXT + ty[s]], rcf, (float)r2, p, f_r);
+            table_pair32<POT>(P, (float)r2, p, f_r);
             ei += (double)p;
             sf = (double)f_r;
           } else {
@@ -1109,9 +1112,11 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
         const double r2 = dist2_rn(vx, vy, vz);
         if (r2 <= rc2) {
           double sf;
-          if (PLJ) {
+          if (TAB) {
+            const PljDev &P = pp[ti * GH_MAXT + ty[s]];
+            if (r2 > P.rc2) return;  // this pair's cut
             double p;
-            plj_pair64(pp[ti * GH_MAXT + ty[s]], rc, r2, p, sf);
+            table_pair64<POT>(P, r2, p, sf);
             ei += p;
           } else {
             const double dr = sqrt(r2);
@@ -1153,8 +1158,8 @@ size_t gh_tile_slot_bytes(int pass, int precision, int nt) {
   if (pass == 0) return 24 + 1 + (multi ? 4 : 0);  // rho: xyz + map (+ type)
   return 32 + 1 + (multi ? 4 : 0);                 // force: xyz + eband + map (+ type)
 }
-// Pair potentials (LJ, plj): FP64 xyz + map per slot (+ the type for plj).
-size_t gh_pair_slot_bytes(int pot) { return 25 + (pot == GH_POT_PLJ ? 1 : 0); }
+// Pair potentials: FP64 xyz + map per slot (+ the type for the tabulated forms).
+size_t gh_pair_slot_bytes(int pot) { return 25 + (gh_is_table_pot(pot) ? 1 : 0); }
 size_t gh_tile_static_bytes(int precision, int nt) {
   const size_t params = nt > 1 ? (precision == GH_MIXED ? sizeof(TbF) : sizeof(TbDev)) *
                                      GH_MAXT * GH_MAXT
@@ -1171,7 +1176,7 @@ static size_t build_slots() { return (GH_BUILD_LDS - build_far_bytes()) / 13; }
 // Bounds of the pair function's inner branch and of its cutoff over all type
 // pairs: rin = min rci, rout = max rce (SMA-TB); rin = rout = rc (LJ).
 static void inner_outer(const gh_ctx *c, double *rin, double *rout) {
-  if (c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ) {
+  if (c->pot == GH_POT_LJ || gh_is_table_pot(c->pot)) {
     *rin = *rout = c->lj_rc;
     return;
   }
@@ -1372,21 +1377,28 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
             c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv);
     }
-  } else if (c->pot == GH_POT_LJ || c->pot == GH_POT_PLJ) {
+  } else if (c->pot == GH_POT_LJ || gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
-    const bool plj = c->pot == GH_POT_PLJ;
     const size_t lds = gh_pair_slot_bytes(c->pot) * slots;
-#define LJ_LAUNCH(MX, PL)                                                                   \
-  k_lj_tiled<MX, PL><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,  \
+#define LJ_LAUNCH(MX, PT)                                                                   \
+  k_lj_tiled<MX, PT><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,  \
                                                   c->nn, c->list16, ld, c->ncap, c->lj_eps, \
                                                   c->lj_sigma, rc2, ev_ui, c->frc, c->epa,  \
                                                   c->type, fv, c->pljd)
+#define LJ_POTS(MX)                                                                         \
+  switch (c->pot) {                                                                         \
+    case GH_POT_PLJ: LJ_LAUNCH(MX, GH_POT_PLJ); break;                                      \
+    case GH_POT_SLJ: LJ_LAUNCH(MX, GH_POT_SLJ); break;                                      \
+    case GH_POT_WCA: LJ_LAUNCH(MX, GH_POT_WCA); break;                                      \
+    default: LJ_LAUNCH(MX, GH_POT_LJ);                                                      \
+  }
     if (mixed) {
-      if (plj) LJ_LAUNCH(true, true); else LJ_LAUNCH(true, false);
+      LJ_POTS(true)
     } else {
-      if (plj) LJ_LAUNCH(false, true); else LJ_LAUNCH(false, false);
+      LJ_POTS(false)
     }
+#undef LJ_POTS
 #undef LJ_LAUNCH
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");

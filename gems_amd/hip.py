@@ -20,7 +20,8 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 SYMBOLS = [
     "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box",
     "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
-    "gh_set_lj", "gh_set_plj", "gh_get_lcg_state", "gh_set_lcg_state",
+    "gh_set_lj", "gh_set_plj", "gh_set_slj", "gh_set_wca", "gh_get_lcg_state",
+    "gh_set_lcg_state",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
     "gh_get_stats", "gh_set_timing", "gh_get_timing", "gh_set_tile", "gh_autotune",
@@ -71,6 +72,8 @@ def load():
     L.gh_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
     L.gh_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.gh_set_plj.argtypes = [vp, C.c_int, _dp, _dp, C.c_double]
+    L.gh_set_slj.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    L.gh_set_wca.argtypes = [vp, C.c_int, _dp, _dp]
     L.gh_set_skin.argtypes = [vp, C.c_double]
     L.gh_set_tile.argtypes = [vp, _ip]
     L.gh_autotune.argtypes = [vp, C.c_int, _ip, _dp]
@@ -196,6 +199,19 @@ class Context:
         nt = e.shape[0]
         self._ck(self.L.gh_set_plj(self.h, nt, e.ctypes.data_as(_dp),
                                    sg.ctypes.data_as(_dp), rc))
+
+    def set_slj(self, eps, sigma, rs):
+        """Splined Lennard-Jones (slj, Pairs.F90:374-493): nt x nt eps (eV),
+        sigma and rs (A); the cut follows from them."""
+        e, sg, r = (np.ascontiguousarray(a, dtype=np.float64) for a in (eps, sigma, rs))
+        self._ck(self.L.gh_set_slj(self.h, e.shape[0], e.ctypes.data_as(_dp),
+                                   sg.ctypes.data_as(_dp), r.ctypes.data_as(_dp)))
+
+    def set_wca(self, eps, sigma):
+        """Weeks-Chandler-Andersen (wca, Pairs.F90:495-560): nt x nt eps, sigma."""
+        e, sg = (np.ascontiguousarray(a, dtype=np.float64) for a in (eps, sigma))
+        self._ck(self.L.gh_set_wca(self.h, e.shape[0], e.ctypes.data_as(_dp),
+                                   sg.ctypes.data_as(_dp)))
 
     def set_skin(self, skin):
         self._ck(self.L.gh_set_skin(self.h, skin))

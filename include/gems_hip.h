@@ -48,6 +48,8 @@ typedef struct gh_ctx gh_ctx;
 #define GH_POT_SMATB 1
 #define GH_POT_LJ 2
 #define GH_POT_PLJ 3
+#define GH_POT_SLJ 4
+#define GH_POT_WCA 5
 
 /* integrators */
 #define GH_INT_VV 0     /* velocity Verlet   (Integration.F90:1150-1186) */
@@ -101,6 +103,18 @@ int gh_set_lj(gh_ctx *ctx, double eps, double sigma, double rc);
  * directed pair adds half its energy to atom i, as the full-list build
  * does. */
 int gh_set_plj(gh_ctx *ctx, int nt, const double *eps, const double *sigma, double rc);
+
+/* Lennard-Jones with a splined tail (Gronbech-Jensen & Farago 2014):
+ * `interact ... slj e s rs` (slj_set / slj_interact, Pairs.F90:374-493).
+ * 12-6 below rs, sa4 (r-rc)^4 + sa8 (r-rc)^8 from rs to the cut rc, which
+ * slj_set derives from (e, s, rs).  eps, sigma, rs are nt x nt symmetric
+ * (eV, A, A). */
+int gh_set_slj(gh_ctx *ctx, int nt, const double *eps, const double *sigma, const double *rs);
+
+/* Weeks-Chandler-Andersen: 12-6 + e, cut at 2^(1/6) s (wca_set /
+ * wca_interact, Pairs.F90:495-560; the exponent 1/6 is a default-real
+ * literal there and here).  eps, sigma nt x nt symmetric. */
+int gh_set_wca(gh_ctx *ctx, int nt, const double *eps, const double *sigma);
 
 /* Verlet skin `nb_dcut` (Neighbour.F90:237), A. */
 int gh_set_skin(gh_ctx *ctx, double skin);

@@ -497,14 +497,52 @@ void or_plj_set(or_plj *p, int i, int j, double e, double s) {
     p->s[k[u]] = s;
     p->prc[k[u]] = prc;
     p->frc[k[u]] = frc;
+    p->rcp[k[u]] = rc;
   }
+}
+
+static double pow6(double x) {  /* x**6 as repeated multiplication */
+  double x2 = x * x, x4 = x2 * x2;
+  return x4 * x2;
+}
+
+void or_slj_set(or_plj *p, int i, int j, double e, double s, double rs) {
+  double s_dr = pow6(s / rs);
+  double urs = 4.0 * e * (s_dr * s_dr - s_dr);
+  double dudrs = -e * (48.0 * s_dr - 24.0) * s_dr / rs;
+  double rc = rs - 32.0 * urs / (11.0 * dudrs);
+  double d = rc - rs, d2 = d * d, d4 = d2 * d2, d8 = d4 * d4;
+  double sa4 = (8.0 * urs + d * dudrs) / (4 * d4);
+  double sa8 = -(4.0 * urs + d * dudrs) / (4 * d8);
+  int k[2] = {i * OR_MAXT + j, j * OR_MAXT + i};
+  for (int u = 0; u < 2; ++u) {
+    p->e[k[u]] = e;
+    p->s[k[u]] = s;
+    p->rs[k[u]] = rs;
+    p->sa4[k[u]] = sa4;
+    p->sa8[k[u]] = sa8;
+    p->rcp[k[u]] = rc;
+  }
+  if (rc > p->rc) p->rc = rc;
+}
+
+void or_wca_set(or_plj *p, int i, int j, double e, double s) {
+  /* 2._dp**(1./6.): the exponent is a default-real (FP32) literal */
+  double rc = pow(2.0, (double)(1.0f / 6.0f)) * s;
+  int k[2] = {i * OR_MAXT + j, j * OR_MAXT + i};
+  for (int u = 0; u < 2; ++u) {
+    p->e[k[u]] = e;
+    p->s[k[u]] = s;
+    p->rcp[k[u]] = rc;
+  }
+  if (rc > p->rc) p->rc = rc;
 }
 
 void or_plj_force(const double *x, const int *type, int n, const or_box *b, const or_plj *pl,
                   const int *nn, const int *list, int maxnb, double *f, double *e,
                   double *epot, int nthreads) {
   double ev_ui = 1.0 / c_ui_ev();
-  double rcut = pl->rc, rcut2 = rcut * rcut;
+  double rcut = pl->rc;
   (void)nthreads;
 #pragma omp parallel for schedule(static) num_threads(nthreads)
   for (int i = 0; i < n; ++i) {
@@ -515,13 +553,25 @@ void or_plj_force(const double *x, const int *type, int n, const or_box *b, cons
       double vd[3];
       or_vdistance(b, x + 3 * j, x + 3 * i, vd);
       double dr = dot3(vd);
-      if (dr > rcut2) continue;
+      double rck = pl->rcp[k];
+      if (dr > rck * rck) continue;
       dr = sqrt(dr);
-      double t = pl->s[k] / dr;
-      double t2 = t * t, t4 = t2 * t2;
-      double s_dr = t4 * t2; /* (s/dr)**6 */
-      double p = 4.0 * pl->e[k] * (s_dr * s_dr - s_dr) - pl->prc[k] - pl->frc[k] * (dr - rcut);
-      double fr = pl->e[k] * (48.0 * s_dr - 24.0) * s_dr / dr + pl->frc[k];
+      double p, fr;
+      if (pl->kind == 1 && !(dr < pl->rs[k])) {
+        double d = dr - rck, d2 = d * d;
+        double s4 = d2 * d2; /* (dr-rcut)**4 */
+        p = pl->sa4[k] * s4 + pl->sa8[k] * s4 * s4;
+        fr = -(4.0 * pl->sa4[k] * s4 + 8.0 * pl->sa8[k] * s4 * s4) / (dr - rck);
+      } else {
+        double s_dr = pow6(pl->s[k] / dr); /* (s/dr)**6 */
+        if (pl->kind == 0)
+          p = 4.0 * pl->e[k] * (s_dr * s_dr - s_dr) - pl->prc[k] - pl->frc[k] * (dr - rcut);
+        else if (pl->kind == 1)
+          p = 4.0 * pl->e[k] * (s_dr - 1.0) * s_dr;
+        else
+          p = 4.0 * pl->e[k] * (s_dr * s_dr - s_dr) + pl->e[k];
+        fr = pl->e[k] * (48.0 * s_dr - 24.0) * s_dr / dr + (pl->kind == 0 ? pl->frc[k] : 0.0);
+      }
       p = p * ev_ui;
       fr = fr * ev_ui;
       ei = ei + p * 0.5;

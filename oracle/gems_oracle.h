@@ -98,16 +98,27 @@ void or_lj_force(const double *x, int n, const or_box *b, double eps,
 /* ---- shifted Lennard-Jones "plj" (src/Pairs.F90:282-370) ----
  * One table over all type pairs (the reference has one plj intergroup per
  * group pair); one cut rc for all (the script form fixes 10 A). */
+/* The same table serves the splined (slj, Pairs.F90:374-493) and WCA
+ * (wca, Pairs.F90:495-560) forms: kind 0 = plj, 1 = slj, 2 = wca; rcp holds
+ * each pair's cut (plj: rc; slj: from slj_set; wca: 2^(1/6) s) and rc their
+ * maximum. */
 typedef struct {
-  int nt;
+  int nt, kind;
   double rc;
   double e[OR_MAXT * OR_MAXT], s[OR_MAXT * OR_MAXT], prc[OR_MAXT * OR_MAXT],
-      frc[OR_MAXT * OR_MAXT];
+      frc[OR_MAXT * OR_MAXT], rs[OR_MAXT * OR_MAXT], sa4[OR_MAXT * OR_MAXT],
+      sa8[OR_MAXT * OR_MAXT], rcp[OR_MAXT * OR_MAXT];
 } or_plj;
 void or_plj_init(or_plj *p, int nt, double rc);
 /* plj_set (Pairs.F90:282-291) for types i, j (0-based), both orders */
 void or_plj_set(or_plj *p, int i, int j, double e, double s);
-/* plj_interact (Pairs.F90:293-370), full list, newton off; f, e overwritten */
+/* slj_set (Pairs.F90:374-401) / wca_set (Pairs.F90:495-506); the table's
+ * kind must be 1 / 2 (or_plj_init then set kind) */
+void or_slj_set(or_plj *p, int i, int j, double e, double s, double rs);
+void or_wca_set(or_plj *p, int i, int j, double e, double s);
+/* plj_interact (Pairs.F90:293-370), slj_interact (:403-493) or
+ * wca_interact (:508-560) by the table's kind; full list, newton off; f, e
+ * overwritten */
 void or_plj_force(const double *x, const int *type, int n, const or_box *b, const or_plj *p,
                   const int *nn, const int *list, int maxnb, double *f, double *e,
                   double *epot, int nthreads);

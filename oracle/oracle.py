@@ -65,8 +65,8 @@ class Ermak(C.Structure):
 
 
 class Plj(C.Structure):
-    _fields_ = [("nt", C.c_int), ("rc", C.c_double)] + [
-        (nm, C.c_double * _MM) for nm in ("e", "s", "prc", "frc")]
+    _fields_ = [("nt", C.c_int), ("kind", C.c_int), ("rc", C.c_double)] + [
+        (nm, C.c_double * _MM) for nm in ("e", "s", "prc", "frc", "rs", "sa4", "sa8", "rcp")]
 
 
 class MD(C.Structure):
@@ -111,6 +111,9 @@ def _setup(L):
                               C.c_double, _ip, _ip, C.c_int, _dp, _dp, _dp, C.c_int]
     L.or_plj_init.argtypes = [C.POINTER(Plj), C.c_int, C.c_double]
     L.or_plj_set.argtypes = [C.POINTER(Plj), C.c_int, C.c_int, C.c_double, C.c_double]
+    L.or_slj_set.argtypes = [C.POINTER(Plj), C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.c_double]
+    L.or_wca_set.argtypes = [C.POINTER(Plj), C.c_int, C.c_int, C.c_double, C.c_double]
     L.or_plj_force.argtypes = [_dp, _ip, C.c_int, C.POINTER(Box), C.POINTER(Plj), _ip, _ip,
                                C.c_int, _dp, _dp, _dp, C.c_int]
     L.or_ermak_set.argtypes = [C.POINTER(Ermak), C.c_double, C.c_double, C.c_double]
@@ -251,6 +254,32 @@ def plj_params(eps, sigma, rc=10.0):
     for i in range(nt):
         for j in range(i, nt):
             lib().or_plj_set(C.byref(p), i, j, eps[i, j], sigma[i, j])
+    return p
+
+
+def slj_params(eps, sigma, rs):
+    """slj_set (Pairs.F90:374-401) over nt x nt symmetric eps, sigma, rs."""
+    eps, sigma, rs = (np.asarray(a, dtype=np.float64) for a in (eps, sigma, rs))
+    nt = eps.shape[0]
+    p = Plj()
+    lib().or_plj_init(C.byref(p), nt, 0.0)
+    p.kind = 1
+    for i in range(nt):
+        for j in range(i, nt):
+            lib().or_slj_set(C.byref(p), i, j, eps[i, j], sigma[i, j], rs[i, j])
+    return p
+
+
+def wca_params(eps, sigma):
+    """wca_set (Pairs.F90:495-506) over nt x nt symmetric eps, sigma."""
+    eps, sigma = (np.asarray(a, dtype=np.float64) for a in (eps, sigma))
+    nt = eps.shape[0]
+    p = Plj()
+    lib().or_plj_init(C.byref(p), nt, 0.0)
+    p.kind = 2
+    for i in range(nt):
+        for j in range(i, nt):
+            lib().or_wca_set(C.byref(p), i, j, eps[i, j], sigma[i, j])
     return p
 
 

@@ -69,3 +69,34 @@ def force_err(F, Fref):
     """max |dF| / rms |F| (per component)."""
     rms = np.sqrt(np.mean(Fref ** 2))
     return np.max(np.abs(F - Fref)) / rms
+
+
+# examples/LJ/{lj,slj}.gms: two H atoms in a 5.78 A non-periodic box, a scan
+# of their distance with `out state` rows, then velocity Verlet from rest
+LJ_EXAMPLE_BOX = 5.78
+LJ_EXAMPLE_DT = 0.002 * 0.01018  # `time step {0.002*$tau_ps$}`, tau_ps 0.01018
+MASS_H = float(np.float32(1.008))
+
+
+def lj_example_scan(energy_at):
+    """The rows of the scan (`out state` at distance 1.2, then 250 steps of
+    +0.01 from 0.9) and the positions the dynamics starts from, replaying the
+    script's `set move` additions in order."""
+    x = np.array([[1.0, 1.0, 1.0], [2.2, 1.0, 1.0]])
+    rows = [energy_at(x)]
+    x[1, 0] += -1.2
+    x[1, 0] += 0.9
+    for _ in range(250):
+        x[1, 0] += 0.01
+        rows.append(energy_at(x))
+    x[1, 0] += -2.5
+    x[1, 0] += -0.9
+    x[1, 0] += 1.2
+    return np.array(rows), x
+
+
+def last_digit_units(E, kat):
+    """|E - kat| in units of the 7-significant-digit file's last digit."""
+    unit = 10.0 ** (np.floor(np.log10(np.where(kat == 0, 1.0, np.abs(kat)))) - 6)
+    unit = np.where(kat == 0, 1e-7, unit)
+    return np.abs(E - kat) / unit

@@ -438,3 +438,105 @@ def test_reference_kat_nve_plj():
     unit = 10.0 ** (np.floor(np.log10(np.abs(kat))) - 6)  # last printed digit
     assert np.all(np.abs(E[:18] - kat[:18]) <= 0.6 * unit[:18])
     assert np.all(np.abs(E[:, 2] - kat[:, 2]) <= 1e-5 * np.abs(kat[:, 2]))
+
+
+# --------------------------------------------------------------------------
+# splined LJ (slj) and WCA (Pairs.F90:374-560); the LJ examples' own files
+# --------------------------------------------------------------------------
+def _table_energy(p, box):
+    def at(x):
+        nn, lst, _ = O.neighbours(x, box, p.rc + 1.0)
+        _, _, ep = O.plj_force(x, np.zeros(len(x), np.int32), box, p, nn, lst)
+        return [ep * cm.UI_EV, 0.0, ep * cm.UI_EV]
+    return at
+
+
+def test_slj_and_wca_closed_forms():
+    e, s = 0.7, 1.1
+    box = O.make_box(None)
+
+    def u(r):
+        return 4 * e * ((s / r) ** 12 - (s / r) ** 6)
+
+    def du(r):
+        return 4 * e * (-12 * s ** 12 / r ** 13 + 6 * s ** 6 / r ** 7)
+
+    # slj: value and slope continuous at rs, zero value and slope at rc
+    rs = 1.3
+    p = O.slj_params([[e]], [[s]], [[rs]])
+    rc = p.rcp[0]
+    assert rc == pytest.approx(rs - 32 * u(rs) / (11 * du(rs)), rel=1e-14)
+    at = _table_energy(p, box)
+
+    def pe(r):
+        return at(np.array([[0.0, 0, 0], [r, 0, 0]]))[0]
+
+    h = 1e-6
+    assert pe(rs - 1e-12) == pytest.approx(pe(rs + 1e-12), rel=1e-9)
+    assert (pe(rs - 1e-9) - pe(rs - 1e-9 - h)) / h == pytest.approx(
+        (pe(rs + 1e-9 + h) - pe(rs + 1e-9)) / h, rel=1e-4)
+    assert abs(pe(rc - 1e-7)) < 1e-20 and pe(rc + 1e-7) == 0.0
+    assert pe(1.0) == pytest.approx(u(1.0), rel=1e-13)
+    # wca: 12-6 + e below 2^(1/6) s (the exponent an FP32 literal), zero beyond
+    p = O.wca_params([[e]], [[s]])
+    at = _table_energy(p, box)
+    rcw = 2.0 ** float(np.float32(1.0 / 6.0)) * s
+    assert p.rcp[0] == rcw
+    for r in (0.95, 1.0, 1.2):
+        assert pe(r) == pytest.approx(u(r) + e, rel=1e-12)
+    assert pe(rcw * (1 + 1e-9)) == 0.0 and abs(pe(rcw * (1 - 1e-12))) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["slj", "wca"])
+def test_slj_wca_force_is_minus_gradient(kind):
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 3.0, (24, 3))
+    keep = [0]
+    for i in range(1, len(x)):
+        if np.min(np.linalg.norm(x[keep] - x[i], axis=1)) > 0.9:
+            keep.append(i)
+    x = x[keep]
+    types = (np.arange(len(x)) % 2).astype(np.int32)
+    eps, sig = [[0.5, 0.7], [0.7, 0.9]], [[0.9, 1.0], [1.0, 1.1]]
+    p = (O.slj_params(eps, sig, [[1.05, 1.15], [1.15, 1.25]]) if kind == "slj"
+         else O.wca_params(eps, sig))
+    box = O.make_box(None)
+
+    def energy(y):
+        nn, lst, _ = O.neighbours(y, box, p.rc + 1.0)
+        F, _, ep = O.plj_force(y, types, box, p, nn, lst)
+        return F, ep
+
+    F, _ = energy(x)
+    h = 1e-6
+    for (i, d) in [(0, 0), (1, 2), (len(x) - 1, 1)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        g = (energy(xp)[1] - energy(xm)[1]) / (2 * h)
+        assert -g == pytest.approx(F[i, d], rel=1e-5, abs=1e-7)
+
+
+@pytest.mark.parametrize("kind", ["lj", "slj"])
+def test_reference_kat_lj_examples(kind):
+    """examples/LJ/{lj,slj}.gms against their ref/Energy.*.dat (7 digits):
+    the distance scan and 1000 velocity-Verlet steps of two atoms."""
+    box = O.make_box([cm.LJ_EXAMPLE_BOX] * 3, (0, 0, 0))
+    if kind == "slj":
+        p = O.slj_params([[1.0]], [[0.890898718140339]], [[1.108683]])
+        at, mass = _table_energy(p, box), cm.MASS_H
+        md_kw = dict(pot="plj", plj=p)
+    else:
+        def at(x):
+            nn, lst, _ = O.neighbours(x, box, 3.0)
+            _, _, ep = O.lj_force(x, box, 1.0, 0.890898718140339, 2.0, nn, lst)
+            return [ep * cm.UI_EV, 0.0, ep * cm.UI_EV]
+        mass = 3.0  # `set mass 3.`
+        md_kw = dict(pot="lj", lj=(1.0, 0.890898718140339, 2.0))
+    rows, x = cm.lj_example_scan(at)
+    md = O.MDRun(x, np.zeros((2, 3)), np.full(2, mass), box, types=np.zeros(2), skin=1.0,
+                 integ="vv", dt=cm.LJ_EXAMPLE_DT, **md_kw)
+    E = np.vstack([rows, md.run(1000, each=10)])
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.%s.dat" % kind)
+    assert E.shape == kat.shape
+    assert np.all(cm.last_digit_units(E, kat) <= 0.6)
