@@ -19,6 +19,7 @@
 !   gh_run              dinamica n            src/Programs.f90:114-167
 !   gh_prng_lcg         prng lcg / seed       src/Random.F90:137-172
 !   gh_get/set_lcg_state lcg chp record       src/Random.F90:178-190
+!   gh_lbfgs            lbfgs                 src/Quasi_Newton.f90:160-232
 !   gh_set_gdist        set tempgdist T       src/Set_Properties.f90:301-377
 module gems_hip
   use iso_c_binding
@@ -176,6 +177,15 @@ module gems_hip
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
       integer(c_int), intent(in) :: state(3)
+    end function
+
+    integer(c_int) function gh_lbfgs(ctx, m, eps, max_iter, iters, epot_ev) bind(C, name='gh_lbfgs')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: m, max_iter
+      real(c_double), value :: eps
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: epot_ev
     end function
 
     integer(c_int) function gh_set_gdist(ctx, T) bind(C, name='gh_set_gdist')

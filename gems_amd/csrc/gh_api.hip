@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <new>
+#include <vector>
 
 #include "gh_internal.h"
 
@@ -1177,6 +1178,101 @@ int gh_energies(gh_ctx *c, double *epot, double *ekin) {
   GH_TRY(gh_launch_energies(c, &ep, &ek));
   if (epot) *epot = ep * ghc::ui_ev();
   if (ekin) *ekin = ek * ghc::ui_ev();
+  return 0;
+}
+
+// lbfgs_minimizator (Quasi_Newton.f90:160-232): limited-memory BFGS on all
+// atoms with the device force path.  The reference hands GEMS's energy
+// (eV) and gradient to Nocedal's lbfgs (m = 5 pairs, eps = 1e-8, stop when
+// |g| / max(1, |x|) < eps); here the same quantities, the two-loop recursion
+// with the newest pair's scaling as the initial Hessian, and a backtracking
+// line search to the Armijo condition (c1 = 1e-4), first step 1/|g|.
+int gh_lbfgs(gh_ctx *c, int m, double eps, int max_iter, int *iters, double *epot_ev) {
+  if (!c || m < 1 || !(eps > 0) || max_iter < 0) return gh_fail(c, GH_EINVAL, "gh_lbfgs: bad arguments");
+  if (!c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms");
+  const int n3 = 3 * c->n;
+  const double ui_ev = ghc::ui_ev();
+  std::vector<double> x(n3), g(n3), F(n3);
+  GH_TRY(gh_get_state(c, x.data(), nullptr, nullptr, nullptr));
+  auto eval = [&](const std::vector<double> &xx, double &f, std::vector<double> &gg) -> int {
+    GH_TRY(gh_set_positions(c, xx.data()));
+    double et = 0.0;
+    GH_TRY(gh_compute(c, F.data(), nullptr, &et));
+    f = et * ui_ev;
+    for (int k = 0; k < n3; ++k) gg[k] = -F[k] * ui_ev;
+    return 0;
+  };
+  auto dot = [&](const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0.0;
+    for (int k = 0; k < n3; ++k) s += a[k] * b[k];
+    return s;
+  };
+  double f = 0.0;
+  GH_TRY(eval(x, f, g));
+  std::vector<std::vector<double>> S, Y;
+  std::vector<double> rho;
+  std::vector<double> d(n3), xn(n3), gn(n3), q(n3);
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    if (std::sqrt(dot(g, g)) / std::max(1.0, std::sqrt(dot(x, x))) < eps) break;
+    // search direction: the two-loop recursion
+    q = g;
+    std::vector<double> alpha(S.size());
+    for (int i = (int)S.size() - 1; i >= 0; --i) {
+      alpha[i] = rho[i] * dot(S[i], q);
+      for (int k = 0; k < n3; ++k) q[k] -= alpha[i] * Y[i][k];
+    }
+    double h0 = 1.0;
+    if (!S.empty()) h0 = dot(S.back(), Y.back()) / dot(Y.back(), Y.back());
+    for (int k = 0; k < n3; ++k) q[k] *= h0;
+    for (size_t i = 0; i < S.size(); ++i) {
+      const double b = rho[i] * dot(Y[i], q);
+      for (int k = 0; k < n3; ++k) q[k] += S[i][k] * (alpha[i] - b);
+    }
+    for (int k = 0; k < n3; ++k) d[k] = -q[k];
+    double gd = dot(g, d);
+    if (!(gd < 0.0)) {  // not a descent direction: restart from steepest descent
+      S.clear();
+      Y.clear();
+      rho.clear();
+      for (int k = 0; k < n3; ++k) d[k] = -g[k];
+      gd = dot(g, d);
+    }
+    double stp = S.empty() ? 1.0 / std::sqrt(dot(g, g)) : 1.0;
+    double fn = 0.0;
+    int ls = 0;
+    for (;; ++ls) {
+      for (int k = 0; k < n3; ++k) xn[k] = x[k] + stp * d[k];
+      GH_TRY(eval(xn, fn, gn));
+      if (fn <= f + 1e-4 * stp * gd || ls >= 40) break;
+      stp *= 0.5;
+    }
+    if (ls >= 40 && fn > f) break;  // no decrease along d: stop where we are
+    std::vector<double> sv(n3), yv(n3);
+    for (int k = 0; k < n3; ++k) {
+      sv[k] = xn[k] - x[k];
+      yv[k] = gn[k] - g[k];
+    }
+    const double sy = dot(sv, yv);
+    if (sy > 1e-300) {
+      if ((int)S.size() == m) {
+        S.erase(S.begin());
+        Y.erase(Y.begin());
+        rho.erase(rho.begin());
+      }
+      S.push_back(sv);
+      Y.push_back(yv);
+      rho.push_back(1.0 / sy);
+    }
+    x.swap(xn);
+    g.swap(gn);
+    f = fn;
+  }
+  GH_TRY(gh_set_positions(c, x.data()));
+  double et = 0.0;
+  GH_TRY(gh_compute(c, nullptr, nullptr, &et));
+  if (iters) *iters = it;
+  if (epot_ev) *epot_ev = et * ui_ev;
   return 0;
 }
 

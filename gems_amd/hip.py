@@ -21,7 +21,7 @@ SYMBOLS = [
     "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box",
     "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
     "gh_set_lj", "gh_set_plj", "gh_set_slj", "gh_set_wca", "gh_get_lcg_state",
-    "gh_set_lcg_state",
+    "gh_set_lcg_state", "gh_lbfgs",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
     "gh_integrator_vv", "gh_integrator_ermak", "gh_run", "gh_energies", "gh_get_state",
     "gh_get_stats", "gh_set_timing", "gh_get_timing", "gh_set_tile", "gh_autotune",
@@ -81,6 +81,7 @@ def load():
     L.gh_neighbors.argtypes = [vp, _ip, _ip, C.c_int]
     L.gh_prng_lcg.argtypes = [vp, C.c_int]
     L.gh_get_lcg_state.argtypes = [vp, _ip]
+    L.gh_lbfgs.argtypes = [vp, C.c_int, C.c_double, C.c_int, _ip, _dp]
     L.gh_set_lcg_state.argtypes = [vp, _ip]
     L.gh_set_gdist.argtypes = [vp, C.c_double]
     L.gh_integrator_vv.argtypes = [vp, C.c_double]
@@ -245,6 +246,14 @@ class Context:
 
     def prng_lcg(self, seed):
         self._ck(self.L.gh_prng_lcg(self.h, seed))
+
+    def lbfgs(self, m=5, eps=1e-8, max_iter=10000):
+        """`lbfgs` (Quasi_Newton.f90:160-232) on the device force path; returns
+        (iterations, energy in eV); the minimum stays in the context."""
+        it = C.c_int(0)
+        ep = C.c_double(0.0)
+        self._ck(self.L.gh_lbfgs(self.h, m, eps, max_iter, C.byref(it), C.byref(ep)))
+        return it.value, ep.value
 
     def lcg_state(self, state=None):
         """The lcg stream's (idum, iy, ix) as a checkpoint stores it; with

@@ -183,6 +183,13 @@ int gh_run(gh_ctx *ctx, int nsteps, int each, double *energy);
 /* Energies of the current state in eV (write_energy). */
 int gh_energies(gh_ctx *ctx, double *epot, double *ekin);
 
+/* `lbfgs` (lbfgs_minimizator, Quasi_Newton.f90:160-232): limited-memory
+ * BFGS with m correction pairs (the reference: 5) on all atoms, forces from
+ * the device path; stops when |grad| / max(1, |x|) < eps (the reference's
+ * test, eps = 1e-8 there, gradient in eV/A) or after max_iter iterations.
+ * The minimum stays in the context; iterations and the energy (eV) out. */
+int gh_lbfgs(gh_ctx *ctx, int m, double eps, int max_iter, int *iters, double *epot_ev);
+
 /* Copy the state back: 3n positions, 3n velocities, 3n forces (ui/A), n
  * per-atom energies (ui).  Any pointer may be NULL. */
 int gh_get_state(gh_ctx *ctx, double *x, double *v, double *F, double *e);

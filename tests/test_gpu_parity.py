@@ -376,3 +376,38 @@ def test_host_positions_keep_the_lists(name, prec):
     assert np.sqrt(np.mean((F - Fo) ** 2)) / np.sqrt(np.mean(Fo ** 2)) <= tol
     assert c.stats()["nupd"] == nupd0 + 1
     c.close()
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_lbfgs_reaches_the_reference_minimum(precision):
+    """gh_lbfgs (lbfgs_minimizator, Quasi_Newton.f90:160-232) from the start
+    of examples/SMA-TB/md.gms (Ag13, Baletto parameters) reaches the minimum
+    the reference's own `lbfgs` found (tests/golden/ag13_lbfgs.npz): the same
+    energy (1e-9 eV/atom FP64, 1e-6 mixed) and shape (sorted pair distances
+    within 1e-4 A FP64, 1e-3 mixed)."""
+    lines = open(cm.GOLDEN + "/ag13_coords.xyz").read().splitlines()[2:15]
+    x0 = np.array([[float(t) for t in ln.split()[1:4]] for ln in lines])
+    ref = cm.load("ag13_lbfgs")
+    r0 = 2.89
+    m = [np.array([[v]]) for v in (0.1031, 1.1899, 10.85, 3.18, r0, r0 * np.sqrt(2.0),
+                                   r0 * np.sqrt(3.0))]
+    with hip.Context(0, precision) as c:
+        c.set_box(None)
+        c.set_atoms(x0)
+        c.set_masses([float(np.float32(107.87))])
+        c.set_smatb(1, *m)
+        eps = 1e-8 if precision == hip.GH_FP64 else 1e-6
+        it, ep = c.lbfgs(m=5, eps=eps, max_iter=2000)
+        x, _, F, _ = c.state()
+
+    def dists(y):
+        d = np.linalg.norm(y[:, None] - y[None], axis=-1)
+        return np.sort(d[np.triu_indices(len(y), 1)])
+
+    fp64 = precision == hip.GH_FP64
+    print("lbfgs %s: %d iterations, epot %.12f eV (reference %.12f), max |F| %.2e eV/A"
+          % ("fp64" if fp64 else "mixed", it, ep, float(ref["epot_ev"]),
+             np.abs(F).max() * cm.UI_EV))
+    assert it < 2000
+    assert ep == pytest.approx(float(ref["epot_ev"]), abs=(1e-9 if fp64 else 1e-6) * 13)
+    assert np.abs(dists(x) - dists(ref["x"])).max() < (1e-4 if fp64 else 1e-3)
