@@ -63,6 +63,12 @@ module gems_hip
       type(c_ptr), value :: ctx, L, pbc     ! pass c_null_ptr: no box
     end function
 
+    integer(c_int) function gh_set_images(ctx, ghost) bind(C, name='gh_set_images')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ghost          ! 0: box mic T, 1: box mic F
+    end function
+
     integer(c_int) function gh_set_atoms(ctx, n, x, type) bind(C, name='gh_set_atoms')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

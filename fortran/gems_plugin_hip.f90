@@ -13,9 +13,12 @@
 !   * calls gh_compute and accumulates forces (ui/A) and per-atom energies
 !     (ui) into the atoms as the reference routines do, and sets ig%epot (ui).
 ! GEMS_HIP_PRECISION=mixed selects the mixed-precision form (default FP64).
-! Only the minimum-image mode (mic, Program_Types.F90:98) is supported: the
-! device builds its own periodic images.  One context per potential kind: a
-! script with one SMA-TB group and one LJ group is served, more are not.
+! Both periodic-image modes are served (mic, Program_Types.F90:98): `box mic
+! T` pairs nearest images, `box mic F` every first-shell image as GEMS's ghost
+! atoms do (gh_set_images); the device builds its own images either way and
+! only the group's local atoms (ig%n(1)) travel.  One context per potential
+! kind: a script with one SMA-TB group and one LJ group is served, more are
+! not.
 module gems_plugin_hip
 use iso_c_binding
 use gems_hip
@@ -32,6 +35,7 @@ type plugin
   type(c_ptr) :: ctx = c_null_ptr
   integer :: nat = -1
   real(dp) :: box_set(dm) = -1.0_dp
+  logical :: mic_set = .true.
   real(c_double), allocatable :: x(:,:), F(:,:), e(:)
 end type
 
@@ -63,7 +67,6 @@ logical function pack_positions(pl, ig)
   type(plugin), intent(inout) :: pl
   class(intergroup), intent(in) :: ig
   integer :: i, n
-  call werr('gems_hip: only the minimum-image mode (mic) is supported', .not. mic)
   n = ig%n(1)
   if (.not. allocated(pl%x) .or. n /= pl%nat) then
     if (allocated(pl%x)) deallocate(pl%x, pl%F, pl%e)
@@ -73,7 +76,7 @@ logical function pack_positions(pl, ig)
     pl%x(:, i) = ig%at(i)%o%pos(1:3)
   enddo
   pack_positions = .not. c_associated(pl%ctx) .or. n /= pl%nat .or. &
-                   any(box(1:3) /= pl%box_set(1:3))
+                   any(box(1:3) /= pl%box_set(1:3)) .or. (mic .neqv. pl%mic_set)
 end function
 
 ! A new context with the group's box and atoms (types t, 0-based).
@@ -102,10 +105,12 @@ subroutine new_context(pl, ig, t)
   else
     call check(pl, gh_set_box_free(pl%ctx, c_null_ptr, c_null_ptr))
   endif
+  call check(pl, gh_set_images(pl%ctx, merge(0_c_int, 1_c_int, mic)))
   call check(pl, gh_set_atoms(pl%ctx, int(ig%n(1), c_int), pl%x, t))
   call check(pl, gh_set_skin(pl%ctx, real(nb_dcut, c_double)))
   pl%nat = ig%n(1)
   pl%box_set(1:3) = box(1:3)
+  pl%mic_set = mic
 end subroutine
 
 ! Forces and energies out, accumulated as the reference routines do.

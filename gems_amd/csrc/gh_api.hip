@@ -683,6 +683,14 @@ int gh_set_box(gh_ctx *c, const double *L, const int *pbc) {
   return 0;
 }
 
+int gh_set_images(gh_ctx *c, int ghost) {
+  if (!c || (ghost != 0 && ghost != 1))
+    return gh_fail(c, GH_EINVAL, "gh_set_images: ghost must be 0 or 1");
+  c->ghost = ghost;
+  c->ready = false;
+  return 0;
+}
+
 int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
   if (!c || n <= 0 || !x) return gh_fail(c, GH_EINVAL, "gh_set_atoms: bad arguments");
   if (n > (int)GH_JMASK_HOST) return gh_fail(c, GH_EINVAL, "too many atoms (max 134217727)");

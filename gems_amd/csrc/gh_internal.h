@@ -134,6 +134,7 @@ struct gh_ctx {
   int boxed = 0;
   double L[3] = {1e6, 1e6, 1e6};
   int pbc[3] = {0, 0, 0};
+  int ghost = 0;  // periodic images as ghost atoms (box mic F) instead of minimum image
 
   // atoms
   int n = 0, ncap = 0;

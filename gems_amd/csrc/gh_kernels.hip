@@ -64,12 +64,20 @@ struct GridArgs {
   int nc[3], gn[3], gofs[3], pbc[3];
   double L[3], one_cell[3];
   int ncell;
+  int ghost;
 };
 
 
 
 // O(N^2) build (verlet_self, Neighbour.F90:676-759): rd > rl^2 -> skip, so
-// equality is kept; minimum image by rounding as vdistance does.
+// equality is kept.  Minimum image by rounding as vdistance does; or, with
+// ghost images (box mic F), every first-shell image x_j + w L, w in {-1,0,1}
+// on each periodic axis, of every atom j including i itself (not w = 0), as
+// the reference's ghost atoms are (pbcghost / pbcfullghost,
+// Neighbour.F90:1520-1794).  Positions are wrapped into [0, L) just before a
+// ghost-image build, as do_pbc does before the reference creates its ghosts,
+// so the window is the reference's: it holds every image within the cut when
+// each periodic side is at least the cut plus skin.
 __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
     GhStatus *st, const double4 *__restrict__ pos, int n, GridArgs g, double rl2,
     int cap, int ld, int count_only, int *__restrict__ nn, unsigned *__restrict__ list,
@@ -80,7 +88,21 @@ __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
   if (s < n) {
     const double4 pi = pos[s];
     const double ob0 = 1.0 / g.L[0], ob1 = 1.0 / g.L[1], ob2 = 1.0 / g.L[2];
-    for (int j = 0; j < n; ++j) {
+    for (int j = 0; j < n && g.ghost; ++j) {
+      const double4 pj = pos[j];
+      const double ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
+      for (int w = 0; w < 27; ++w) {
+        const int wx = w % 3 - 1, wy = (w / 3) % 3 - 1, wz = w / 9 - 1;
+        if ((wx && !g.pbc[0]) || (wy && !g.pbc[1]) || (wz && !g.pbc[2])) continue;
+        if (j == s && w == 13) continue;
+        const double rd = dist2_rn(ux + wx * g.L[0], uy + wy * g.L[1], uz + wz * g.L[2]);
+        if (rd <= rl2) {
+          if (!count_only && m < cap) list[(size_t)m * ld + s] = (unsigned)j | ((unsigned)w << 27);
+          ++m;
+        }
+      }
+    }
+    for (int j = 0; j < n && !g.ghost; ++j) {
       if (j == s) continue;
       const double4 pj = pos[j];
       double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
@@ -813,6 +835,7 @@ GridArgs make_grid(const gh_ctx *c) {
     g.one_cell[d] = 1.0 / c->cell[d];
   }
   g.ncell = c->ncell;
+  g.ghost = c->ghost;
   return g;
 }
 

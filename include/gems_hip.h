@@ -73,6 +73,15 @@ int gh_set_precision(gh_ctx *ctx, int precision);
  * system; the reference then uses the O(N^2) verlet_self list). */
 int gh_set_box(gh_ctx *ctx, const double *L, const int *pbc);
 
+/* Periodic images.  Replaces `box mic T|F` (CLInterpreter.F90:1723-1726):
+ * ghost = 0 (the default, `mic T`) pairs each atom with the nearest image of
+ * another (vdistance); ghost = 1 (`mic F`) with every first-shell image of
+ * every atom, itself included, as the reference's ghost atoms do (pbcghost /
+ * pbcfullghost, Neighbour.F90:1520-1794).  The two agree once each periodic
+ * side is at least twice the cut plus skin; below that one atom can see
+ * several images of another and only the ghost form counts them all. */
+int gh_set_images(gh_ctx *ctx, int ghost);
+
 /* Atoms.  Replaces the atom creation commands and the intergroup atom table
  * (intergroup%at, Neighbour.F90:114; adda, :359-392).  x: 3n positions (A);
  * type: n internal types 0..nt-1 (TB.F90:194-215 numbering). */
@@ -149,7 +158,8 @@ int gh_compute(gh_ctx *ctx, double *F, double *e_atom, double *e_tot);
 
 /* The current full neighbour list (ig%nn, ig%list; Neighbour.F90:120-121)
  * at radius rcut+skin, in the caller's atom indices, each row sorted
- * ascending.  nn: n counts; idx: n*cap entries, row i at idx[i*cap].
+ * ascending (with ghost images, an atom once per image in range, itself
+ * included).  nn: n counts; idx: n*cap entries, row i at idx[i*cap].
  * Returns GH_EOVERFLOW if a row is longer than cap. */
 int gh_neighbors(gh_ctx *ctx, int *nn, int *idx, int cap);
 

@@ -100,3 +100,29 @@ def last_digit_units(E, kat):
     unit = 10.0 ** (np.floor(np.log10(np.where(kat == 0, 1.0, np.abs(kat)))) - 6)
     unit = np.where(kat == 0, 1e-7, unit)
     return np.abs(E - kat) / unit
+
+
+def ghost_supercell(x, L, reach):
+    """Local atoms (wrapped into [0, L)) followed by every periodic image of
+    them that lies within `reach` of the box: a free-boundary system whose
+    first len(x) atoms feel every image within `reach`, as the reference's
+    ghost atoms make them do (pbcfullghost, Neighbour.F90:1703-1752; the
+    images there are first-shell only, which is every image within the cut
+    plus skin when each side is at least that long).  For SMA-TB the reach is
+    twice the cut, so the densities of the images within the cut are whole.
+    Shifted by +reach + 1 so that all coordinates are positive.  Returns the
+    coordinates and, per atom, the local atom it is an image of."""
+    L = np.asarray(L, float)
+    x = np.mod(np.asarray(x, float), L)
+    k = int(np.ceil(reach / L.min())) + 1
+    imgs, tags = [x], [np.arange(len(x))]
+    for sz in range(-k, k + 1):
+        for sy in range(-k, k + 1):
+            for sx in range(-k, k + 1):
+                if sx == sy == sz == 0:
+                    continue
+                y = x + np.array([sx, sy, sz]) * L
+                keep = np.all((y >= -reach) & (y <= L + reach), axis=1)
+                imgs.append(y[keep])
+                tags.append(np.nonzero(keep)[0])
+    return np.concatenate(imgs) + (reach + 1.0), np.concatenate(tags)

@@ -5,6 +5,7 @@ oracle/ref.mk) on seeded inputs.  Run in the build container only:
     make -f oracle/ref.mk REF=<path to the GEMS sources>
     python tests/golden/make_golden.py
     python tests/golden/make_golden.py --plj <path to the GEMS sources>
+    python tests/golden/make_golden.py --ghost
 
 Inputs are written with literal 17-digit numbers (no `{}` expressions or
 `$var$` expansion), so the reference reads exactly the values stored here.
@@ -153,7 +154,7 @@ def forces_fixture(name, syms, x, L, pbc, pot_block, types, mass=None, extra="")
 
 
 def md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each, mass=None,
-              dt=None):
+              dt=None, extra=""):
     """Writes the inputs of an MD fixture into wd and returns the GEMS script:
     `dinamica nsteps` with the energy columns every `each` steps (E.dat) and
     the final positions (X.xyz)."""
@@ -168,7 +169,7 @@ def md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each,
     s += "> sys\n"
     if mass is not None:
         s += "set mass %s\n" % fmt(mass)
-    s += "group 1 add\n" + pot_block
+    s += "group 1 add\n" + extra + pot_block
     if dt is not None:
         s += "time step %s\n" % fmt(dt)
     s += setup
@@ -179,11 +180,12 @@ def md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each,
 
 
 def md_fixture(name, syms, x, L, pbc, pot_block, types, seed, setup, evolve, nsteps, each,
-               mass=None, dt=None):
+               mass=None, dt=None, extra=""):
     """Reference energy trace of `dinamica nsteps` (16 digits) + final state."""
     wd = tempfile.mkdtemp(prefix="gold_")
     try:
-        s = md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each, mass, dt)
+        s = md_script(wd, syms, x, L, pbc, pot_block, seed, setup, evolve, nsteps, each, mass, dt,
+                      extra)
         log, nupd = run(wd, s)
         E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3)
         X = read_frames(os.path.join(wd, "X.xyz"))[-1]
@@ -331,6 +333,27 @@ def drift_main(nsteps=10000, each=100):
         shutil.rmtree(wd)
 
 
+def ghost_main():
+    """A ghost-image periodic box (`box mic F`, Neighbour.F90:1520-1794) small
+    enough that one atom sees several images of another: LJ 2^3 fcc cells (32
+    atoms, L = 3.36 sigma < 2 rc = 5 sigma); the forces of one configuration
+    and an NVE trace.  (SMA-TB has no such fixture: the reference sizes its
+    band/eband arrays to the local atoms when the interaction is declared
+    (TB.F90:240-241) and writes the ghosts' densities past them
+    (TB.F90:444-448), which faults for boxes this small.)"""
+    rng = np.random.default_rng(20261202)
+    ghost = "box mic F\n"
+    alj = (4.0 / 0.8442) ** (1.0 / 3.0)
+    x = np.mod(fcc(2, alj) + rng.normal(0.0, 0.02, size=(32, 3)), 2 * alj)
+    L = [2 * alj] * 3
+    lj_block = "nb_dcut = 0.3\ninteract 1 under lj 1. 1. 2.5\n"
+    forces_fixture("lj32_ghost", ["Ar"] * 32, x, L, (1, 1, 1), lj_block, [0] * 32, mass=1.0,
+                   extra=ghost)
+    md_fixture("lj32_ghost_nve", ["Ar"] * 32, x, L, (1, 1, 1), lj_block, [0] * 32, seed=777,
+               setup="> sys\nset tempgdist %s\n" % fmt(0.72 / KB_EV), evolve="evolve v_verlet\n",
+               nsteps=400, each=10, mass=1.0, dt=0.005 * np.sqrt(1.0 / EV_UI), extra=ghost)
+
+
 def nve_plj_script(ref_dir, nsteps, outputs):
     """The reference's examples/NVE/v_verlet.gms (100 He + 100 Ne placed by
     `fill` from the lcg stream, shifted LJ for the three group pairs, VV) with
@@ -417,6 +440,8 @@ def nve_plj_checkpoints(ref_dir):
 if __name__ == "__main__":
     if "--drift" in sys.argv:
         drift_main()
+    elif "--ghost" in sys.argv:
+        ghost_main()
     elif "--plj" in sys.argv:
         nve_plj_main(sys.argv[sys.argv.index("--plj") + 1])
         nve_plj_checkpoints(sys.argv[sys.argv.index("--plj") + 1])

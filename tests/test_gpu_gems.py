@@ -10,7 +10,8 @@ scripts are the ones that wrote the au864_nve and lj4000_nve fixtures with the
 reference binary (tests/golden/make_golden.py, md_script): 864 Au atoms
 (`prng lcg` 4242, `set tempgdist 300`, dt 2 fs, `dinamica 200`) and 4000 LJ
 atoms (seed 777, T* 0.72, dt 0.005 tau, `dinamica 100`), periodic, velocity
-Verlet, energy every 10 steps.
+Verlet, energy every 10 steps; and the 32-atom LJ box with ghost images
+(`box mic F`, side 3.36 sigma, `dinamica 400`).
 
 Bars: FP64 energy columns within 1e-9 relative of the reference's 16-digit
 columns and final positions within 1e-8 A; mixed precision within 1e-6
@@ -37,7 +38,8 @@ pytestmark = [pytest.mark.gpu,
 
 
 CASES = [("au864_nve", "fp64", 1e-9, 1e-8), ("au864_nve", "mixed", 1e-6, 1e-4),
-         ("lj4000_nve", "fp64", 1e-9, 1e-8), ("lj4000_nve", "mixed", 1e-5, 1e-3)]
+         ("lj4000_nve", "fp64", 1e-9, 1e-8), ("lj4000_nve", "mixed", 1e-5, 1e-3),
+         ("lj32_ghost_nve", "fp64", 1e-9, 1e-8), ("lj32_ghost_nve", "mixed", 1e-5, 1e-3)]
 
 
 def _script(name, f, wd):
@@ -52,7 +54,8 @@ def _script(name, f, wd):
                         "nb_dcut = 0.3\ninteract 1 under lj 1. 1. 2.5\n", int(f["seed"]),
                         "> sys\nset tempgdist %s\n" % mg.fmt(T), "evolve v_verlet\n",
                         int(f["nsteps"]), int(f["each"]), mass=1.0,
-                        dt=0.005 * np.sqrt(1.0 / mg.EV_UI))
+                        dt=0.005 * np.sqrt(1.0 / mg.EV_UI),
+                        extra="box mic F\n" if "ghost" in name else "")
 
 
 @pytest.mark.parametrize("name,precision,etol,xtol", CASES)

@@ -540,3 +540,25 @@ def test_reference_kat_lj_examples(kind):
     kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.%s.dat" % kind)
     assert E.shape == kat.shape
     assert np.all(cm.last_digit_units(E, kat) <= 0.6)
+
+
+# --------------------------------------------------------------------------
+# ghost-image periodic boxes (box mic F)
+# --------------------------------------------------------------------------
+def test_ghost_images_lj_match_reference_binary():
+    # 32 LJ atoms in a 3.36 sigma box (< 2 rc): the reference's ghost atoms
+    # against a free-boundary supercell of every image within rc + skin
+    f = cm.load("lj32_ghost")
+    n = len(f["x"])
+    X, _ = cm.ghost_supercell(f["x"], f["L"], 2.8)
+    box = O.make_box(None)
+    nn, lst, _ = O.neighbours(X, box, 2.8)
+    F, e, ep = O.lj_force(X, box, 1.0, 1.0, 2.5, nn, lst)
+    assert cm.force_err(F[:n], f["F"]) < 1e-10
+    assert e[:n].sum() * cm.UI_EV == pytest.approx(f["epot_ev"], rel=1e-13)
+    # the box is small enough that the nearest image alone is not the same
+    boxp = O.make_box(f["L"], f["pbc"])
+    nn, lst, _ = O.neighbours(f["x"], boxp, 2.8)
+    Fm, _, epm = O.lj_force(f["x"], boxp, 1.0, 1.0, 2.5, nn, lst)
+    assert cm.force_err(Fm, f["F"]) > 0.1
+    assert abs(epm * cm.UI_EV - f["epot_ev"]) > 1.0
