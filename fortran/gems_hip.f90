@@ -69,6 +69,12 @@ module gems_hip
       integer(c_int), value :: ghost          ! 0: box mic T, 1: box mic F
     end function
 
+    integer(c_int) function gh_set_pair_masks(ctx, feel, felt) bind(C, name='gh_set_pair_masks')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: feel(*), felt(*)   ! per atom: i pairs j if iand(feel(i), felt(j)) /= 0
+    end function
+
     integer(c_int) function gh_set_atoms(ctx, n, x, type) bind(C, name='gh_set_atoms')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

@@ -18,7 +18,7 @@
 ! atoms do (gh_set_images); the device builds its own images either way and
 ! only the group's local atoms (ig%n(1)) travel.  One context per potential
 ! kind: a script with one SMA-TB group and one LJ group is served, more are
-! not.
+! not, and only self (`under`) groups: a cross group stops with an error.
 module gems_plugin_hip
 use iso_c_binding
 use gems_hip
@@ -67,6 +67,9 @@ logical function pack_positions(pl, ig)
   type(plugin), intent(inout) :: pl
   class(intergroup), intent(in) :: ig
   integer :: i, n
+  ! cross groups (`A with B`, `A feels B`) hold two atom lists; the device
+  ! form of `feels` is gh_set_pair_masks, not wired into this hook
+  call werr('gems_hip: only self (`under`) interaction groups are served', .not. ig%self)
   n = ig%n(1)
   if (.not. allocated(pl%x) .or. n /= pl%nat) then
     if (allocated(pl%x)) deallocate(pl%x, pl%F, pl%e)

@@ -284,7 +284,7 @@ static void setup_grid(gh_ctx *c) {
     c->cell[d] = c->L[d];
   }
   c->ncell = 1;
-  if (!c->boxed) return;
+  if (!c->boxed || c->masked) return;
   int nc[3];
   bool all_small = true, pbc_small = false;
   for (int d = 0; d < 3; ++d) {
@@ -324,6 +324,8 @@ static int alloc_cells(gh_ctx *c) {
 static int ensure_ready(gh_ctx *c) {
   if (!c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms: call gh_set_atoms first");
   if (c->pot == GH_POT_NONE) return gh_fail(c, GH_ESTATE, "no potential set");
+  if (c->masked && c->pot == GH_POT_SMATB)
+    return gh_fail(c, GH_EINVAL, "one-sided interactions are served for pair potentials only");
   if (c->ready) return 0;
   setup_grid(c);
   if (c->boxed) GH_TRY(gh_launch_wrap(c));
@@ -564,7 +566,7 @@ void gh_destroy(gh_ctx *c) {
                  c->rho, c->eband, c->eband32, c->type, c->type_t, c->orig, c->orig_t,
                  c->xold, c->posv, c->velv, c->gauss, c->cellof, c->cellid, c->cell_count,
                  c->cell_start, c->perm, c->nn, c->list, c->tbd, c->tbf, c->status,
-                 c->red, c->list16, c->xin, c->chain_dev, c->pljd};
+                 c->red, c->list16, c->xin, c->chain_dev, c->pljd, c->mfeel, c->mfelt};
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->chain_pin) (void)hipHostFree(c->chain_pin);
@@ -683,6 +685,28 @@ int gh_set_box(gh_ctx *c, const double *L, const int *pbc) {
   return 0;
 }
 
+int gh_set_pair_masks(gh_ctx *c, const int *feel, const int *felt) {
+  if (!c) return GH_EINVAL;
+  if ((feel == nullptr) != (felt == nullptr))
+    return gh_fail(c, GH_EINVAL, "gh_set_pair_masks: give both masks or neither");
+  if (!c->have_atoms) return gh_fail(c, GH_ESTATE, "gh_set_pair_masks: call gh_set_atoms first");
+  GH_HIP(c, hipSetDevice(c->device));
+  c->ready = false;
+  if (!feel) {
+    c->masked = false;
+    return 0;
+  }
+  if (!c->mfeel) {
+    GH_TRY(dalloc(c, &c->mfeel, c->ncap));
+    GH_TRY(dalloc(c, &c->mfelt, c->ncap));
+  }
+  GH_HIP(c, hipMemcpyAsync(c->mfeel, feel, sizeof(int) * c->n, hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipMemcpyAsync(c->mfelt, felt, sizeof(int) * c->n, hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));
+  c->masked = true;
+  return 0;
+}
+
 int gh_set_images(gh_ctx *c, int ghost) {
   if (!c || (ghost != 0 && ghost != 1))
     return gh_fail(c, GH_EINVAL, "gh_set_images: ghost must be 0 or 1");
@@ -696,8 +720,11 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
   if (n > (int)GH_JMASK_HOST) return gh_fail(c, GH_EINVAL, "too many atoms (max 134217727)");
   GH_HIP(c, hipSetDevice(c->device));
   const int ncap = roundup(n, 256);
+  c->masked = false;  // new atoms: masks, if any, are set again after this call
   if (ncap != c->ncap) {
     c->ncap = ncap;
+    GH_TRY(dalloc(c, &c->mfeel, 0));
+    GH_TRY(dalloc(c, &c->mfelt, 0));
     GH_TRY(dalloc(c, &c->pos, ncap));
     GH_TRY(dalloc(c, &c->pos_t, ncap));
     GH_TRY(dalloc(c, &c->pos_next, ncap));

@@ -202,6 +202,9 @@ struct gh_ctx {
   TbF *tbf = nullptr;                         // [MAXT*MAXT]
   TbF tbf0;                                   // host copy of tbf[0] (one-type kernel argument)
   PljDev *pljd = nullptr;                     // [MAXT*MAXT] shifted LJ per type pair
+  // one-sided interactions: i feels j iff mfeel[i] & mfelt[j] (caller order)
+  int *mfeel = nullptr, *mfelt = nullptr;
+  bool masked = false;
   GhStatus *status = nullptr;
   double *red = nullptr;                      // reduction scratch
   int nred = 0;

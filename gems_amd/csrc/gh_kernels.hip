@@ -77,18 +77,26 @@ struct GridArgs {
 // Neighbour.F90:1520-1794).  Positions are wrapped into [0, L) just before a
 // ghost-image build, as do_pbc does before the reference creates its ghosts,
 // so the window is the reference's: it holds every image within the cut when
-// each periodic side is at least the cut plus skin.
+// each periodic side is at least the cut plus skin.  With pair masks (a
+// one-sided `feels` interaction: a cross group with half and newton off,
+// Interaction.f90:69-77, 128-131) atom i lists j only when
+// mfeel[i] & mfelt[j] is nonzero; i then takes the whole pair force and half
+// the pair energy, j no reaction.
 __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
     GhStatus *st, const double4 *__restrict__ pos, int n, GridArgs g, double rl2,
     int cap, int ld, int count_only, int *__restrict__ nn, unsigned *__restrict__ list,
-    double *__restrict__ xold, int ncap) {
+    double *__restrict__ xold, int ncap, const int *__restrict__ orig,
+    const int *__restrict__ mfeel, const int *__restrict__ mfelt) {
   if (!st->rebuild) return;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   int m = 0;
   if (s < n) {
     const double4 pi = pos[s];
+    // one-sided interactions: j enters i's row only if i feels what j is felt by
+    const int fi = mfeel ? mfeel[orig[s]] : 0;
     const double ob0 = 1.0 / g.L[0], ob1 = 1.0 / g.L[1], ob2 = 1.0 / g.L[2];
     for (int j = 0; j < n && g.ghost; ++j) {
+      if (mfeel && !(fi & mfelt[orig[j]])) continue;
       const double4 pj = pos[j];
       const double ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
       for (int w = 0; w < 27; ++w) {
@@ -104,6 +112,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
     }
     for (int j = 0; j < n && !g.ghost; ++j) {
       if (j == s) continue;
+      if (mfeel && !(fi & mfelt[orig[j]])) continue;
       const double4 pj = pos[j];
       double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
       int wx = 0, wy = 0, wz = 0;
@@ -905,7 +914,9 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   const double rl2 = rl * rl;
   const int ld = c->ncap;
   k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
-                                     (unsigned *)c->list, c->xold, c->ncap);
+                                     (unsigned *)c->list, c->xold, c->ncap, c->orig,
+                                     c->masked ? c->mfeel : nullptr,
+                                     c->masked ? c->mfelt : nullptr);
   GH_HIP(c, hipGetLastError());
   return 0;
 }

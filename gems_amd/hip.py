@@ -18,7 +18,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 
 # every symbol declared in include/gems_hip.h
 SYMBOLS = [
-    "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box", "gh_set_images",
+    "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box", "gh_set_images", "gh_set_pair_masks",
     "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
     "gh_set_lj", "gh_set_plj", "gh_set_slj", "gh_set_wca", "gh_get_lcg_state",
     "gh_set_lcg_state", "gh_lbfgs",
@@ -66,6 +66,7 @@ def load():
     L.gh_set_precision.argtypes = [vp, C.c_int]
     L.gh_set_box.argtypes = [vp, _dp, _ip]
     L.gh_set_images.argtypes = [vp, C.c_int]
+    L.gh_set_pair_masks.argtypes = [vp, _ip, _ip]
     L.gh_set_atoms.argtypes = [vp, C.c_int, _dp, _ip]
     L.gh_set_masses.argtypes = [vp, C.c_int, _dp]
     L.gh_set_velocities.argtypes = [vp, _dp]
@@ -170,6 +171,16 @@ class Context:
         atom, as the reference's ghost atoms; `box mic T` (False, the
         default): the nearest image only."""
         self._ck(self.L.gh_set_images(self.h, 1 if ghost else 0))
+
+    def set_pair_masks(self, feel=None, felt=None):
+        """One-sided interactions (`interact A feels B`): atom i pairs with j
+        only when feel[i] & felt[j] != 0; None, None: every pair."""
+        if feel is None:
+            self._ck(self.L.gh_set_pair_masks(self.h, None, None))
+            return
+        fe = np.ascontiguousarray(feel, np.int32)
+        ft = np.ascontiguousarray(felt, np.int32)
+        self._ck(self.L.gh_set_pair_masks(self.h, _p(fe, _ip), _p(ft, _ip)))
 
     def set_atoms(self, x, types=None):
         x = np.ascontiguousarray(x, np.float64)

@@ -82,6 +82,18 @@ int gh_set_box(gh_ctx *ctx, const double *L, const int *pbc);
  * several images of another and only the ghost form counts them all. */
 int gh_set_images(gh_ctx *ctx, int ghost);
 
+/* One-sided interactions.  Replaces `interact A feels B` (also `A <- B`,
+ * `B -> A`; Interaction.f90:69-77, a cross group with half and newton off,
+ * :128-131) and lets several interactions of one pair potential share a
+ * context: atom i pairs with atom j only when feel[i] & felt[j] is nonzero.
+ * Give interaction k its bit: `under G` sets it in feel and felt of G's
+ * atoms, `A feels B` in feel of A's and felt of B's.  i takes the whole pair
+ * force and half the pair energy, as in the reference; j gets no reaction
+ * unless it feels i too.  n ints each, caller order; NULL, NULL restores
+ * every pair (the default).  Call after gh_set_atoms (which clears them).
+ * Pair potentials only; a masked context uses the O(N^2) list. */
+int gh_set_pair_masks(gh_ctx *ctx, const int *feel, const int *felt);
+
 /* Atoms.  Replaces the atom creation commands and the intergroup atom table
  * (intergroup%at, Neighbour.F90:114; adda, :359-392).  x: 3n positions (A);
  * type: n internal types 0..nt-1 (TB.F90:194-215 numbering). */
