@@ -63,6 +63,12 @@ module gems_hip
       type(c_ptr), value :: ctx, L, pbc     ! pass c_null_ptr: no box
     end function
 
+    integer(c_int) function gh_set_sm1(ctx, r1, r2, e) bind(C, name='gh_set_sm1')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: r1, r2, e
+    end function
+
     integer(c_int) function gh_set_images(ctx, ghost) bind(C, name='gh_set_images')
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

@@ -944,6 +944,23 @@ int gh_set_wca(gh_ctx *c, int nt, const double *eps, const double *sigma) {
   });
 }
 
+int gh_set_sm1(gh_ctx *c, double r1, double r2, double e) {
+  if (!c) return GH_EINVAL;
+  if (!(r1 >= 0.0) || !(r2 > r1)) return gh_fail(c, GH_EINVAL, "gh_set_sm1: need 0 <= r1 < r2");
+  double eps[GH_MAXT * GH_MAXT], one[GH_MAXT * GH_MAXT];
+  for (int k = 0; k < GH_MAXT * GH_MAXT; ++k) {
+    eps[k] = e;
+    one[k] = 1.0;
+  }
+  const int nt = std::max(c->nt, 1);
+  return set_pair_table(c, GH_POT_SM1, nt, eps, one, "gh_set_sm1", [&](PljDev &d, int, int) {
+    d.rs = r1;
+    d.rc = r2;
+    d.sa8 = (r1 + r2) * 0.5;
+    return true;
+  });
+}
+
 int gh_set_skin(gh_ctx *c, double skin) {
   if (!c || !(skin >= 0.0)) return gh_fail(c, GH_EINVAL, "skin must be >= 0");
   c->skin = skin;

@@ -274,7 +274,17 @@ __device__ __forceinline__ void table_pair64(const PljDev &P, double r2, double 
                                              double &f_r) {
   const double dr = sqrt(r2);
   double f;
-  if (POT == 4 && !(dr < P.rs)) {
+  if (POT == 6) {  // sm1: f here is minus the reference's attractive factor
+    if (dr > P.rs) {
+      const double w = P.rc - P.rs;
+      const double a = M_PI * (dr - P.sa8) / w;
+      p = (0.5 + 0.5 * sin(a)) * P.e - P.e;
+      f = -(cos(a) * M_PI_2 * dr / w) * P.e;
+    } else {
+      p = -P.e;
+      f = 0.0;
+    }
+  } else if (POT == 4 && !(dr < P.rs)) {
     const double d = dr - P.rc, d2 = d * d, s4 = d2 * d2;  // (dr - rcut)**4
     p = P.sa4 * s4 + P.sa8 * s4 * s4;
     f = -(4.0 * P.sa4 * s4 + 8.0 * P.sa8 * s4 * s4) / (dr - P.rc);
@@ -286,13 +296,23 @@ __device__ __forceinline__ void table_pair64(const PljDev &P, double r2, double 
     else p = 4.0 * P.e * (s6 * s6 - s6) + P.e;
     f = P.e * (48.0 * s6 - 24.0) * s6 / dr + (POT == 3 ? P.frc : 0.0);
   }
-  f_r = f / dr;
+  f_r = POT == 6 && f == 0.0 ? 0.0 : f / dr;
 }
 template <int POT>
 __device__ __forceinline__ void table_pair32(const PljDev &P, float r2f, float &p, float &f_r) {
   const float ir = __builtin_amdgcn_rsqf(r2f);
   const float dr = r2f * ir, ir2 = ir * ir;
-  if (POT == 4 && !(dr < P.rsf)) {
+  if (POT == 6) {
+    if (dr > P.rsf) {
+      const float w = P.rcf - P.rsf;
+      const float a = (float)M_PI * (dr - P.sa8f) / w;
+      p = (0.5f + 0.5f * sinf(a)) * P.ef - P.ef;
+      f_r = -(cosf(a) * (float)M_PI_2 / w) * P.ef;
+    } else {
+      p = -P.ef;
+      f_r = 0.0f;
+    }
+  } else if (POT == 4 && !(dr < P.rsf)) {
     const float d = dr - P.rcf, d2 = d * d, s4 = d2 * d2;
     p = fmaf(P.sa8f * s4, s4, P.sa4f * s4);
     f_r = -fmaf(8.0f * P.sa8f * s4, s4, 4.0f * P.sa4f * s4) / d * ir;

@@ -77,11 +77,12 @@ struct TbF {
 // pot says which): prc and frc are plj's value and slope terms at the cut
 // (plj_set, Pairs.F90:282-291); rs, sa4, sa8 slj's spline (slj_set,
 // Pairs.F90:374-401); rc this pair's cut.  FP64 and FP32 forms.
+// per type pair; sm1 keeps r1 in rs, r2 in rc and (r1 + r2)/2 in sa8
 struct PljDev {
   double e, s, prc, frc, rs, sa4, sa8, rc, rc2;
   float ef, s2f, prcf, frcf, rsf, sa4f, sa8f, rcf;
 };
-inline bool gh_is_table_pot(int pot) { return pot >= 3 && pot <= 5; }
+inline bool gh_is_table_pot(int pot) { return pot >= 3 && pot <= 6; }
 
 #define GH_FXS 8388608.0         // fixed-point steps per Angstrom (2^23)
 #define GH_FXS2 70368744177664.0 // GH_FXS^2 (2^46)

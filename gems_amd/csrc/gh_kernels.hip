@@ -998,6 +998,7 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   switch (c->pot) {                                                                         \
     case GH_POT_SLJ: PLJ_LAUNCH(MX, GH_POT_SLJ); break;                                     \
     case GH_POT_WCA: PLJ_LAUNCH(MX, GH_POT_WCA); break;                                     \
+    case GH_POT_SM1: PLJ_LAUNCH(MX, GH_POT_SM1); break;                                     \
     default: PLJ_LAUNCH(MX, GH_POT_PLJ);                                                    \
   }
     if (mixed) {

@@ -1391,6 +1391,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     case GH_POT_PLJ: LJ_LAUNCH(MX, GH_POT_PLJ); break;                                      \
     case GH_POT_SLJ: LJ_LAUNCH(MX, GH_POT_SLJ); break;                                      \
     case GH_POT_WCA: LJ_LAUNCH(MX, GH_POT_WCA); break;                                      \
+    case GH_POT_SM1: LJ_LAUNCH(MX, GH_POT_SM1); break;                                      \
     default: LJ_LAUNCH(MX, GH_POT_LJ);                                                      \
   }
     if (mixed) {

@@ -18,7 +18,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 
 # every symbol declared in include/gems_hip.h
 SYMBOLS = [
-    "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box", "gh_set_images", "gh_set_pair_masks",
+    "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box", "gh_set_images", "gh_set_pair_masks", "gh_set_sm1",
     "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
     "gh_set_lj", "gh_set_plj", "gh_set_slj", "gh_set_wca", "gh_get_lcg_state",
     "gh_set_lcg_state", "gh_lbfgs",
@@ -67,6 +67,7 @@ def load():
     L.gh_set_box.argtypes = [vp, _dp, _ip]
     L.gh_set_images.argtypes = [vp, C.c_int]
     L.gh_set_pair_masks.argtypes = [vp, _ip, _ip]
+    L.gh_set_sm1.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.gh_set_atoms.argtypes = [vp, C.c_int, _dp, _ip]
     L.gh_set_masses.argtypes = [vp, C.c_int, _dp]
     L.gh_set_velocities.argtypes = [vp, _dp]
@@ -225,6 +226,11 @@ class Context:
         e, sg, r = (np.ascontiguousarray(a, dtype=np.float64) for a in (eps, sigma, rs))
         self._ck(self.L.gh_set_slj(self.h, e.shape[0], e.ctypes.data_as(_dp),
                                    sg.ctypes.data_as(_dp), r.ctypes.data_as(_dp)))
+
+    def set_sm1(self, r1, r2, e):
+        """sm1 smooth step (Pairs.F90:916-990): -e inside r1, a sine ramp to
+        zero at the cut r2 (A, eV)."""
+        self._ck(self.L.gh_set_sm1(self.h, r1, r2, e))
 
     def set_wca(self, eps, sigma):
         """Weeks-Chandler-Andersen (wca, Pairs.F90:495-560): nt x nt eps, sigma."""

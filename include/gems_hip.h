@@ -50,6 +50,7 @@ typedef struct gh_ctx gh_ctx;
 #define GH_POT_PLJ 3
 #define GH_POT_SLJ 4
 #define GH_POT_WCA 5
+#define GH_POT_SM1 6
 
 /* integrators */
 #define GH_INT_VV 0     /* velocity Verlet   (Integration.F90:1150-1186) */
@@ -136,6 +137,14 @@ int gh_set_slj(gh_ctx *ctx, int nt, const double *eps, const double *sigma, cons
  * wca_interact, Pairs.F90:495-560; the exponent 1/6 is a default-real
  * literal there and here).  eps, sigma nt x nt symmetric. */
 int gh_set_wca(gh_ctx *ctx, int nt, const double *eps, const double *sigma);
+
+/* sm1, a smooth attractive step (sm1_set / sm1_interact,
+ * Pairs.F90:916-990): p = -e for r <= r1; from r1 to the cut r2
+ * p = e (1/2 + 1/2 sin a) - e with a = pi (r - (r1 + r2)/2) / (r2 - r1), and
+ * the force along the pair e cos(a) (pi/2) r / (r2 - r1) toward the other
+ * atom (the reference's factor r reproduced as written).  e in eV, r1 < r2
+ * in A; one set for every type pair. */
+int gh_set_sm1(gh_ctx *ctx, double r1, double r2, double e);
 
 /* Verlet skin `nb_dcut` (Neighbour.F90:237), A. */
 int gh_set_skin(gh_ctx *ctx, double skin);

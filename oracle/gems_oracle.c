@@ -538,6 +538,16 @@ void or_wca_set(or_plj *p, int i, int j, double e, double s) {
   if (rc > p->rc) p->rc = rc;
 }
 
+void or_sm1_set(or_plj *p, double r1, double r2, double e) {
+  p->kind = 3;
+  for (int k = 0; k < OR_MAXT * OR_MAXT; ++k) {
+    p->e[k] = e;
+    p->rs[k] = r1;
+    p->rcp[k] = r2;
+  }
+  p->rc = r2;
+}
+
 void or_plj_force(const double *x, const int *type, int n, const or_box *b, const or_plj *pl,
                   const int *nn, const int *list, int maxnb, double *f, double *e,
                   double *epot, int nthreads) {
@@ -557,7 +567,22 @@ void or_plj_force(const double *x, const int *type, int n, const or_box *b, cons
       if (dr > rck * rck) continue;
       dr = sqrt(dr);
       double p, fr;
-      if (pl->kind == 1 && !(dr < pl->rs[k])) {
+      if (pl->kind == 3) {
+        /* sm1_interact (Pairs.F90:925-990): the reference adds f vd / dr to
+         * atom i, so fr = -f here; pi = 4 atan(1), pio2 = pi / 2 */
+        const double pi = 4.0 * atan(1.0), pio2 = pi * 0.5;
+        const double r1 = pl->rs[k], r2 = rck;
+        if (dr > r1) {
+          double aux = pi * (dr - (r1 + r2) * 0.5) / (r2 - r1);
+          p = 0.5 + 0.5 * sin(aux);
+          fr = cos(aux) * pio2 * dr / (r2 - r1);
+          p = p * pl->e[k] - pl->e[k];
+          fr = -(fr * pl->e[k]);
+        } else {
+          p = -pl->e[k];
+          fr = 0.0;
+        }
+      } else if (pl->kind == 1 && !(dr < pl->rs[k])) {
         double d = dr - rck, d2 = d * d;
         double s4 = d2 * d2; /* (dr-rcut)**4 */
         p = pl->sa4[k] * s4 + pl->sa8[k] * s4 * s4;

@@ -99,7 +99,7 @@ void or_lj_force(const double *x, int n, const or_box *b, double eps,
  * One table over all type pairs (the reference has one plj intergroup per
  * group pair); one cut rc for all (the script form fixes 10 A). */
 /* The same table serves the splined (slj, Pairs.F90:374-493) and WCA
- * (wca, Pairs.F90:495-560) forms: kind 0 = plj, 1 = slj, 2 = wca; rcp holds
+ * (wca, Pairs.F90:495-560) forms: kind 0 = plj, 1 = slj, 2 = wca, 3 = sm1; rcp holds
  * each pair's cut (plj: rc; slj: from slj_set; wca: 2^(1/6) s) and rc their
  * maximum. */
 typedef struct {
@@ -116,6 +116,9 @@ void or_plj_set(or_plj *p, int i, int j, double e, double s);
  * kind must be 1 / 2 (or_plj_init then set kind) */
 void or_slj_set(or_plj *p, int i, int j, double e, double s, double rs);
 void or_wca_set(or_plj *p, int i, int j, double e, double s);
+/* sm1_set (Pairs.F90:916-922): one (r1, r2, e) for every type pair; kind 3
+ * (rs holds r1, rcp the cut r2) */
+void or_sm1_set(or_plj *p, double r1, double r2, double e);
 /* plj_interact (Pairs.F90:293-370), slj_interact (:403-493) or
  * wca_interact (:508-560) by the table's kind; full list, newton off; f, e
  * overwritten */

@@ -114,6 +114,7 @@ def _setup(L):
     L.or_slj_set.argtypes = [C.POINTER(Plj), C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_double]
     L.or_wca_set.argtypes = [C.POINTER(Plj), C.c_int, C.c_int, C.c_double, C.c_double]
+    L.or_sm1_set.argtypes = [C.POINTER(Plj), C.c_double, C.c_double, C.c_double]
     L.or_plj_force.argtypes = [_dp, _ip, C.c_int, C.POINTER(Box), C.POINTER(Plj), _ip, _ip,
                                C.c_int, _dp, _dp, _dp, C.c_int]
     L.or_ermak_set.argtypes = [C.POINTER(Ermak), C.c_double, C.c_double, C.c_double]
@@ -280,6 +281,14 @@ def wca_params(eps, sigma):
     for i in range(nt):
         for j in range(i, nt):
             lib().or_wca_set(C.byref(p), i, j, eps[i, j], sigma[i, j])
+    return p
+
+
+def sm1_params(r1, r2, e, nt=1):
+    """sm1_set (Pairs.F90:916-922): r1, cut r2 (A), e (eV) for every pair."""
+    p = Plj()
+    lib().or_plj_init(C.byref(p), nt, 0.0)
+    lib().or_sm1_set(C.byref(p), r1, r2, e)
     return p
 
 

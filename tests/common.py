@@ -126,3 +126,21 @@ def ghost_supercell(x, L, reach):
                 imgs.append(y[keep])
                 tags.append(np.nonzero(keep)[0])
     return np.concatenate(imgs) + (reach + 1.0), np.concatenate(tags)
+
+
+def sm1_example_scan(energy_at):
+    """examples/LJ/sm1.gms: two H atoms start on one site; `out state`, then
+    300 steps of +0.01 A with `out state`; then -2 A and atom 2 at 10 A/ps
+    along x.  Returns the scan rows and the start of the dynamics."""
+    x = np.array([[1.0, 1.0, 1.0], [1.0, 1.0, 1.0]])
+    rows = [energy_at(x)]
+    for _ in range(300):
+        x[1, 0] += 0.01
+        rows.append(energy_at(x))
+    x[1, 0] += -2.0
+    v = np.zeros((2, 3))
+    v[1, 0] = 10.0
+    return np.array(rows), x, v
+
+
+SM1_DT = 0.001  # `time step 0.001`

@@ -1,6 +1,6 @@
 """The Lennard-Jones family on the device path: the reference's own
-examples/LJ/{lj,slj}.gms known-answer files (ref/Energy.lj.dat,
-ref/Energy.slj.dat, 7 digits: two H atoms, a 251-point distance scan with
+examples/LJ/{lj,slj,sm1}.gms known-answer files (ref/Energy.lj.dat,
+ref/Energy.slj.dat, ref/Energy.sm1.dat, 7 digits; lj and slj: two H atoms, a 251-point distance scan with
 `out state`, then 1000 velocity-Verlet steps from rest, energy every 10),
 and slj / wca forces (slj_interact, wca_interact, Pairs.F90:403-560) on a
 two-type cluster against the oracle.
@@ -100,3 +100,38 @@ def test_slj_wca_forces_match_oracle(kind, precision):
         assert mode == (1 if boxed else 0)
         assert cm.force_err(F, Fo) <= tol, (boxed, cm.force_err(F, Fo))
         assert et == pytest.approx(epo, rel=1e-10 if precision == hip.GH_FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_sm1_example_known_answer(precision):
+    """examples/LJ/sm1.gms (ref/Energy.sm1.dat): two H atoms from one site, a
+    301-point scan, then 1000 velocity-Verlet steps with atom 2 at 10 A/ps,
+    energy every step.  Bars as above, the mixed one 2e-8 eV (2e-6 of the
+    step's depth e = 0.01 eV)."""
+    def ctx():
+        c = hip.Context(0, precision)
+        c.set_box(None)
+        c.set_atoms(np.array([[1.0, 1.0, 1.0], [1.0, 1.0, 1.0]]))
+        c.set_sm1(1.0, 2.0, 0.01)
+        c.set_masses([cm.MASS_H])
+        c.set_skin(1.0)
+        return c
+    with ctx() as c:
+        def at(x):
+            c.set_positions(x)
+            _, _, et = c.compute()
+            return [et * cm.UI_EV, 0.0, et * cm.UI_EV]
+        rows, x, v = cm.sm1_example_scan(at)
+    with ctx() as c:
+        c.set_atoms(x)
+        c.set_velocities(v)
+        c.integrator_vv(cm.SM1_DT)
+        E = np.vstack([rows, c.run(1000, each=1)])
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.sm1.dat")
+    u = cm.last_digit_units(E, kat)
+    print("sm1 %s: max %.2f units of the last digit, max abs %.2e eV"
+          % ("fp64" if precision == hip.GH_FP64 else "mixed", u.max(), np.abs(E - kat).max()))
+    if precision == hip.GH_FP64:
+        assert np.all(u <= 0.6)
+    else:
+        assert np.all(np.abs(E - kat) <= 2e-8)

@@ -562,3 +562,51 @@ def test_ghost_images_lj_match_reference_binary():
     Fm, _, epm = O.lj_force(f["x"], boxp, 1.0, 1.0, 2.5, nn, lst)
     assert cm.force_err(Fm, f["F"]) > 0.1
     assert abs(epm * cm.UI_EV - f["epot_ev"]) > 1.0
+
+
+# --------------------------------------------------------------------------
+# sm1 (Pairs.F90:916-990)
+# --------------------------------------------------------------------------
+def _sm1_pair(p, r):
+    box = O.make_box(None)
+    x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+    nn, lst, _ = O.neighbours(x, box, p.rc + 0.5)
+    F, _, ep = O.plj_force(x, np.zeros(2, np.int32), box, p, nn, lst)
+    return F, ep * cm.UI_EV
+
+
+def test_sm1_pair_closed_form():
+    # -e inside r1, 0 at the cut, and the reference's force r dp/dr (its
+    # factor r, reproduced as written) pulling the atoms together; the
+    # derivative by central differences of the energy the oracle books
+    r1, r2, e = 1.0, 2.0, 0.01
+    p = O.sm1_params(r1, r2, e)
+    for r in (0.5, 1.0):
+        F, E = _sm1_pair(p, r)
+        assert E == pytest.approx(-e, rel=1e-14) and np.all(F == 0.0)
+    assert _sm1_pair(p, r2)[1] == pytest.approx(0.0, abs=1e-18)
+    for r in (1.1, 1.37, 1.5, 1.83, 1.97):
+        F, E = _sm1_pair(p, r)
+        h = 1e-6
+        dE = (_sm1_pair(p, r + h)[1] - _sm1_pair(p, r - h)[1]) / (2 * h)  # eV/A, pair total
+        # the pair energy is split e/2 + e/2; atom 1 is pulled toward atom 2
+        assert F[0, 0] * cm.UI_EV == pytest.approx(r * dE, rel=1e-7)
+        assert F[1, 0] == pytest.approx(-F[0, 0], rel=1e-14)
+
+
+def test_sm1_example_known_answer():
+    # examples/LJ/sm1.gms against ref/Energy.sm1.dat (1301 rows, 7 digits)
+    p = O.sm1_params(1.0, 2.0, 0.01)
+    box = O.make_box(None)
+
+    def at(x):
+        nn, lst, _ = O.neighbours(x, box, p.rc + 1.0)
+        _, _, ep = O.plj_force(x, np.zeros(2, np.int32), box, p, nn, lst)
+        return [ep * cm.UI_EV, 0.0, ep * cm.UI_EV]
+    rows, x, v = cm.sm1_example_scan(at)
+    md = O.MDRun(x, v, np.full(2, cm.MASS_H), box, types=np.zeros(2), skin=1.0, integ="vv",
+                 dt=cm.SM1_DT, pot="plj", plj=p)
+    E = np.vstack([rows, md.run(1000, each=1)])
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.sm1.dat")
+    assert E.shape == kat.shape
+    assert np.all(cm.last_digit_units(E, kat) <= 0.6)
