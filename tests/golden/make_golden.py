@@ -6,6 +6,7 @@ oracle/ref.mk) on seeded inputs.  Run in the build container only:
     python tests/golden/make_golden.py
     python tests/golden/make_golden.py --plj <path to the GEMS sources>
     python tests/golden/make_golden.py --ghost
+    python tests/golden/make_golden.py --nvt <path to the GEMS sources>
 
 Inputs are written with literal 17-digit numbers (no `{}` expressions or
 `$var$` expansion), so the reference reads exactly the values stored here.
@@ -437,9 +438,33 @@ def nve_plj_checkpoints(ref_dir):
         shutil.rmtree(wd)
 
 
+def nvt_ermak_main(ref_dir):
+    """chp_nvt_s0.chp: the reference's checkpoint of examples/NVT/ermak.gms
+    (the NVE example's 100 He + 100 Ne, `evolve ermak 300 10`) written just
+    before `dinamica`, i.e. after `fill` and `set tempgdist 300` have drawn
+    from the lcg stream; ref_Energy.ermak.dat: the example's own energy file.
+    The 16-digit rows of its first 200 steps are printed as a check that the
+    literal-parameter script is the example's run."""
+    wd = tempfile.mkdtemp(prefix="gold_")
+    try:
+        outs = "outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each 100\n"
+        sc, _, _ = nve_plj_script(ref_dir, 200, outs)
+        sc = sc.replace("evolve v_verlet\ndinamica 200\n",
+                        "evolve ermak 300 10\ncheckpoint write s0.chp\ndinamica 200\n")
+        run(wd, sc)
+        shutil.copy(os.path.join(wd, "s0.chp"), os.path.join(HERE, "chp_nvt_s0.chp"))
+        shutil.copy(os.path.join(ref_dir, "examples", "NVT", "ref", "Energy.ermak.dat"),
+                    os.path.join(HERE, "ref_Energy.ermak.dat"))
+        print(open(os.path.join(wd, "E.dat")).read())
+    finally:
+        shutil.rmtree(wd)
+
+
 if __name__ == "__main__":
     if "--drift" in sys.argv:
         drift_main()
+    elif "--nvt" in sys.argv:
+        nvt_ermak_main(sys.argv[sys.argv.index("--nvt") + 1])
     elif "--ghost" in sys.argv:
         ghost_main()
     elif "--plj" in sys.argv:

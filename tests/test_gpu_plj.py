@@ -152,3 +152,40 @@ def test_restart_from_the_reference_checkpoint(precision):
         back = FM.read_chp(path)
     assert back["nlocal"] == 200 and np.array_equal(back["pos"], x)
     assert back["lcg"] == ck100["lcg"]  # velocity Verlet draws nothing
+
+
+@pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
+def test_nvt_ermak_example_known_answer(precision):
+    """examples/NVT/ermak.gms (ref/Energy.ermak.dat, 7 digits): the NVE
+    example's gas under `evolve ermak 300 10` for 10^4 steps, energy and
+    temperature every 100, from the reference's checkpoint written just before
+    `dinamica` (tests/golden/chp_nvt_s0.chp: positions, velocities and the lcg
+    stream after `fill` and `set tempgdist`).  The Langevin noise comes from
+    the same lcg stream, drawn in the reference's order.  Temperature column:
+    ekin / (3/2 N kB).  Bars: FP64 every entry of all 100 rows within 0.6 of
+    the file's last digit (gamma = 10/ps damps the trajectories' differences,
+    so no window is needed); mixed every entry within 2e-6 of the thermal
+    scale (the largest ekin, or the largest temperature for that column)."""
+    f = _fixture()
+    ck = FM.read_chp(cm.GOLDEN + "/chp_nvt_s0.chp")
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.ermak.dat")
+    with hip.Context(0, precision) as c:
+        FM.load_checkpoint(c, ck, f["types"])
+        c.set_masses(f["mass"])
+        c.set_plj(f["eps"], f["sigma"], float(f["rc"]))
+        c.set_skin(float(f["skin"]))
+        c.integrator_ermak(0.002, 300.0, 10.0, rng=hip.GH_RNG_LCG)
+        en = c.run(10000, each=100)
+    E = np.column_stack([en, en[:, 1] / (1.5 * len(f["types"]) * cm.KB_EV)])
+    u = cm.last_digit_units(E, kat)
+    print("nvt ermak %s: max %.2f units of the last digit (rows 1-10: %.2f), max rel %.2e"
+          % ("fp64" if precision == hip.GH_FP64 else "mixed", u.max(), u[:10].max(),
+             (np.abs(E - kat) / np.abs(kat)).max()))
+    scale = np.array([np.abs(kat[:, 1]).max()] * 3 + [np.abs(kat[:, 3]).max()])
+    dev = np.abs(E - kat).max(axis=0) / scale
+    print("per column max |dE| / thermal scale:", dev)
+    assert E.shape == kat.shape
+    if precision == hip.GH_FP64:
+        assert np.all(u <= 0.6)
+    else:
+        assert np.all(dev <= 2e-6)
