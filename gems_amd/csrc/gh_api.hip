@@ -952,8 +952,8 @@ int gh_set_sm1(gh_ctx *c, double r1, double r2, double e) {
     eps[k] = e;
     one[k] = 1.0;
   }
-  const int nt = std::max(c->nt, 1);
-  return set_pair_table(c, GH_POT_SM1, nt, eps, one, "gh_set_sm1", [&](PljDev &d, int, int) {
+  // the same entry for every pair of the eight types, whatever types the atoms carry
+  return set_pair_table(c, GH_POT_SM1, GH_MAXT, eps, one, "gh_set_sm1", [&](PljDev &d, int, int) {
     d.rs = r1;
     d.rc = r2;
     d.sa8 = (r1 + r2) * 0.5;

@@ -2,7 +2,8 @@
 examples/LJ/{lj,slj,sm1}.gms known-answer files (ref/Energy.lj.dat,
 ref/Energy.slj.dat, ref/Energy.sm1.dat, 7 digits; lj and slj: two H atoms, a 251-point distance scan with
 `out state`, then 1000 velocity-Verlet steps from rest, energy every 10),
-and slj / wca forces (slj_interact, wca_interact, Pairs.F90:403-560) on a
+and slj / wca / sm1 forces (slj_interact, wca_interact, Pairs.F90:403-560;
+sm1_interact, :925-990) on a
 two-type cluster against the oracle.
 
 Bars: FP64 every row of the example files within 0.6 of their last digit;
@@ -66,7 +67,7 @@ def test_lj_examples_known_answer(kind, precision):
         assert np.all(np.abs(E - kat) <= 2e-6)
 
 
-@pytest.mark.parametrize("kind", ["slj", "wca"])
+@pytest.mark.parametrize("kind", ["slj", "wca", "sm1"])
 @pytest.mark.parametrize("precision", [hip.GH_FP64, hip.GH_MIXED], ids=["fp64", "mixed"])
 def test_slj_wca_forces_match_oracle(kind, precision):
     # a jittered two-type simple-cubic cluster, 6^3 sites 1.05 apart: every
@@ -78,7 +79,10 @@ def test_slj_wca_forces_match_oracle(kind, precision):
     types = (np.arange(len(x)) % 2).astype(np.int32)
     eps, sig = np.array([[0.5, 0.7], [0.7, 0.9]]), np.array([[0.9, 1.0], [1.0, 1.1]])
     rs = np.array([[1.05, 1.15], [1.15, 1.25]])
-    p = O.slj_params(eps, sig, rs) if kind == "slj" else O.wca_params(eps, sig)
+    if kind == "sm1":  # one (r1, r2, e) for all pairs: 0.9 < r < 1.6 is the ramp
+        p = O.sm1_params(0.9, 1.6, 0.3, nt=2)
+    else:
+        p = O.slj_params(eps, sig, rs) if kind == "slj" else O.wca_params(eps, sig)
     box = O.make_box(None)
     nn, lst, _ = O.neighbours(x, box, p.rc + 0.5)
     Fo, eo, epo = O.plj_force(x, types, box, p, nn, lst)
@@ -91,6 +95,8 @@ def test_slj_wca_forces_match_oracle(kind, precision):
             c.set_atoms(x + (10.0 if boxed else 0.0), types)
             if kind == "slj":
                 c.set_slj(eps, sig, rs)
+            elif kind == "sm1":
+                c.set_sm1(0.9, 1.6, 0.3)
             else:
                 c.set_wca(eps, sig)
             c.set_skin(0.5)
