@@ -27,6 +27,15 @@ module gems_hip
 
   integer(c_int), parameter :: GH_FP64 = 0, GH_MIXED = 1
   integer(c_int), parameter :: GH_RNG_LCG = 0, GH_RNG_PHILOX = 1
+  integer(c_int), parameter :: GH_POT_SMATB = 1, GH_POT_LJ = 2, GH_POT_PLJ = 3, GH_POT_SLJ = 4, &
+                               GH_POT_WCA = 5, GH_POT_SM1 = 6
+
+  ! gh_stats of the C header
+  type, bind(C) :: gh_stats
+    integer(c_int64_t) :: nupd
+    integer(c_int32_t) :: maxnn, cap, ncells(3), list_mode
+    integer(c_int64_t) :: steps
+  end type
 
   interface
     integer(c_int) function gh_create(ctx, device, precision) bind(C, name='gh_create')
@@ -325,6 +334,54 @@ module gems_hip
       import :: c_ptr, c_double, c_int
       type(c_ptr), value :: b
       real(c_double), intent(out) :: x(*), v(*)
+    end function
+    integer(c_int) function gh_get_stats(ctx, st) bind(C, name='gh_get_stats')
+      import :: c_ptr, c_int, gh_stats
+      type(c_ptr), value :: ctx
+      type(gh_stats), intent(out) :: st
+    end function
+
+    integer(c_int) function gh_set_timing(ctx, enable) bind(C, name='gh_set_timing')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function gh_get_timing(ctx, ms, count) bind(C, name='gh_get_timing')
+      import :: c_ptr, c_int, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms(4)
+      integer(c_int64_t), intent(out) :: count(4)
+    end function
+
+    type(c_ptr) function gh_batch_last_error_c(b) bind(C, name='gh_batch_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    integer(c_int) function gh_batch_set_precision(b, precision) bind(C, name='gh_batch_set_precision')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: precision
+    end function
+
+    integer(c_int) function gh_batch_set_skin(b, skin) bind(C, name='gh_batch_set_skin')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      real(c_double), value :: skin
+    end function
+
+    integer(c_int) function gh_batch_set_velocities(b, v) bind(C, name='gh_batch_set_velocities')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      real(c_double), intent(in) :: v(*)        ! 3 x (all atoms of all systems)
+    end function
+
+    integer(c_int) function gh_batch_compute(b, F, epot) bind(C, name='gh_batch_compute')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: b
+      real(c_double), intent(out) :: F(*)       ! 3 x (all atoms), ui/A
+      real(c_double), intent(out) :: epot(*)    ! per system, eV
     end function
   end interface
 

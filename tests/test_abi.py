@@ -41,3 +41,17 @@ def test_create_without_device_fails_cleanly():
         pytest.skip("a device is present")
     assert not h.value
     assert L.gh_create(ctypes.byref(h), 0, 7) != 0  # bad precision
+
+
+def test_fortran_interfaces_cover_the_header():
+    """Every entry point of include/gems_hip.h has an ISO_C_BINDING interface
+    in fortran/gems_hip.f90 (bind(C, name=...)), so a GEMS build can call all
+    of them."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    h = open(os.path.join(root, "include", "gems_hip.h")).read()
+    f = open(os.path.join(root, "fortran", "gems_hip.f90")).read()
+    declared = set(re.findall(r"\b(gh_[a-z0-9_]+)\s*\(", h))
+    bound = set(re.findall(r"name='(gh_[a-z0-9_]+)'", f))
+    assert declared - bound == set()
+    assert bound - declared == set()
