@@ -31,11 +31,6 @@ static int dalloc(gh_ctx *c, T **p, size_t count) {
   return 0;
 }
 
-#define GH_TRY(x)            \
-  do {                       \
-    int rc_ = (x);           \
-    if (rc_ != 0) return rc_; \
-  } while (0)
 
 static int read_status(gh_ctx *c, GhStatus *h) {
   GH_HIP(c, hipMemcpyAsync(h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, c->stream));
@@ -119,10 +114,60 @@ void gh_tb_coeffs(GhTbPair &c) {
   c.r3 = aux * (20.0 * ab - 8.0 * bb + cb) / 2.0;
 }
 
+double gh_tb_e0(const GhTbPair *tb) { return 1.0 / (tb[0].xi * sqrt(12.0)); }
+
+void gh_tbf_fill(const GhTbPair &t, int k, double e0, TbF &o) {
+  memset(&o, 0, sizeof(o));
+  const double log2e = 1.4426950408889634;
+  const double fxs = ldexp(1.0, k);
+  const float fxsf = (float)fxs, uf = (float)ldexp(1.0, -k);  // powers of two: exact scalings
+  const float r0f = (float)t.r0;
+  const double dr0 = (double)r0f - t.r0;  // r - r0 = s + dr0 with s = r - r0f
+  const double bq = -2.0 * t.q / t.r0 * log2e;
+  const double gam = (t.p - 2.0 * t.q) / t.r0;
+  const double kap = exp(-gam * dr0);  // e^{-p(r/r0-1)} = E2 e^{-gam s} kap
+  o.rce2q = t.rce * t.rce * fxs * fxs;
+  o.r0q = r0f * fxsf;
+  o.rciq = (float)t.rci * fxsf;
+  o.rceq = (float)t.rce * fxsf;
+  o.sc = uf;
+  o.bq = (float)bq * uf;
+  o.eb_a = (float)exp2(bq * dr0);
+  o.rho_a = (float)(t.xi * t.xi * exp2(bq * dr0));
+  const double a = 2.0 * t.A * t.p / t.r0 * kap, B0 = t.xi * t.xi * t.q / t.r0;
+  o.a = (float)a;
+  o.B0 = (float)B0;
+  o.c0 = (float)(a - 2.0 * B0 * e0);
+  o.Ar = (float)(t.A * kap);
+  o.g = (float)(-gam) * uf;
+  o.e02 = (float)(2.0 * e0);
+  o.b1f = (float)t.b1;
+  o.b2f = (float)t.b2;
+  o.b3f = (float)t.b3;
+  o.r1f = (float)t.r1;
+  o.r2f = (float)t.r2;
+  o.r3f = (float)t.r3;
+}
+
+// FP32 forms of every type pair in the unit 2^-k A (the tiled passes' fixed
+// point), uploaded on the context's stream.
+int gh_upload_tbf(gh_ctx *c, int k) {
+  TbF f[GH_MAXT * GH_MAXT];
+  memset(f, 0, sizeof(f));
+  for (int a = 0; a < c->nt; ++a)
+    for (int b = 0; b < c->nt; ++b) gh_tbf_fill(c->tb[a * GH_MAXT + b], k, c->tb_e0, f[a * GH_MAXT + b]);
+  if (!c->tbf) GH_TRY(dalloc(c, &c->tbf, GH_MAXT * GH_MAXT));
+  GH_HIP(c, hipMemcpyAsync(c->tbf, f, sizeof(f), hipMemcpyHostToDevice, c->stream));
+  GH_HIP(c, hipStreamSynchronize(c->stream));  // f lives until the copy is done
+  c->tbf0 = f[0];
+  c->tbf_k = k;
+  return 0;
+}
+
 int gh_upload_tb(gh_ctx *c) {
   TbDev h[GH_MAXT * GH_MAXT];
   memset(h, 0, sizeof(h));
-  const double log2e = 1.4426950408889634;
+  c->tb_e0 = gh_tb_e0(c->tb);
   for (int a = 0; a < c->nt; ++a)
     for (int b = 0; b < c->nt; ++b) {
       const GhTbPair &t = c->tb[a * GH_MAXT + b];
@@ -144,62 +189,14 @@ int gh_upload_tb(gh_ctx *c) {
       d.r3 = t.r3;
       d.p_r0 = t.p / t.r0;
       d.q2_r0 = 2.0 * t.q / t.r0;
-      // A e^{-p(r/r0-1)} = A 2^{-(p/r0) log2e (r - r0)}
-      //                 = [A 2^{b (r0f - r0)}] 2^{b (r - r0f)},  b = -(p/r0) log2e
-      d.r0f = (float)t.r0;
-      const double r0f = (double)d.r0f;
-      const double bp = -t.p / t.r0 * log2e, bq = -2.0 * t.q / t.r0 * log2e;
-      d.rep_b = (float)bp;
-      d.rep_a = (float)(t.A * exp2((double)d.rep_b * (r0f - t.r0)));
-      d.rho_b = (float)bq;
-      d.rho_a = (float)(t.xi * t.xi * exp2((double)d.rho_b * (r0f - t.r0)));
-      d.frep = (float)(2.0 * t.p / t.r0);
-      d.fband = (float)(t.q / t.r0);  // xi^2/2 * 2q/r0 = xi^2 q/r0 (xi^2 inside band)
-      d.rci_f = (float)t.rci;
-      d.rce_f = (float)t.rce;
-      d.b1f = (float)t.b1;
-      d.b2f = (float)t.b2;
-      d.b3f = (float)t.b3;
-      d.r1f = (float)t.r1;
-      d.r2f = (float)t.r2;
-      d.r3f = (float)t.r3;
+      gh_tbf_fill(t, 0, c->tb_e0, d.f);
     }
-  TbF f[GH_MAXT * GH_MAXT];
-  memset(f, 0, sizeof(f));
-  for (int k = 0; k < GH_MAXT * GH_MAXT; ++k) {
-    const TbDev &d = h[k];
-    TbF &o = f[k];
-    o.rce2q = d.rce2 * GH_FXS2;
-    o.r0f = d.r0f;
-    o.rep_a = d.rep_a;
-    o.rep_b = d.rep_b;
-    o.rho_a = d.rho_a;
-    o.rho_b = d.rho_b;
-    o.frep = d.frep;
-    o.fband = d.fband;
-    o.rci_f = d.rci_f;
-    o.rce_f = d.rce_f;
-    o.b1f = d.b1f;
-    o.b2f = d.b2f;
-    o.b3f = d.b3f;
-    o.r1f = d.r1f;
-    o.r2f = d.r2f;
-    o.r3f = d.r3f;
-    o.r0q = d.r0f * 8388608.0f;
-    o.rciq = d.rci_f * 8388608.0f;
-    o.rceq = d.rce_f * 8388608.0f;
-    o.rep_bq = d.rep_b * 1.1920928955078125e-07f;  // 2^-23
-    o.rho_bq = d.rho_b * 1.1920928955078125e-07f;
-  }
   GH_TRY(dalloc(c, &c->tbd, GH_MAXT * GH_MAXT));
-  GH_TRY(dalloc(c, &c->tbf, GH_MAXT * GH_MAXT));
   // ordered on the context's stream (a non-blocking stream does not wait for
-  // null-stream copies); h and f live until the stream has drained
+  // null-stream copies); h lives until the stream has drained
   GH_HIP(c, hipMemcpyAsync(c->tbd, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-  GH_HIP(c, hipMemcpyAsync(c->tbf, f, sizeof(f), hipMemcpyHostToDevice, c->stream));
   GH_HIP(c, hipStreamSynchronize(c->stream));
-  c->tbf0 = f[0];
-  return 0;
+  return gh_upload_tbf(c, c->tbf_k < 0 ? GH_FXK_MAX : c->tbf_k);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,6 +269,15 @@ static void finish_tiles(gh_ctx *c) {
   }
   t.nkey = t.nbrick * t.cpb;
   c->ncell = t.nkey;
+  // fixed-point unit of the mixed SMA-TB tiles: a staged atom lies within half
+  // a brick plus its halo cell of the brick centre, plus half the skin (the
+  // largest displacement between rebuilds); 1 A of margin
+  double ext = 0.0;
+  for (int d = 0; d < 3; ++d) ext = std::max(ext, (0.5 * t.bd[d] + 1.0) * c->cell[d]);
+  ext += 0.5 * c->skin + 1.0;
+  t.fxk = GH_FXK_MAX;
+  while (t.fxk > 0 && ext * ldexp(1.0, t.fxk) >= 1073741824.0) --t.fxk;
+  t.fxs = ldexp(1.0, t.fxk);
 }
 
 static void setup_grid(gh_ctx *c) {

@@ -48,6 +48,7 @@ struct gh_batch {
   unsigned short *list = nullptr;
   TbDev *tbd = nullptr;
   TbF *tbf = nullptr;
+  double tb_e0 = 0.0;
   int nst = 0;   // LDS staging slots per workgroup (largest system + sentinel)
   int nmax = 0;  // atoms of the largest system
   // systems above GH_BIGSYS atoms (the first nbig of the largest-first order)
@@ -82,6 +83,7 @@ struct BatchArgs {
   double *out;  // per system: [0] epot ui, [1] ekin ui, [2] rebuilds, [3] fmax eV/A,
                 // [4] iterations, [5] converged
   int *flags;   // [0] list overflow
+  double e0;    // eband reference of the FP32 pair forms
 };
 
 #define OUTW 6
@@ -129,7 +131,7 @@ __device__ __forceinline__ double bmax(BlockRed<BS> &R, double v) {
 // positions (fixed point around its first atom) and the parameter tables.
 struct Sys {
   int a0, n;
-  bool staged;          // positions are in q (all within 64 A of the origin)
+  bool staged;          // positions are in q (all within 64 A of the origin, 2^24 steps per A)
   int4 *q;              // [n + 1]: positions, payload; q[n] = sentinel
   unsigned char *ty;    // [n + 1] types
   const TbDev *sp;      // FP64 parameters
@@ -147,8 +149,8 @@ __device__ void sys_stage(const BatchArgs &B, Sys &S, int *flag, bool multi) {
   __syncthreads();
   for (int k = threadIdx.x; k < S.n; k += blockDim.x) {
     const double4 p = B.pos[S.a0 + k];
-    const double dx = (p.x - o.x) * GH_FXS, dy = (p.y - o.y) * GH_FXS,
-                 dz = (p.z - o.z) * GH_FXS;
+    const double dx = (p.x - o.x) * GH_BFXS, dy = (p.y - o.y) * GH_BFXS,
+                 dz = (p.z - o.z) * GH_BFXS;
     if (fabs(dx) >= GH_FXLIM || fabs(dy) >= GH_FXLIM || fabs(dz) >= GH_FXLIM) *flag = 1;
     const int t = B.type[S.a0 + k];
     S.q[k] = make_int4(__double2int_rn(dx), __double2int_rn(dy), __double2int_rn(dz), t);
@@ -183,7 +185,7 @@ __device__ void sys_build(const BatchArgs &B, const Sys &S) {
           const int4 qj = S.q[j];
           const float ux = (float)(qj.x - qi.x), uy = (float)(qj.y - qi.y),
                       uz = (float)(qj.z - qi.z);
-          const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (float)(1.0 / GH_FXS2);
+          const float r2f = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (float)(1.0 / (GH_BFXS * GH_BFXS));
           if (r2f > B.hi2f || ((r2f < B.near2f) != (sweep == 0))) continue;
           if (r2f >= B.lo2f) {
             const double4 pj = B.pos[a0 + j];
@@ -223,8 +225,7 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
     if (fx) {
       const int4 qi = S.q[k];
       auto pair = [&](const int4 &q) {
-        const double dx = (double)(q.x - qi.x), dy = (double)(q.y - qi.y),
-                     dz = (double)(q.z - qi.z);
+        const double dx = fxd(q.x, qi.x), dy = fxd(q.y, qi.y), dz = fxd(q.z, qi.z);
         float band;
         if (tb_rho_fx(SINGLE ? S.sf[0] : S.sf[ti * GH_MAXT + q.w],
                       fma(dx, dx, fma(dy, dy, dz * dz)), band))
@@ -247,11 +248,8 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
         const TbDev &P = SINGLE ? S.sp[0] : S.sp[ti * GH_MAXT + B.type[j]];
         const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
         if (MIXED) {
-          const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-          if (r2 <= P.rce2) {
-            const float r2f = (float)r2;
-            acc += (double)tb_band32(P, r2f * __builtin_amdgcn_rsqf(r2f));
-          }
+          float band;
+          if (tb_rho_fx(P.f, fma(vx, vx, fma(vy, vy, vz * vz)), band)) acc += (double)band;
         } else {
           const double dr = sqrt(dist2_rn(vx, vy, vz));
           if (dr <= P.rce) acc += tb_band64(P, dr);
@@ -261,7 +259,7 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
     B.rho[i] = acc;
     const double eb = 1.0 / sqrt(acc);
     B.eband[i] = eb;
-    B.eband32[i] = (float)eb;
+    B.eband32[i] = (float)(eb - B.e0);  // the FP32 forms take 1/sqrt(rho) - e0
   }
   __syncthreads();
   if (fx) {
@@ -276,14 +274,13 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
     double fxs = 0.0, fys = 0.0, fzs = 0.0, ei = 0.0, scale = B.ev_ui;
     if (fx) {
       const int4 qi = S.q[k];
-      const float ebi = __int_as_float(qi.w);
+      const float di = __int_as_float(qi.w);
       auto pair = [&](const int4 &q, int s) {
-        const double dx = (double)(q.x - qi.x), dy = (double)(q.y - qi.y),
-                     dz = (double)(q.z - qi.z);
+        const double dx = fxd(q.x, qi.x), dy = fxd(q.y, qi.y), dz = fxd(q.z, qi.z);
         float sf, repul;
-        if (tb_force_fx(SINGLE ? S.sf[0] : S.sf[ti * GH_MAXT + S.ty[s]],
-                        fma(dx, dx, fma(dy, dy, dz * dz)), ebi + __int_as_float(q.w), sf,
-                        repul)) {
+        if (tb_force_fx<true>(SINGLE ? S.sf[0] : S.sf[ti * GH_MAXT + S.ty[s]],
+                              fma(dx, dx, fma(dy, dy, dz * dz)), di + __int_as_float(q.w), sf,
+                              repul)) {
           const double sfd = (double)sf;
           fxs = fma(-sfd, dx, fxs);
           fys = fma(-sfd, dy, fys);
@@ -300,11 +297,11 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
         pair(q2, s2);
         pair(q3, s3);
       });
-      scale = B.ev_ui;  // tb_force_fx returns factor / r_q: forces come out in ui/A
+      scale = B.ev_ui;  // tb_force_fx returns factor / r per step: forces come out in ui/A
     } else {
       const double4 pi = B.pos[i];
       const double ebi = B.eband[i];
-      const float ebi32 = B.eband32[i];
+      const float di = B.eband32[i];
       for_each_slot(B.list, B.ld, i, B.nn[i], [&](int s) {
         if (s >= n) return;  // sentinel
         const int j = a0 + s;
@@ -312,13 +309,10 @@ __device__ void sys_forces(const BatchArgs &B, const Sys &S) {
         const TbDev &P = SINGLE ? S.sp[0] : S.sp[ti * GH_MAXT + B.type[j]];
         const double vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
         if (MIXED) {
-          const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-          if (r2 <= P.rce2) {
-            const float r2f = (float)r2;
-            const float rinv = __builtin_amdgcn_rsqf(r2f);
-            float factor, repul;
-            tb_pair32(P, r2f * rinv, ebi32 + B.eband32[j], factor, repul);
-            const double sf = (double)(factor * rinv);
+          float sf32, repul;
+          if (tb_force_fx<true>(P.f, fma(vx, vx, fma(vy, vy, vz * vz)), di + B.eband32[j],
+                                sf32, repul)) {
+            const double sf = (double)sf32;
             fxs = fma(-sf, vx, fxs);
             fys = fma(-sf, vy, fys);
             fzs = fma(-sf, vz, fzs);
@@ -688,6 +682,7 @@ static BatchArgs bargs(const gh_batch *b) {
   A.ld = b->ld;
   A.tbd = b->tbd;
   A.tbf = b->tbf;
+  A.e0 = b->tb_e0;
   A.nt = b->nt;
   A.nst = b->nst;
   for (int t = 0; t < GH_MAXT; ++t) {
@@ -921,6 +916,7 @@ int gh_batch_set_smatb(gh_batch *b, int nt, const double *A, const double *xi, c
   // reuse the single-system parameter upload through a scratch context
   gh_ctx tmp;
   tmp.device = b->device;
+  tmp.tbf_k = GH_BFXK;  // the FP32 forms in the unit of the staged positions
   int rc = gh_set_smatb(&tmp, nt, A, xi, p, q, r0, rci, rce);
   if (rc) return bfail(b, rc, tmp.err);
   for (int k = 0; k < GH_MAXT * GH_MAXT; ++k) b->tb[k] = tmp.tb[k];
@@ -930,6 +926,7 @@ int gh_batch_set_smatb(gh_batch *b, int nt, const double *A, const double *xi, c
   if (b->tbf) BH(b, hipFree(b->tbf));
   b->tbf = tmp.tbf;
   tmp.tbf = nullptr;
+  b->tb_e0 = tmp.tb_e0;
   b->nt = nt;
   b->rcut = tmp.rcut;
   b->have_pot = true;

@@ -79,15 +79,6 @@ __device__ __forceinline__ double tb_band64(const TbDev &P, double dr) {
   return b * b;
 }
 
-// FP32 forms take TbDev or TbF (same FP32 field names).
-template <class TP>
-__device__ __forceinline__ float tb_band32(const TP &P, float r) {
-  if (r < P.rci_f) return P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * (r - P.r0f));
-  const float d = r - P.rce_f;
-  const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-  return pp * pp;
-}
-
 // factor = -dE/dr of the pair; repul = the pair's repulsion booked on atom i.
 __device__ __forceinline__ void tb_pair64(const TbDev &P, double dr, double ebs, double &factor,
                                           double &repul) {
@@ -105,25 +96,6 @@ __device__ __forceinline__ void tb_pair64(const TbDev &P, double dr, double ebs,
   }
 }
 
-template <class TP>
-__device__ __forceinline__ void tb_pair32(const TP &P, float r, float ebs, float &factor,
-                                          float &repul) {
-  if (r <= P.rci_f) {
-    const float s0 = r - P.r0f;
-    repul = P.rep_a * __builtin_amdgcn_exp2f(P.rep_b * s0);
-    const float band = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * s0);
-    factor = fmaf(repul, P.frep, -band * P.fband * ebs);
-  } else {
-    const float d = r - P.rce_f;
-    const float d2 = d * d, d3 = d2 * d;
-    const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-    const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
-    const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
-    factor = fmaf(pp * dp, ebs, -2.0f * dr_);
-    repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Neighbour rows ("quad ELL"): entry k of atom a is the 16-bit word
 // ((k / 4) * ld + a) * 4 + k % 4, so one 8-byte load per lane fetches four
@@ -134,12 +106,18 @@ __device__ __forceinline__ unsigned long long ld_quad(const unsigned long long *
   return __builtin_nontemporal_load(p);
 }
 
-// Fixed-point positions (GH_FXS steps per Angstrom).  Staged atoms lie within
-// 2^29 steps (64 A) of their frame's origin and the sentinel at 2^30, so no
-// difference of two staged values overflows 32 bits.
-__device__ __forceinline__ int fxq(double d) { return __double2int_rn(d * GH_FXS); }
-#define GH_FXFAR (1 << 30)
-#define GH_FXLIM 536870912.0  // 2^29
+// Fixed-point positions: staged atoms lie within 2^30 steps of their frame's
+// origin and the sentinel slot holds INT_MIN in every coordinate, so the
+// difference of two atoms never overflows 32 bits and the wrapped difference
+// between the sentinel and any atom exceeds 2^30 steps in magnitude on every
+// axis (far outside every cut).  Differences are taken in unsigned
+// arithmetic (wrapping) and read back as int.
+__device__ __forceinline__ int fxq(double d, double fxs) { return __double2int_rn(d * fxs); }
+__device__ __forceinline__ double fxd(int a, int b) {
+  return (double)(int)((unsigned)a - (unsigned)b);
+}
+#define GH_FXFAR ((int)0x80000000)
+#define GH_FXLIM 1073741824.0  // 2^30
 #define GH_FAR 1.0e30         // sentinel slot, FP64
 
 struct RowWriter {  // quad-packed row of one atom
@@ -221,46 +199,75 @@ __device__ __forceinline__ void for_each_quad(const unsigned short *__restrict__
 }
 #define GH_QSLOT(w, u) ((int)(((w) >> (16 * (u))) & 0xFFFFull))
 
-// SMA-TB pair terms on fixed-point pair vectors (integer steps, exact in
-// FP64): r^2 in FP64, then the FP32 forms of TbDev/TbF evaluated in
-// fixed-point length units (r_q = r * 2^23; every scaling is by a power of
-// two, so each FP32 result equals the Angstrom-unit one bit for bit).
-// rho: the density term; force: factor/r_q (per step of the pair vector)
-// and the repulsion.
-__device__ __forceinline__ float tb_band_q(const TbF &P, float rq) {
-  if (rq < P.rciq) return P.rho_a * __builtin_amdgcn_exp2f(P.rho_bq * (rq - P.r0q));
-  const float d = (rq - P.rceq) * 1.1920928955078125e-07f;
+// SMA-TB FP32 pair forms (the TbF terms; r2 in the TbF unit, FP64).  r is
+// the FP32 r2 times its hardware reciprocal square root y; for the inner
+// branch s = r - r0f is recovered to FP32 precision of s itself from the
+// residual r2 - r_h^2, exact in FP64 (r_h^2 of a float is exact), so the
+// rounding of r (a few 1e-7 A) never enters the exponentials.
+__device__ __forceinline__ float tb_s(const TbF &P, double r2, float y, float rh) {
+  const float e = (float)fma(-(double)rh, (double)rh, r2);
+  return fmaf(e, 0.5f * y, rh - P.r0q);
+}
+// e^x - 1 to about one FP32 ulp of the result: x + x^2 (1/2 + x Q(x)) for
+// |x| < 1/2 (Q a least-squares fit of (e^x - 1 - x - x^2/2)/x^3 there), else
+// from the hardware exponential.
+__device__ __forceinline__ float tb_em1(float x) {
+  if (fabsf(x) < 0.5f) {
+    float q = fmaf(1.9983829406555742e-04f, x, 1.3986654812470078e-03f);
+    q = fmaf(q, x, 8.3331037312746048e-03f);
+    q = fmaf(q, x, 4.1665747761726379e-02f);
+    q = fmaf(q, x, 1.6666667163372040e-01f);
+    return fmaf(x * x, fmaf(q, x, 0.5f), x);
+  }
+  return __builtin_amdgcn_exp2f(x * 1.4426950408889634f) - 1.0f;
+}
+__device__ __forceinline__ float tb_tail_band(const TbF &P, float d) {
   const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
   return pp * pp;
 }
-__device__ __forceinline__ bool tb_rho_fx(const TbF &P, double r2q, float &band) {
-  if (r2q > P.rce2q) return false;
-  const float r2f = (float)r2q;
-  band = tb_band_q(P, r2f * __builtin_amdgcn_rsqf(r2f));
+// Density term of a pair within rce (smatb_preinteraction, TB.F90:405-423).
+__device__ __forceinline__ bool tb_rho_fx(const TbF &P, double r2, float &band) {
+  if (r2 > P.rce2q) return false;
+  const float r2f = (float)r2;
+  const float y = __builtin_amdgcn_rsqf(r2f);
+  const float rh = r2f * y;
+  if (rh < P.rciq) {
+    band = P.rho_a * __builtin_amdgcn_exp2f(P.bq * tb_s(P, r2, y, rh));
+  } else {
+    band = tb_tail_band(P, (rh - P.rceq) * P.sc);
+  }
   return true;
 }
-__device__ __forceinline__ bool tb_force_fx(const TbF &P, double r2q, float ebs, float &sf,
+// Force of a pair within rce (smatb_interact, TB.F90:493-531): sf = factor/r
+// per unit of the pair vector, repul the pair's repulsion (ENERGY only);
+// dl = d_i + d_j (eband offsets from e0).
+template <bool ENERGY>
+__device__ __forceinline__ bool tb_force_fx(const TbF &P, double r2, float dl, float &sf,
                                             float &repul) {
-  if (r2q > P.rce2q) return false;
-  const float r2f = (float)r2q;
-  const float rinv = __builtin_amdgcn_rsqf(r2f);  // 1 / r_q
-  const float rq = r2f * rinv;
+  if (r2 > P.rce2q) return false;
+  const float r2f = (float)r2;
+  const float y = __builtin_amdgcn_rsqf(r2f);
+  const float rh = r2f * y;
   float factor;
-  if (rq <= P.rciq) {
-    const float s0 = rq - P.r0q;
-    repul = P.rep_a * __builtin_amdgcn_exp2f(P.rep_bq * s0);
-    const float band = P.rho_a * __builtin_amdgcn_exp2f(P.rho_bq * s0);
-    factor = fmaf(repul, P.frep, -band * P.fband * ebs);
+  if (rh <= P.rciq) {
+    const float s = tb_s(P, r2, y, rh);
+    const float e2 = P.eb_a * __builtin_amdgcn_exp2f(P.bq * s);
+    const float em1 = tb_em1(P.g * s);
+    factor = e2 * fmaf(P.a, em1, fmaf(-P.B0, dl, P.c0));
+    if (ENERGY) {
+      const float ar = P.Ar * e2;
+      repul = fmaf(ar, em1, ar);
+    }
   } else {
-    const float d = (rq - P.rceq) * 1.1920928955078125e-07f;
+    const float d = (rh - P.rceq) * P.sc;
     const float d2 = d * d, d3 = d2 * d;
     const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
     const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
     const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
-    factor = fmaf(pp * dp, ebs, -2.0f * dr_);
-    repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
+    factor = fmaf(pp * dp, P.e02 + dl, -2.0f * dr_);
+    if (ENERGY) repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
   }
-  sf = factor * rinv;
+  sf = factor * y;
   return true;
 }
 

@@ -47,32 +47,45 @@ struct GhTbPair {
 };
 void gh_tb_coeffs(GhTbPair &c);
 
-// Kernel-side parameter block (one per type pair), FP64 and FP32 forms.
-struct TbDev {
-  // FP64 form
-  double ca, eps2, p, q, r0, one_r0, rci, rce, rce2, b1, b2, b3, r1, r2, r3;
-  double p_r0, q2_r0;  // p/r0, 2q/r0
-  // FP32 form: A e^{-p(r/r0-1)} = rep_a * 2^{rep_b (r - r0f)} with r0f the
-  // float nearest r0 (r - r0f is exact near r0) and rep_a compensating the
-  // rounding of r0 in FP64, so no constant of the exponent carries a bias.
-  float r0f;
-  float rep_a, rep_b;
-  float rho_a, rho_b;  // xi^2 e^{-2q(r/r0-1)} = rho_a * 2^{rho_b (r - r0f)}
-  float frep, fband;   // 2p/r0 and q/r0 (for -dE/dr)
-  float rci_f, rce_f;
+// FP32 form of one type pair's SMA-TB terms (the mixed-precision passes), in
+// a length unit u = 2^-k Angstrom (k = 0: Angstrom; the fixed-point tiles
+// use the unit of their integer positions).  With s = r - r0f (r0f the float
+// nearest r0; s is formed from r^2 in FP64 so that it carries no rounding of
+// r itself) and E2 = e^{-2q(r/r0-1)} = eb_a 2^{bq s}:
+//   density   xi^2 E2                               = rho_a 2^{bq s}
+//   force     2A(p/r0) e^{-p(r/r0-1)} - xi^2 (q/r0) E2 (eb_i + eb_j)
+//           = E2 [a em1(g s) + c0 - B0 (d_i + d_j)]
+//   energy    A e^{-p(r/r0-1)} = Ar E2 (1 + em1(g s))
+// where em1(x) = e^x - 1, g s = -(p - 2q)(r - r0f)/r0, eb = e0 + d with e0 a
+// context constant near every atom's 1/sqrt(rho) (d stays small, so its FP32
+// rounding is far below eb's), a and Ar carry e^{-(p-2q)(r0f-r0)/r0} and c0 =
+// a - 2 B0 e0 is rounded once from FP64.  The two large terms of the force
+// (about 1.46 eV/A each for an Au nearest-neighbour pair, 0.03 apart) thereby
+// cancel before any FP32 rounding.  Tail (rci < r <= rce): the quintic
+// polynomials of TB.F90:134-149 in Angstrom, d = (r - rce).
+struct TbF {
+  double rce2q;           // rce^2 in u^2
+  float r0q, rciq, rceq;  // r0f, rci, rce in u
+  float sc;               // u in Angstrom (2^-k)
+  float rho_a, bq;        // density amplitude; -2q/r0 log2(e) per u
+  float eb_a;             // E2 = eb_a 2^{bq s}
+  float a, c0, B0, Ar;    // force bracket and repulsion amplitude (eV, eV/A)
+  float g;                // -(p - 2q)/r0 per u
+  float e02;              // 2 e0 (tail terms)
   float b1f, b2f, b3f, r1f, r2f, r3f;
 };
 
-// Compact FP32 form for the mixed-precision tiled passes (positions in LDS
-// as fixed point, 2^23 steps per Angstrom; rce2q = rce^2 in those units).
-struct TbF {
-  double rce2q;
-  float r0f, rep_a, rep_b, rho_a, rho_b, frep, fband, rci_f, rce_f;
-  float b1f, b2f, b3f, r1f, r2f, r3f;
-  // the same in fixed-point length units (r_q = r * 2^23, exact scalings):
-  // r0q, rciq, rceq = r0f, rci_f, rce_f * 2^23; rep_bq, rho_bq = rep_b, rho_b * 2^-23
-  float r0q, rciq, rceq, rep_bq, rho_bq, pad;
+// Kernel-side parameter block (one per type pair): the FP64 form (the
+// reference's arithmetic) and the FP32 form in Angstrom.
+struct TbDev {
+  double ca, eps2, p, q, r0, one_r0, rci, rce, rce2, b1, b2, b3, r1, r2, r3;
+  double p_r0, q2_r0;  // p/r0, 2q/r0
+  TbF f;               // FP32 form, u = 1 A
 };
+// Fills the FP32 form of pair t in the unit 2^-k A with the context constant e0.
+void gh_tbf_fill(const GhTbPair &t, int k, double e0, TbF &o);
+// e0 of a parameter set: 1/sqrt(rho) of a type-0 atom with twelve neighbours at r0.
+double gh_tb_e0(const GhTbPair *tb);
 // Tabulated Lennard-Jones forms per type pair (plj, slj, wca; the context's
 // pot says which): prc and frc are plj's value and slope terms at the cut
 // (plj_set, Pairs.F90:282-291); rs, sa4, sa8 slj's spline (slj_set,
@@ -84,8 +97,14 @@ struct PljDev {
 };
 inline bool gh_is_table_pot(int pot) { return pot >= 3 && pot <= 6; }
 
-#define GH_FXS 8388608.0         // fixed-point steps per Angstrom (2^23)
-#define GH_FXS2 70368744177664.0 // GH_FXS^2 (2^46)
+// Fixed-point positions of the mixed-precision SMA-TB tiles: 2^fxk steps per
+// Angstrom relative to each brick's centre, with fxk (at most GH_FXK_MAX)
+// chosen at grid set-up so that every staged atom lies within 2^30 steps of
+// it.  The batched clusters stage positions relative to each system's first
+// atom at 2^GH_BFXK steps per Angstrom (systems within 64 A of that atom).
+#define GH_FXK_MAX 25
+#define GH_BFXK 24
+#define GH_BFXS 16777216.0  // 2^GH_BFXK
 
 // Cell grid and its grouping into bricks.  Atoms are sorted by key =
 // brick * cpb + cell-within-brick, so each brick's atoms are contiguous and
@@ -100,6 +119,8 @@ struct TileGeom {
   int nbrick;     // bricks
   int nkey;       // nbrick * cpb
   int slot_cap;   // LDS slots per workgroup (halo atoms)
+  int fxk;        // fixed-point steps per Angstrom: 2^fxk (mixed SMA-TB)
+  double fxs;     // 2^fxk
   double L[3];
   double one_cell[3];
 };
@@ -202,6 +223,8 @@ struct gh_ctx {
   TbDev *tbd = nullptr;                       // [MAXT*MAXT]
   TbF *tbf = nullptr;                         // [MAXT*MAXT]
   TbF tbf0;                                   // host copy of tbf[0] (one-type kernel argument)
+  int tbf_k = -1;                             // fixed-point unit exponent tbf was filled for
+  double tb_e0 = 0.0;                         // eband reference of the FP32 forms
   PljDev *pljd = nullptr;                     // [MAXT*MAXT] shifted LJ per type pair
   // one-sided interactions: i feels j iff mfeel[i] & mfelt[j] (caller order)
   int *mfeel = nullptr, *mfelt = nullptr;
@@ -232,6 +255,12 @@ int gh_fail(gh_ctx *c, int code, const std::string &msg);
     hipError_t e_ = (call);                                                \
     if (e_ != hipSuccess)                                                  \
       return gh_fail(c, GH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define GH_TRY(x)             \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ != 0) return rc_; \
   } while (0)
 
 // Velocity Verlet fused into the force pass (stepb of this step, stepa of the
@@ -268,3 +297,4 @@ int gh_launch_wrap(gh_ctx *c);
 int gh_tile_occupancy(gh_ctx *c, int *max_halo, int *max_brick);
 int gh_tiled_to_global(gh_ctx *c, int *dst);
 int gh_upload_tb(gh_ctx *c);
+int gh_upload_tbf(gh_ctx *c, int k);

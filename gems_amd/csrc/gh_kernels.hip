@@ -4,7 +4,7 @@
 //   pos    double4[ncap]   x, y, z, w unused             32 B/atom
 //   vel, acc, frc          3 x double[ncap] (SoA)         24 B/atom each
 //   epa, rho, eband        double[ncap]
-//   eband32                float[ncap] (mixed precision gathers)
+//   eband32                float[ncap] (mixed precision: 1/sqrt(rho) - e0)
 //   type, orig, cellof     int[ncap]
 //   list   uint32[cap][ld] ELL: row k of all atoms is contiguous, so a
 //          wavefront reading entry k of 64 consecutive atoms is coalesced.
@@ -277,15 +277,16 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force64(
 
 // ---------------------------------------------------------------------------
 // Mixed precision SMA-TB: positions and the pair vector stay FP64 (the
-// difference x_j - x_i is exact to FP64 rounding); distance, exponentials and
-// the pair force factor are FP32; rho, energies and forces accumulate in FP64.
+// difference x_j - x_i is exact to FP64 rounding); the pair terms are the
+// FP32 forms of TbDev::f (Angstrom units; r^2 in FP64); rho, energies and
+// forces accumulate in FP64.  Pass 1 stores d = 1/sqrt(rho) - e0 in FP32.
 // ---------------------------------------------------------------------------
 template <bool SINGLE>
 __global__ __launch_bounds__(GH_BLOCK) void k_tb_rho_mx(
     const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
     const unsigned *__restrict__ list, int ld, int n, const TbDev *__restrict__ tbd, int nt,
     double L0, double L1, double L2, double *__restrict__ rho, double *__restrict__ eband,
-    float *__restrict__ eband32) {
+    float *__restrict__ ebd32, double e0) {
   __shared__ ShiftLds sh;
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
   load_shift(sh, L0, L1, L2);
@@ -310,24 +311,13 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_rho_mx(
     const double vy = (pj.y - pi.y) + sh.y[w];
     const double vz = (pj.z - pi.z) + sh.z[w];
     const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
-    const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-    if (r2 > P.rce2) continue;
-    const float r2f = (float)r2;
-    const float r = r2f * __builtin_amdgcn_rsqf(r2f);
-    float phi;
-    if (r < P.rci_f) {
-      phi = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * (r - P.r0f));
-    } else {
-      const float d = r - P.rce_f;
-      const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-      phi = pp * pp;
-    }
-    acc += (double)phi;
+    float band;
+    if (tb_rho_fx(P.f, fma(vx, vx, fma(vy, vy, vz * vz)), band)) acc += (double)band;
   }
   rho[i] = acc;
   const double eb = 1.0 / sqrt(acc);
   eband[i] = eb;
-  eband32[i] = (float)eb;
+  ebd32[i] = (float)(eb - e0);
 }
 
 template <bool SINGLE>
@@ -335,7 +325,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
     const double4 *__restrict__ pos, const int *__restrict__ type, const int *__restrict__ nn,
     const unsigned *__restrict__ list, int ld, int n, int ncap, const TbDev *__restrict__ tbd,
     int nt, double L0, double L1, double L2, const double *__restrict__ rho,
-    const float *__restrict__ eband32, double ev_ui, double *__restrict__ frc,
+    const float *__restrict__ ebd32, double ev_ui, double *__restrict__ frc,
     double *__restrict__ epa) {
   __shared__ ShiftLds sh;
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
@@ -350,7 +340,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
   if (i >= n) return;
   const double4 pi = pos[i];
   const int ti = SINGLE ? 0 : type[i];
-  const float ebi = eband32[i];
+  const float di = ebd32[i];
   const int m = nn[i];
   double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
   for (int k = 0; k < m; ++k) {
@@ -362,28 +352,10 @@ __global__ __launch_bounds__(GH_BLOCK) void k_tb_force_mx(
     const double vy = (pj.y - pi.y) + sh.y[w];
     const double vz = (pj.z - pi.z) + sh.z[w];
     const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + type[j]];
-    const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
-    if (r2 > P.rce2) continue;
-    const float r2f = (float)r2;
-    const float rinv = __builtin_amdgcn_rsqf(r2f);
-    const float r = r2f * rinv;
-    const float ebs = ebi + eband32[j];
-    float factor, repul;
-    if (r <= P.rci_f) {
-      const float s0 = r - P.r0f;
-      repul = P.rep_a * __builtin_amdgcn_exp2f(P.rep_b * s0);
-      const float band = P.rho_a * __builtin_amdgcn_exp2f(P.rho_b * s0);
-      factor = fmaf(repul, P.frep, -band * P.fband * ebs);
-    } else {
-      const float d = r - P.rce_f;
-      const float d2 = d * d, d3 = d2 * d;
-      const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-      const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
-      const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
-      factor = fmaf(pp * dp, ebs, -2.0f * dr_);
-      repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
-    }
-    const double s = (double)(factor * rinv);
+    float sf, repul;
+    if (!tb_force_fx<true>(P.f, fma(vx, vx, fma(vy, vy, vz * vz)), di + ebd32[j], sf, repul))
+      continue;
+    const double s = (double)sf;
     fx = fma(-s, vx, fx);
     fy = fma(-s, vy, fy);
     fz = fma(-s, vz, fz);
@@ -942,11 +914,11 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (single)
         k_tb_rho_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                   c->nt, L0, L1, L2, c->rho, c->eband,
-                                                  c->eband32);
+                                                  c->eband32, c->tb_e0);
       else
         k_tb_rho_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                    c->nt, L0, L1, L2, c->rho, c->eband,
-                                                   c->eband32);
+                                                   c->eband32, c->tb_e0);
     } else {
       if (single)
         k_tb_rho64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,

@@ -17,10 +17,10 @@
 // consecutive entries and a wavefront's load covers 512 contiguous bytes.
 //
 // Mixed precision stages positions as fixed point relative to the brick
-// centre, GH_FXS = 2^23 steps per Angstrom (|x - centre| < 256 A): pair
-// vectors are exact integer differences, converted exactly to FP64, and r^2 is
-// exact in FP64 for every pair within 7 A, so the only rounding of a pair's
-// geometry is the 2^-24 quantisation of each position (6e-8 A).
+// centre, 2^fxk steps per Angstrom (fxk <= 25, chosen so that every staged
+// atom lies within 2^30 steps): pair vectors are exact integer differences,
+// converted exactly to FP64, so the only rounding of a pair's geometry is the
+// 2^-(fxk+1) A quantisation of each position (1.5e-8 A at fxk = 25).
 #include <math.h>
 #include <cstring>
 
@@ -767,16 +767,18 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
 }
 
 // ---------------------------------------------------------------------------
-// SMA-TB, mixed precision (fixed-point LDS).  Pass 1: smatb_preinteraction
-// (TB.F90:367-455); LDS per slot: qx, qy, qz in three int arrays (one
-// 16-B array reads with more bank conflicts), the slot map and the type.
+// SMA-TB, mixed precision (fixed-point LDS, 2^t.fxk steps per Angstrom about
+// the brick centre; the FP32 pair forms of TbF in that unit).  Pass 1:
+// smatb_preinteraction (TB.F90:367-455); LDS per slot: qx, qy, qz in three
+// int arrays (one 16-B array reads with more bank conflicts), the slot map
+// and the type.  Stores rho (when asked) and d = 1/sqrt(rho) - e0 in FP32.
 // ---------------------------------------------------------------------------
 template <bool SINGLE>
 __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbF *__restrict__ tbf,
-    double *__restrict__ rho, float *__restrict__ eband32, TbF p1) {
+    double *__restrict__ rho, float *__restrict__ ebd32, double e0, TbF p1) {
   extern __shared__ int qa[];
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
@@ -790,13 +792,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
   unsigned char *scell = reinterpret_cast<unsigned char *>(qa + 3 * ns);
   unsigned char *ty = scell + ns;
   slot_map(H, scell);
+  const double fxs = t.fxs;
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
     const int src = s + H.delta[h];
     const double4 p = pos[src];
-    qxa[s] = fxq(p.x - (H.org[0] - shift(t, H, 0, h)));
-    qya[s] = fxq(p.y - (H.org[1] - shift(t, H, 1, h)));
-    qza[s] = fxq(p.z - (H.org[2] - shift(t, H, 2, h)));
+    qxa[s] = fxq(p.x - (H.org[0] - shift(t, H, 0, h)), fxs);
+    qya[s] = fxq(p.y - (H.org[1] - shift(t, H, 1, h)), fxs);
+    qza[s] = fxq(p.z - (H.org[2] - shift(t, H, 2, h)), fxs);
     if (!SINGLE) ty[s] = (unsigned char)type[src];
   }
   if (threadIdx.x == 0) {
@@ -806,12 +809,12 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
   __syncthreads();
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
-    const int qx = fxq(pa.x - (H.org[0] - 0.0)), qy = fxq(pa.y - (H.org[1] - 0.0)),
-              qz = fxq(pa.z - (H.org[2] - 0.0));
+    const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
+              qz = fxq(pa.z - H.org[2], fxs);
     const int ti = SINGLE ? 0 : type[a];
     double acc = 0.0;
     auto pair = [&](int jx, int jy, int jz, int s) {
-      const double dx = (double)(jx - qx), dy = (double)(jy - qy), dz = (double)(jz - qz);
+      const double dx = fxd(jx, qx), dy = fxd(jy, qy), dz = fxd(jz, qz);
       float band;
       if (tb_rho_fx(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]], fma(dx, dx, fma(dy, dy, dz * dz)),
                     band))
@@ -829,19 +832,21 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
       pair(x3, y3, z3, s3);
     });
     if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
-    eband32[a] = (float)(1.0 / sqrt(acc));
+    ebd32[a] = (float)(1.0 / sqrt(acc) - e0);
   }
 }
 
-// Pass 2: smatb_interact (TB.F90:457-586); LDS slot = {qx, qy, qz, eband_j}
+// Pass 2: smatb_interact (TB.F90:457-586); LDS slot = {qx, qy, qz, d_j}
 // (one 16-B read per neighbour: here fewer instructions outweigh the bank
-// conflicts that make the density pass prefer separate arrays).
-template <bool SINGLE>
+// conflicts that make the density pass prefer separate arrays).  FUSED: the
+// velocity Verlet update of the atoms (no forces or energies stored, no
+// repulsion summed); otherwise forces and per-atom energies are stored.
+template <bool SINGLE, bool FUSED>
 __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
-    const double *__restrict__ rho, const float *__restrict__ eband32, double ev_ui,
+    const double *__restrict__ rho, const float *__restrict__ ebd32, double ev_ui,
     double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, TbF p1) {
   extern __shared__ int4 q4[];
   __shared__ Halo H;
@@ -855,13 +860,14 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
   unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + ns);
   unsigned char *ty = scell + ns;
   slot_map(H, scell);
+  const double fxs = t.fxs;
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
     const int h = scell[s];
     const int src = s + H.delta[h];
     const double4 p = pos[src];
-    q4[s] = make_int4(fxq(p.x - (H.org[0] - shift(t, H, 0, h))),
-                      fxq(p.y - (H.org[1] - shift(t, H, 1, h))),
-                      fxq(p.z - (H.org[2] - shift(t, H, 2, h))), __float_as_int(eband32[src]));
+    q4[s] = make_int4(fxq(p.x - (H.org[0] - shift(t, H, 0, h)), fxs),
+                      fxq(p.y - (H.org[1] - shift(t, H, 1, h)), fxs),
+                      fxq(p.z - (H.org[2] - shift(t, H, 2, h)), fxs), __float_as_int(ebd32[src]));
     if (!SINGLE) ty[s] = (unsigned char)type[src];
   }
   if (threadIdx.x == 0) {
@@ -869,25 +875,27 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
-  const double fscale = ev_ui;  // tb_force_fx returns factor / r_q: forces come out in ui/A
+  // tb_force_fx returns factor / r per unit of the pair vector, so the sums
+  // over integer pair vectors come out in eV/A
   double dmax2 = 0.0;
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
-    const int qx = fxq(pa.x - (H.org[0] - 0.0)), qy = fxq(pa.y - (H.org[1] - 0.0)),
-              qz = fxq(pa.z - (H.org[2] - 0.0));
+    const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
+              qz = fxq(pa.z - H.org[2], fxs);
     const int ti = SINGLE ? 0 : type[a];
-    const float ebi = eband32[a];
+    const float di = ebd32[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
     auto pair = [&](const int4 &q, int s) {
-      const double dx = (double)(q.x - qx), dy = (double)(q.y - qy), dz = (double)(q.z - qz);
+      const double dx = fxd(q.x, qx), dy = fxd(q.y, qy), dz = fxd(q.z, qz);
       float sf, repul;
-      if (tb_force_fx(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]],
-                      fma(dx, dx, fma(dy, dy, dz * dz)), ebi + __int_as_float(q.w), sf, repul)) {
+      if (tb_force_fx<!FUSED>(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]],
+                              fma(dx, dx, fma(dy, dy, dz * dz)), di + __int_as_float(q.w), sf,
+                              repul)) {
         const double sfd = (double)sf;
         fx = fma(-sfd, dx, fx);
         fy = fma(-sfd, dy, fy);
         fz = fma(-sfd, dz, fz);
-        ei += (double)repul;
+        if (!FUSED) ei += (double)repul;
       }
     };
     for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
@@ -898,8 +906,8 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
       pair(q2, s2);
       pair(q3, s3);
     });
-    const double Fx = fx * fscale, Fy = fy * fscale, Fz = fz * fscale;
-    if (fv.on) {
+    const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
+    if (FUSED) {
       dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
     } else {
       frc[a] = Fx;
@@ -908,7 +916,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
       epa[a] = (ei - sqrt(rho[a])) * ev_ui;
     }
   }
-  if (fv.on) block_max_atomic(dmax2, fv.maxd2);
+  if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1341,23 +1349,29 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     const size_t lds1 = gh_tile_slot_bytes(0, c->precision, c->nt) * slots;
     const size_t lds2 = gh_tile_slot_bytes(1, c->precision, c->nt) * slots;
     if (mixed) {
+      if (c->tbf_k != c->tg.fxk) GH_TRY(gh_upload_tbf(c, c->tg.fxk));
+      const double e0 = c->tb_e0;
       if (single)
         k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                         c->cell_start, c->nn, c->list16, ld,
-                                                        c->tbf, rho1, c->eband32, c->tbf0);
+                                                        c->tbf, rho1, c->eband32, e0, c->tbf0);
       else
         k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                          c->cell_start, c->nn, c->list16, ld,
-                                                         c->tbf, rho1, c->eband32, c->tbf0);
+                                                         c->tbf, rho1, c->eband32, e0, c->tbf0);
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
-      if (single)
-        k_tb_force_fx<true><<<grid, GH_TBLOCK, lds2, s>>>(
-            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0);
-      else
-        k_tb_force_fx<false><<<grid, GH_TBLOCK, lds2, s>>>(
-            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0);
+#define TB_FX(SG, FU)                                                                         \
+  k_tb_force_fx<SG, FU><<<grid, GH_TBLOCK, lds2, s>>>(                                        \
+      c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap, c->tbf, \
+      c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0)
+      if (single) {
+        if (fuse_vv) TB_FX(true, true);
+        else TB_FX(true, false);
+      } else {
+        if (fuse_vv) TB_FX(false, true);
+        else TB_FX(false, false);
+      }
+#undef TB_FX
     } else {
       if (single)
         k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(

@@ -6,8 +6,8 @@ Bars:
   * plain FP64 form: max|dF| / rms|F| <= 1e-10, per-atom and total energy
     within 1e-10 relative, trajectories against the reference's 16-digit
     energy columns within 1e-9 (chaos amplifies last-bit differences);
-  * mixed precision: rms|dF| / rms|F| <= 1e-5 and max|dF| / rms|F| <= 1e-4,
-    total energy within 1e-7 relative.
+  * mixed precision: max|dF| / rms|F| <= 1e-5 (the north-star bar on the
+    force), total energy within 1e-7 relative.
 """
 import numpy as np
 import pytest
@@ -105,8 +105,8 @@ def test_mixed_forces_error_bound(name):
     err_rms = np.sqrt(np.mean((F - Fo) ** 2)) / rms
     err_max = np.max(np.abs(F - Fo)) / rms
     print("%s mixed: rms %.3e max %.3e dE %.3e" % (name, err_rms, err_max, abs(et / epo - 1)))
-    assert err_rms <= 1e-5
-    assert err_max <= 1e-4
+    assert err_rms <= 3e-6
+    assert err_max <= 1e-5
     assert et == pytest.approx(epo, rel=1e-7)
     c.close()
 
@@ -294,7 +294,7 @@ def test_full_size_mixed_vs_fp64_and_momentum():
     err_rms = np.sqrt(np.mean((Fmx - F64) ** 2)) / rms
     err_max = np.max(np.abs(Fmx - F64)) / rms
     print("1M mixed vs fp64: rms %.3e max %.3e dE %.3e" % (err_rms, err_max, etmx / et64 - 1))
-    assert err_rms <= 1e-5 and err_max <= 1e-4
+    assert err_rms <= 3e-6 and err_max <= 1e-5
     assert etmx == pytest.approx(et64, rel=1e-7)
     # a sample of atoms against the CPU oracle
     s = O.smatb_params(cm.au_params(), 1)
