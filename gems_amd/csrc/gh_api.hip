@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "gh_internal.h"
@@ -125,28 +126,34 @@ void gh_tbf_fill(const GhTbPair &t, int k, double e0, TbF &o) {
   const double dr0 = (double)r0f - t.r0;  // r - r0 = s + dr0 with s = r - r0f
   const double bq = -2.0 * t.q / t.r0 * log2e;
   const double gam = (t.p - 2.0 * t.q) / t.r0;
+  const double ebq = exp2(bq * dr0);   // E2 = E2' ebq
   const double kap = exp(-gam * dr0);  // e^{-p(r/r0-1)} = E2 e^{-gam s} kap
   o.rce2q = t.rce * t.rce * fxs * fxs;
-  o.r0q = r0f * fxsf;
+  o.r0q2 = 2.0f * r0f * fxsf;
   o.rciq = (float)t.rci * fxsf;
-  o.rceq = (float)t.rce * fxsf;
   o.sc = uf;
-  o.bq = (float)bq * uf;
-  o.eb_a = (float)exp2(bq * dr0);
-  o.rho_a = (float)(t.xi * t.xi * exp2(bq * dr0));
+  o.rce = (float)t.rce;
+  o.bqh = (float)bq * uf * 0.5f;
+  o.rho_a = (float)(t.xi * t.xi * ebq);
   const double a = 2.0 * t.A * t.p / t.r0 * kap, B0 = t.xi * t.xi * t.q / t.r0;
-  o.a = (float)a;
-  o.B0 = (float)B0;
-  o.c0 = (float)(a - 2.0 * B0 * e0);
-  o.Ar = (float)(t.A * kap);
-  o.g = (float)(-gam) * uf;
+  o.a = (float)(a * ebq);
+  o.B0 = (float)(B0 * ebq);
+  o.c0 = (float)((a - 2.0 * B0 * e0) * ebq);
+  o.Ar = (float)(t.A * kap * ebq);
+  o.gh = (float)(-gam) * uf * 0.5f;
   o.e02 = (float)(2.0 * e0);
   o.b1f = (float)t.b1;
   o.b2f = (float)t.b2;
   o.b3f = (float)t.b3;
+  o.b1x5 = (float)(5.0 * t.b1);
+  o.b2x4 = (float)(4.0 * t.b2);
+  o.b3x3 = (float)(3.0 * t.b3);
   o.r1f = (float)t.r1;
   o.r2f = (float)t.r2;
   o.r3f = (float)t.r3;
+  o.r1m10 = (float)(-10.0 * t.r1);
+  o.r2m8 = (float)(-8.0 * t.r2);
+  o.r3m6 = (float)(-6.0 * t.r3);
 }
 
 // FP32 forms of every type pair in the unit 2^-k A (the tiled passes' fixed
@@ -370,6 +377,7 @@ static int ensure_ready(gh_ctx *c) {
       c->bd_req[0] = c->bd_req[1] = c->bd_req[2] = 0;
       finish_tiles(c);
       h.tile_overflow = 0;
+      h.halt = 0;
       GH_TRY(write_status(c, &h));
     }
   } else {
@@ -399,20 +407,66 @@ static int ensure_ready(gh_ctx *c) {
   c->cap = cap;
   h.overflow = 0;
   h.tile_overflow = 0;
+  h.halt = 0;
+  h.halt_step = 0;
   h.nupd = nupd0;
   GH_TRY(write_status(c, &h));
   GH_TRY(gh_launch_rebuild(c, true));
   GH_TRY(read_status(c, &h));
-  if (h.overflow) return gh_fail(c, GH_EOVERFLOW, "neighbour list overflow at sizing");
-  if (h.tile_overflow) return gh_fail(c, GH_EOVERFLOW, "LDS halo overflow at sizing");
+  if (h.overflow || h.tile_overflow) {
+    const bool rows = h.overflow != 0;
+    h.halt = 0;
+    h.halt_step = 0;
+    h.overflow = 0;
+    h.tile_overflow = 0;
+    GH_TRY(write_status(c, &h));
+    return gh_fail(c, GH_EOVERFLOW,
+                   rows ? "neighbour list overflow at sizing" : "LDS halo overflow at sizing");
+  }
   c->ready = true;
   c->pos_dirty = false;
   return 0;
 }
 
+// Reports a stop the device raised (GhStatus::halt) and re-arms the status:
+// the halt, the outside flag and the overflow flags are cleared on the
+// device; an overflow also makes the next call re-size the list or tiles.
+static int report_halt(gh_ctx *c, GhStatus &h) {
+  const int code = h.halt;
+  const long long at = h.halt_step;
+  const int maxnn = h.maxnn;
+  h.halt = 0;
+  h.halt_step = 0;
+  h.outside = 0;
+  h.overflow = 0;
+  h.tile_overflow = 0;
+  GH_TRY(write_status(c, &h));
+  const std::string where = " at step " + std::to_string(at);
+  switch (code) {
+    case GH_HALT_OVERFLOW:
+      c->ready = false;
+      c->cap_req = maxnn + std::max(16, maxnn / 4);
+      return gh_fail(c, GH_EOVERFLOW,
+                     "neighbour list row capacity exceeded (" + std::to_string(maxnn) + " > " +
+                         std::to_string(c->cap) + ")" + where +
+                         "; the state is that step's start; the next call re-sizes the list");
+    case GH_HALT_TILE:
+      c->ready = false;
+      return gh_fail(c, GH_EOVERFLOW,
+                     "a brick's halo outgrew the LDS slots sized at setup" + where +
+                         "; the state is that step's start; the next call re-sizes the tiles");
+    case GH_HALT_RANGE:
+      return gh_fail(c, GH_ERANGE,
+                     "an atom left the cell grid of a non-periodic box axis" + where);
+    default:
+      return gh_fail(c, GH_ESTATE, "a rebuild was due but not enqueued" + where);
+  }
+}
+
 static int check_status(gh_ctx *c) {
   GhStatus h;
   GH_TRY(read_status(c, &h));
+  if (h.halt) return report_halt(c, h);
   if (h.overflow) {
     c->ready = false;
     c->cap_req = h.maxnn + std::max(16, h.maxnn / 4);
@@ -427,8 +481,11 @@ static int check_status(gh_ctx *c) {
                    "a brick's halo outgrew the LDS slots sized at setup; results of this call "
                    "are invalid; the next call re-sizes the tiles");
   }
-  if (h.outside)
+  if (h.outside) {
+    h.outside = 0;
+    GH_TRY(write_status(c, &h));
     return gh_fail(c, GH_ERANGE, "an atom left the cell grid of a non-periodic box axis");
+  }
   return 0;
 }
 
@@ -547,16 +604,20 @@ int gh_create(gh_ctx **out, int device, int precision) {
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void **)&c->status, sizeof(GhStatus)) != hipSuccess ||
-      hipHostMalloc((void **)&c->red_h, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess) {
-    delete c;
+      hipHostMalloc((void **)&c->red_h, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->prog_h, sizeof(GhProgress), hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->prog_d, c->prog_h, 0) != hipSuccess) {
+    gh_destroy(c);
     return GH_EHIP;
   }
+  memset(c->prog_h, 0, sizeof(GhProgress));
   GhStatus h;
   memset(&h, 0, sizeof(h));
   if (hipMemcpyAsync(c->status, &h, sizeof(h), hipMemcpyHostToDevice, c->stream) !=
           hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
-    delete c;
+    gh_destroy(c);
     return GH_EHIP;
   }
   c->lcg.seed(123456789);
@@ -578,6 +639,7 @@ void gh_destroy(gh_ctx *c) {
   if (c->chain_pin) (void)hipHostFree(c->chain_pin);
   if (c->gauss_h) (void)hipHostFree(c->gauss_h);
   if (c->red_h) (void)hipHostFree(c->red_h);
+  if (c->prog_h) (void)hipHostFree(c->prog_h);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1169,6 +1231,41 @@ static int lcg_step_normals(gh_ctx *c) {
   return 0;
 }
 
+// dinamic (Programs.f90:114-167): nsteps steps of the selected integrator,
+// [epot, ekin, sum] in eV every `each` steps.
+//
+// Each step enqueues stepa (unless the previous step's force pass applied
+// it), the rebuild chain, the force passes and stepb.  On the tiled velocity
+// Verlet path the rebuild chain is enqueued only where the host expects a
+// rebuild: every step the first force pass tests the device's rebuild rule
+// itself, and if a rebuild is due without its chain it stops the run at that
+// step (GhStatus::halt; every state-changing kernel then returns at once, so
+// the state stays the one the step started from).  The last pass of each
+// step publishes the step and any stop into pinned host memory; the host
+// keeps at most kLead steps enqueued beyond the last one published and, on
+// a stop, drains the stream, restores the positions buffer of that step,
+// clears the stop and re-enqueues from it with the chain.  Rows are read
+// after a stream synchronisation, with the status, so a row never holds a
+// stopped step.  A stop for an overflow or an atom out of the grid ends the
+// run with the error and the step; the state is then the start of that
+// step's force evaluation (after its stepa) and the next gh_run completes
+// that step first.
+namespace {
+struct StepRec {
+  double4 *pos, *pos_next;
+};
+}  // namespace
+
+static int finish_mid_step(gh_ctx *c) {
+  // the stopped step's forces (the list re-sized if needed) and its stepb
+  GH_TRY(interact(c));
+  GH_TRY(check_status(c));
+  GH_TRY(c->integ == GH_INT_ERMAK ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
+  c->step += 1;
+  c->mid_step = false;
+  return 0;
+}
+
 int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
   if (!c || nsteps < 0) return GH_EINVAL;
   if (each > 0 && !energy) return gh_fail(c, GH_EINVAL, "energy buffer required when each > 0");
@@ -1179,50 +1276,163 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
     c->evpool.resize((size_t)EV_STEPS * EV_PER_STEP);
     for (auto &e : c->evpool) GH_HIP(c, hipEventCreate(&e));
   }
-  // dinamic: one force evaluation before the loop (Programs.f90:129)
-  GH_TRY(interact(c));
-  int iout = 0;
   const bool ermak = c->integ == GH_INT_ERMAK;
   const bool lcg = ermak && c->rng_mode == GH_RNG_LCG;
-  // velocity Verlet: stepb of step n and stepa of step n+1 run as one kernel
-  // unless the state after stepb is observed (energy output, end of the run)
-  // In the tiled path the force pass itself applies stepb of step n and stepa
-  // of step n+1 (fused); elsewhere one kernel does both (k_vv_ba).
-  bool pending_b = false, stepa_done = false;
-  for (int ns = 1; ns <= nsteps; ++ns) {
-    c->step += 1;
-    if (lcg) GH_TRY(lcg_step_normals(c));
-    GH_TRY(rec(c, 0));
-    if (pending_b) GH_TRY(gh_launch_vv_ba(c));
-    else if (!stepa_done) GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
-    pending_b = stepa_done = false;
-    GH_TRY(rec(c, 1));
-    GH_TRY(gh_launch_rebuild(c, false));
-    GH_TRY(rec(c, 2));
-    const bool observed = ns == nsteps || (each > 0 && ns % each == 0);
-    const bool fuse = !ermak && !observed && gh_can_fuse_vv(c);
-    GH_TRY(gh_launch_forces(c, ev_at(c, 3), fuse));
-    if (fuse) {
-      std::swap(c->pos, c->pos_next);
-      stepa_done = true;
-    }
-    GH_TRY(rec(c, 4));
-    if (ermak || observed) GH_TRY(ermak ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
-    else if (!fuse) pending_b = true;
-    GH_TRY(rec(c, 5));
-    if (c->timing) {
-      c->evused += 1;
-      if (c->evused == EV_STEPS) GH_TRY(timing_flush(c));
-    }
-    if (each > 0 && ns % each == 0) {
+  int ns = 1;     // step of this call being enqueued (1-based)
+  int iout = 0;   // energy rows written
+  if (c->mid_step) {
+    // the stopped step of the previous call is this call's first step
+    if (nsteps == 0) return 0;
+    GH_TRY(finish_mid_step(c));
+    if (each == 1) {
       double ep, ek;
       GH_TRY(gh_launch_energies(c, &ep, &ek));
-      const double f1 = ep * ui_ev, f2 = ek * ui_ev;
-      energy[3 * iout + 0] = f1;
-      energy[3 * iout + 1] = f2;
-      energy[3 * iout + 2] = f1 + f2;
-      ++iout;
+      energy[0] = ep * ui_ev;
+      energy[1] = ek * ui_ev;
+      energy[2] = (ep + ek) * ui_ev;
+      iout = 1;
     }
+    ns = 2;
+  } else {
+    // dinamic: one force evaluation before the loop (Programs.f90:129)
+    GH_TRY(interact(c));
+    GH_TRY(check_status(c));
+  }
+  // the chain only where expected: tiled velocity Verlet without per-step events
+  const bool lazy = !ermak && !c->timing && gh_can_fuse_vv(c);
+  // the tiled passes publish each step: the host then stays within kLead steps
+  const bool publishes = c->list_mode == 1;
+  const int kLead = 3;
+  const int kRing = 64;
+  std::vector<StepRec> ring(kRing);
+  const long long s0 = c->step - (ns - 1);  // step s0 + k is this call's k-th
+  volatile GhProgress *pg = c->prog_h;
+  pg->halt = 0;
+  pg->step = c->step;
+  // velocity Verlet: stepb of step n and stepa of step n+1 run as one kernel
+  // unless the state after stepb is observed (energy output, end of the run).
+  // In the tiled path the force pass itself applies stepb of step n and stepa
+  // of step n+1 (fused); elsewhere one kernel does both (k_vv_ba).
+  bool pending_b = false, stepa_done = false, chain = false;
+  // The host enqueues the rebuild chain where it expects the device's rule to
+  // fire: from the published displacement maximum (the running maximum since
+  // the last build, at the start of the latest published step) and its
+  // fastest growth per step since that build, it chains every step whose
+  // displacement could pass half the skin with half a step's growth to
+  // spare.  A rebuild it misses still stops the run at its step (resume).
+  long long seen_step = -1;
+  double seen_r = 0.0, rate = 0.0;
+  auto rebuild_expected = [&](long long step) {
+    const long long p = pg->step;
+    const double r = sqrt(__builtin_bit_cast(double, (unsigned long long)pg->maxd2));
+    if (p != seen_step) {
+      if (seen_step < 0 || p < seen_step || r < seen_r) rate = 0.0;  // a build since
+      else if (p > seen_step) rate = std::max(rate, (r - seen_r) / (double)(p - seen_step));
+      seen_step = p;
+      seen_r = r;
+    }
+    return rate > 0.0 && 2.0 * (r + rate * ((double)(step - p) + 0.5)) > c->skin;
+  };
+  // resumes at step index k (1-based in this run) after a stop there
+  auto resume = [&](GhStatus &h) -> int {
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    GH_TRY(read_status(c, &h));
+    const long long k = h.halt_step - s0;
+    if (k < 1 || k > ns) return gh_fail(c, GH_ESTATE, "stop outside the run");
+    if (k >= ns - kRing + 1) {
+      const StepRec &r = ring[k % kRing];
+      c->pos = r.pos;
+      c->pos_next = r.pos_next;
+    } else if (publishes) {
+      return gh_fail(c, GH_ESTATE, "stop older than the step record");
+    }  // else no step swaps the positions buffers
+    c->step = h.halt_step - 1;
+    iout = each > 0 ? (int)((k - 1) / each) : 0;
+    if (h.halt != GH_HALT_REBUILD) {
+      c->mid_step = true;
+      return report_halt(c, h);
+    }
+    h.halt = 0;
+    h.halt_step = 0;
+    GH_TRY(write_status(c, &h));
+    pg->halt = 0;
+    pg->step = c->step;
+    ns = (int)k;
+    stepa_done = true;  // the stop came after the step's stepa
+    pending_b = false;
+    chain = true;
+    return 0;
+  };
+  for (;;) {
+    while (ns <= nsteps) {
+      if (publishes) {
+        // at most kLead steps beyond the last one published
+        const long long need = s0 + ns - 1 - kLead;
+        for (int spin = 0; pg->step < need && !pg->halt; ++spin) {
+          std::this_thread::yield();
+          if ((spin & 1023) == 1023) {  // an idle stream has published every step it ran
+            const hipError_t q = hipStreamQuery(c->stream);
+            if (q == hipSuccess && pg->step < need && !pg->halt)
+              return gh_fail(c, GH_ESTATE, "a step was not published");
+            if (q != hipSuccess && q != hipErrorNotReady) GH_HIP(c, q);
+          }
+        }
+        if (pg->halt) {
+          GhStatus h;
+          GH_TRY(resume(h));
+          continue;
+        }
+      }
+      if (lazy && !chain) chain = rebuild_expected(s0 + ns);
+      c->step = s0 + ns;
+      ring[ns % kRing] = StepRec{c->pos, c->pos_next};
+      if (lcg) GH_TRY(lcg_step_normals(c));
+      GH_TRY(rec(c, 0));
+      if (pending_b) GH_TRY(gh_launch_vv_ba(c));
+      else if (!stepa_done) GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
+      pending_b = stepa_done = false;
+      GH_TRY(rec(c, 1));
+      if (!lazy || chain) GH_TRY(gh_launch_rebuild(c, false));
+      chain = false;
+      GH_TRY(rec(c, 2));
+      const bool observed = ns == nsteps || (each > 0 && ns % each == 0);
+      const bool fuse = !ermak && !observed && gh_can_fuse_vv(c);
+      GH_TRY(gh_launch_forces(c, ev_at(c, 3), fuse));
+      if (fuse) {
+        std::swap(c->pos, c->pos_next);
+        stepa_done = true;
+      }
+      GH_TRY(rec(c, 4));
+      if (ermak || observed) GH_TRY(ermak ? gh_launch_ermak_b(c) : gh_launch_vv_b(c));
+      else if (!fuse) pending_b = true;
+      GH_TRY(rec(c, 5));
+      if (c->timing) {
+        c->evused += 1;
+        if (c->evused == EV_STEPS) GH_TRY(timing_flush(c));
+      }
+      if (each > 0 && ns % each == 0) {
+        double ep, ek;
+        GH_TRY(gh_launch_energies(c, &ep, &ek));  // synchronises the stream
+        GhStatus h;
+        GH_TRY(read_status(c, &h));
+        if (h.halt) {
+          GH_TRY(resume(h));
+          continue;
+        }
+        const double f1 = ep * ui_ev, f2 = ek * ui_ev;
+        energy[3 * iout + 0] = f1;
+        energy[3 * iout + 1] = f2;
+        energy[3 * iout + 2] = f1 + f2;
+        ++iout;
+      }
+      ++ns;
+    }
+    // the last steps: no stop may remain unseen
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    GhStatus h;
+    GH_TRY(read_status(c, &h));
+    if (!h.halt) break;
+    GH_TRY(resume(h));
   }
   GH_TRY(timing_flush(c));
   GH_TRY(check_status(c));

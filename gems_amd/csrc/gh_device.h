@@ -18,14 +18,40 @@ __device__ __forceinline__ bool rebuild_due(const GhStatus *st, double skin) {
 
 // Books the decision in st->rebuild once every kernel that tests
 // rebuild_due has run: counts the rebuild, clears the host request and the
-// row maximum, and restarts the displacement maximum for the next step.
+// row maximum, and restarts the displacement maximum.  Between rebuilds the
+// maximum runs over all steps since the build (an upper bound of the
+// current displacement, so a test on it never rebuilds late).
 __device__ __forceinline__ void rebuild_book(GhStatus *st) {
   if (st->rebuild) {
     st->nupd += 1;
     st->force_rebuild = 0;
     st->maxnn = 0;
+    st->maxdisp2_bits = 0ull;
   }
-  st->maxdisp2_bits = 0ull;
+}
+
+// Stops the run at `step` (the first raiser wins and records the step):
+// every kernel that changes the atoms' state returns at once while
+// st->halt is set, so the state stays the one the step started from.
+__device__ __forceinline__ void raise_halt(GhStatus *st, int code, long long step) {
+  if (atomicCAS(&st->halt, 0, code) == 0) st->halt_step = step;
+}
+// A kernel reads the stop as it stood when the kernel started (a plain,
+// wave-uniform load: every earlier kernel's writes are visible then).
+__device__ __forceinline__ int halted(const GhStatus *st) { return st->halt; }
+
+// The step's first pass publishes, from its workgroup 0 as it starts, the
+// step and the stop as they stand into pinned host memory (three stores;
+// the host throttles on the step and reads the stop without a stream
+// operation; a stop raised later in the step is seen at the next step's
+// start, and always by the host's status read after a synchronisation).
+__device__ __forceinline__ void step_publish(GhProgress *prog, long long step, int halt,
+                                             long long halt_step, unsigned long long maxd2) {
+  if (!prog) return;
+  __hip_atomic_store(&prog->maxd2, maxd2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&prog->halt, halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&prog->halt_step, halt_step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&prog->step, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
@@ -201,29 +227,30 @@ __device__ __forceinline__ void for_each_quad(const unsigned short *__restrict__
 
 // SMA-TB FP32 pair forms (the TbF terms; r2 in the TbF unit, FP64).  r is
 // the FP32 r2 times its hardware reciprocal square root y; for the inner
-// branch s = r - r0f is recovered to FP32 precision of s itself from the
-// residual r2 - r_h^2, exact in FP64 (r_h^2 of a float is exact), so the
-// rounding of r (a few 1e-7 A) never enters the exponentials.
-__device__ __forceinline__ float tb_s(const TbF &P, double r2, float y, float rh) {
+// branch s2 = 2 (r - r0f) is recovered to FP32 precision of s itself from
+// the residual r2 - r_h^2, exact in FP64 (r_h^2 of a float is exact), so the
+// rounding of r (a few 1e-7 A) never enters the exponentials; 2 r_h - 2 r0f
+// is exact for r_h within a factor two of r0f.
+__device__ __forceinline__ float tb_s2(const TbF &P, double r2, float y, float rh) {
   const float e = (float)fma(-(double)rh, (double)rh, r2);
-  return fmaf(e, 0.5f * y, rh - P.r0q);
+  return fmaf(e, y, fmaf(rh, 2.0f, -P.r0q2));
 }
 // e^x - 1 to about one FP32 ulp of the result: x + x^2 (1/2 + x Q(x)) for
 // |x| < 1/2 (Q a least-squares fit of (e^x - 1 - x - x^2/2)/x^3 there), else
-// from the hardware exponential.
+// from the hardware exponential, evaluated only when a lane of the wave
+// needs it (rows ordered by distance class keep the nearest shell inside).
 __device__ __forceinline__ float tb_em1(float x) {
-  if (fabsf(x) < 0.5f) {
-    float q = fmaf(1.9983829406555742e-04f, x, 1.3986654812470078e-03f);
-    q = fmaf(q, x, 8.3331037312746048e-03f);
-    q = fmaf(q, x, 4.1665747761726379e-02f);
-    q = fmaf(q, x, 1.6666667163372040e-01f);
-    return fmaf(x * x, fmaf(q, x, 0.5f), x);
+  float q = fmaf(1.9983829406555742e-04f, x, 1.3986654812470078e-03f);
+  q = fmaf(q, x, 8.3331037312746048e-03f);
+  q = fmaf(q, x, 4.1665747761726379e-02f);
+  q = fmaf(q, x, 1.6666667163372040e-01f);
+  float em1 = fmaf(x * x, fmaf(q, x, 0.5f), x);
+  const bool far = !(fabsf(x) < 0.5f);
+  if (__builtin_expect(__any(far), 0)) {
+    const float t = __builtin_amdgcn_exp2f(x * 1.4426950408889634f) - 1.0f;
+    if (far) em1 = t;
   }
-  return __builtin_amdgcn_exp2f(x * 1.4426950408889634f) - 1.0f;
-}
-__device__ __forceinline__ float tb_tail_band(const TbF &P, float d) {
-  const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-  return pp * pp;
+  return em1;
 }
 // Density term of a pair within rce (smatb_preinteraction, TB.F90:405-423).
 __device__ __forceinline__ bool tb_rho_fx(const TbF &P, double r2, float &band) {
@@ -232,9 +259,11 @@ __device__ __forceinline__ bool tb_rho_fx(const TbF &P, double r2, float &band) 
   const float y = __builtin_amdgcn_rsqf(r2f);
   const float rh = r2f * y;
   if (rh < P.rciq) {
-    band = P.rho_a * __builtin_amdgcn_exp2f(P.bq * tb_s(P, r2, y, rh));
+    band = P.rho_a * __builtin_amdgcn_exp2f(P.bqh * tb_s2(P, r2, y, rh));
   } else {
-    band = tb_tail_band(P, (rh - P.rceq) * P.sc);
+    const float d = fmaf(rh, P.sc, -P.rce);
+    const float pp = d * d * d * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
+    band = pp * pp;
   }
   return true;
 }
@@ -250,21 +279,21 @@ __device__ __forceinline__ bool tb_force_fx(const TbF &P, double r2, float dl, f
   const float rh = r2f * y;
   float factor;
   if (rh <= P.rciq) {
-    const float s = tb_s(P, r2, y, rh);
-    const float e2 = P.eb_a * __builtin_amdgcn_exp2f(P.bq * s);
-    const float em1 = tb_em1(P.g * s);
+    const float s2 = tb_s2(P, r2, y, rh);
+    const float e2 = __builtin_amdgcn_exp2f(P.bqh * s2);
+    const float em1 = tb_em1(P.gh * s2);
     factor = e2 * fmaf(P.a, em1, fmaf(-P.B0, dl, P.c0));
     if (ENERGY) {
       const float ar = P.Ar * e2;
       repul = fmaf(ar, em1, ar);
     }
   } else {
-    const float d = (rh - P.rceq) * P.sc;
+    const float d = fmaf(rh, P.sc, -P.rce);
     const float d2 = d * d, d3 = d2 * d;
     const float pp = d3 * fmaf(d, fmaf(d, P.b1f, P.b2f), P.b3f);
-    const float dp = d2 * fmaf(d, fmaf(d, 5.0f * P.b1f, 4.0f * P.b2f), 3.0f * P.b3f);
-    const float dr_ = d2 * fmaf(d, fmaf(d, 5.0f * P.r1f, 4.0f * P.r2f), 3.0f * P.r3f);
-    factor = fmaf(pp * dp, P.e02 + dl, -2.0f * dr_);
+    const float dp = d2 * fmaf(d, fmaf(d, P.b1x5, P.b2x4), P.b3x3);
+    const float drm = d2 * fmaf(d, fmaf(d, P.r1m10, P.r2m8), P.r3m6);  // -2 dR/dd
+    factor = fmaf(pp * dp, P.e02 + dl, drm);
     if (ENERGY) repul = d3 * fmaf(d, fmaf(d, P.r1f, P.r2f), P.r3f);
   }
   sf = factor * y;

@@ -51,11 +51,12 @@ void gh_tb_coeffs(GhTbPair &c);
 // a length unit u = 2^-k Angstrom (k = 0: Angstrom; the fixed-point tiles
 // use the unit of their integer positions).  With s = r - r0f (r0f the float
 // nearest r0; s is formed from r^2 in FP64 so that it carries no rounding of
-// r itself) and E2 = e^{-2q(r/r0-1)} = eb_a 2^{bq s}:
-//   density   xi^2 E2                               = rho_a 2^{bq s}
+// r itself; the kernels carry s2 = 2 s) and E2' = 2^{bq s} (E2 = e^{-2q(r/r0
+// - 1)} = E2' 2^{bq (r0f - r0)}, the last factor folded into the amplitudes):
+//   density   xi^2 E2                               = rho_a E2'
 //   force     2A(p/r0) e^{-p(r/r0-1)} - xi^2 (q/r0) E2 (eb_i + eb_j)
-//           = E2 [a em1(g s) + c0 - B0 (d_i + d_j)]
-//   energy    A e^{-p(r/r0-1)} = Ar E2 (1 + em1(g s))
+//           = E2' [a em1(g s) + c0 - B0 (d_i + d_j)]
+//   energy    A e^{-p(r/r0-1)} = Ar E2' (1 + em1(g s))
 // where em1(x) = e^x - 1, g s = -(p - 2q)(r - r0f)/r0, eb = e0 + d with e0 a
 // context constant near every atom's 1/sqrt(rho) (d stays small, so its FP32
 // rounding is far below eb's), a and Ar carry e^{-(p-2q)(r0f-r0)/r0} and c0 =
@@ -64,15 +65,15 @@ void gh_tb_coeffs(GhTbPair &c);
 // cancel before any FP32 rounding.  Tail (rci < r <= rce): the quintic
 // polynomials of TB.F90:134-149 in Angstrom, d = (r - rce).
 struct TbF {
-  double rce2q;           // rce^2 in u^2
-  float r0q, rciq, rceq;  // r0f, rci, rce in u
-  float sc;               // u in Angstrom (2^-k)
-  float rho_a, bq;        // density amplitude; -2q/r0 log2(e) per u
-  float eb_a;             // E2 = eb_a 2^{bq s}
-  float a, c0, B0, Ar;    // force bracket and repulsion amplitude (eV, eV/A)
-  float g;                // -(p - 2q)/r0 per u
-  float e02;              // 2 e0 (tail terms)
-  float b1f, b2f, b3f, r1f, r2f, r3f;
+  double rce2q;            // rce^2 in u^2
+  float r0q2, rciq;        // 2 r0f and rci in u
+  float sc, rce;           // u and rce in Angstrom (tail: d = r_u sc - rce)
+  float rho_a, bqh;        // density xi^2 E2 = rho_a 2^{bqh s2}, s2 = 2 s in u
+  float a, c0, B0, Ar;     // force bracket and repulsion amplitude (E2 = 2^{bqh s2})
+  float gh;                // -(p - 2q)/(2 r0) per u: g s = gh s2
+  float e02;               // 2 e0 (tail terms)
+  float b1f, b2f, b3f, b1x5, b2x4, b3x3;          // tail density polynomial and 5b1, 4b2, 3b3
+  float r1f, r2f, r3f, r1m10, r2m8, r3m6;          // tail repulsion and -10r1, -8r2, -6r3
 };
 
 // Kernel-side parameter block (one per type pair): the FP64 form (the
@@ -144,6 +145,27 @@ struct GhStatus {                     // device-resident counters
   int max_halo;                       // largest halo (atoms) at the last build
   int max_brick;                      // largest brick (atoms) at the last build
   int ticket;                         // binning workgroups done (last one scans)
+  int halt;                           // GH_HALT_*: every state-changing kernel returns
+  long long halt_step;                // step at which halt was raised
+};
+
+// Why the device stopped a run: the step's rebuild was due but not enqueued
+// (the host re-runs the step with the rebuild chain), or the step cannot be
+// computed (a row or a halo outgrew its capacity, an atom left the grid).
+#define GH_HALT_REBUILD 1
+#define GH_HALT_OVERFLOW 2
+#define GH_HALT_TILE 3
+#define GH_HALT_RANGE 4
+
+// The step the device has started and the stop as it stood then, published
+// by the step's first force pass into pinned host memory (the host throttles
+// on `step` and reads halt without a stream operation).
+struct GhProgress {
+  long long step;
+  long long halt_step;
+  unsigned long long maxd2;  // displacement maximum (bits of |dx|^2) at the step's start
+  int halt;
+  int pad;
 };
 
 struct gh_ctx {
@@ -180,6 +202,9 @@ struct gh_ctx {
   GhLcg lcg;
   bool first_force = true;   // forces not yet valid (interact not called)
   long long step = 0;
+  // a run stopped at step `step` + 1 after its stepa: the forces and stepb of
+  // that step are still due (the next gh_run applies them first)
+  bool mid_step = false;
 
   // neighbour grid
   int list_mode = 1;          // 0 = O(N^2), 1 = cells
@@ -230,6 +255,8 @@ struct gh_ctx {
   int *mfeel = nullptr, *mfelt = nullptr;
   bool masked = false;
   GhStatus *status = nullptr;
+  GhProgress *prog_h = nullptr;               // pinned: the latest step's outcome
+  GhProgress *prog_d = nullptr;               // its device address
   double *red = nullptr;                      // reduction scratch
   int nred = 0;
   double *red_h = nullptr;                    // pinned result

@@ -51,12 +51,14 @@ __global__ void k_fill_int(int *__restrict__ p, int v, int n) {
 // the skin.  Here the displacement is the true (minimum-image) one, and the
 // test uses 2*max, which bounds d1+d2 from above.  Integrators fold each
 // workgroup's max |x - x_old|^2 into GhStatus::maxdisp2_bits with one
-// atomicMax (non-negative doubles order as their bit patterns); the first
-// kernel of the next rebuild sequence reads it (rebuild_due, rebuild_book).
+// atomicMax (non-negative doubles order as their bit patterns); it runs over
+// the steps since the last build and the first kernel of a rebuild sequence
+// reads it (rebuild_due, rebuild_book).  State-changing kernels return at
+// once while GhStatus::halt is set (raise_halt).
 // ---------------------------------------------------------------------------
 // Decision of the O(N^2) list mode (one thread).
 __global__ void k_decide(GhStatus *st, double skin) {
-  st->rebuild = rebuild_due(st, skin);
+  st->rebuild = !st->halt && rebuild_due(st, skin);
   rebuild_book(st);
 }
 
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
     GhStatus *st, const double4 *__restrict__ pos, int n, GridArgs g, double rl2,
     int cap, int ld, int count_only, int *__restrict__ nn, unsigned *__restrict__ list,
     double *__restrict__ xold, int ncap, const int *__restrict__ orig,
-    const int *__restrict__ mfeel, const int *__restrict__ mfelt) {
+    const int *__restrict__ mfeel, const int *__restrict__ mfelt, long long step) {
   if (!st->rebuild) return;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   int m = 0;
@@ -138,7 +140,10 @@ __global__ __launch_bounds__(GH_BLOCK) void k_build_n2(
   int wm = wave_imax(m);
   if ((threadIdx.x & 63) == 0 && wm > 0) {
     atomicMax(&st->maxnn, wm);
-    if (wm > cap) st->overflow = 1;
+    if (wm > cap) {
+      st->overflow = 1;
+      if (cap > 0) raise_halt(st, GH_HALT_OVERFLOW, step);  // cap 0: the sizing count
+    }
   }
 }
 
@@ -536,7 +541,8 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     int n, int ncap, double dt, MassArgs ma,
     const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
     double *__restrict__ acc, const double *__restrict__ frc, const double *__restrict__ xold,
-    unsigned long long *__restrict__ maxd2) {
+    GhStatus *st) {
+  if (halted(st)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -559,7 +565,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, maxd2);
+  disp_reduce(d2, &st->maxdisp2_bits);
 }
 
 // velocity_verlet_b of one step fused with velocity_verlet_a of the next
@@ -569,8 +575,8 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_a(
 __global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
     int n, int ncap, double dt, MassArgs ma,
     const int *__restrict__ type, double4 *__restrict__ pos, double *__restrict__ vel,
-    const double *__restrict__ frc, const double *__restrict__ xold,
-    unsigned long long *__restrict__ maxd2) {
+    const double *__restrict__ frc, const double *__restrict__ xold, GhStatus *st) {
+  if (halted(st)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -592,7 +598,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_ba(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, maxd2);
+  disp_reduce(d2, &st->maxdisp2_bits);
 }
 
 // velocity_verlet_b (Integration.F90:1171-1186).
@@ -600,7 +606,9 @@ __global__ __launch_bounds__(GH_BLOCK) void k_vv_b(int n, int ncap, double dt, M
                                                    const int *__restrict__ type,
                                                    double *__restrict__ vel,
                                                    double *__restrict__ acc,
-                                                   const double *__restrict__ frc) {
+                                                   const double *__restrict__ frc,
+                                                   const GhStatus *st) {
+  if (halted(st)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double acel_dv = dt * 0.5;
@@ -637,7 +645,8 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     const int *__restrict__ type,
     const int *__restrict__ orig, const double *__restrict__ gauss, double4 *__restrict__ pos,
     const double *__restrict__ vel, const double *__restrict__ acc,
-    const double *__restrict__ xold, unsigned long long *__restrict__ maxd2) {
+    const double *__restrict__ xold, GhStatus *st) {
+  if (halted(st)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -658,7 +667,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
     p.z = x[2];
     pos[i] = p;
   }
-  disp_reduce(d2, maxd2);
+  disp_reduce(d2, &st->maxdisp2_bits);
 }
 
 // ermak_b (Integration.F90:1013-1031); vel_v is re-derived from the same
@@ -666,7 +675,8 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_a(
 __global__ __launch_bounds__(GH_BLOCK) void k_ermak_b(
     int n, int ncap, ErmakArgs k, MassArgs ma, const int *__restrict__ type,
     const int *__restrict__ orig, const double *__restrict__ gauss, double *__restrict__ vel,
-    double *__restrict__ acc, const double *__restrict__ frc) {
+    double *__restrict__ acc, const double *__restrict__ frc, const GhStatus *st) {
+  if (halted(st)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double g[6];
@@ -888,7 +898,7 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
                                      (unsigned *)c->list, c->xold, c->ncap, c->orig,
                                      c->masked ? c->mfeel : nullptr,
-                                     c->masked ? c->mfelt : nullptr);
+                                     c->masked ? c->mfelt : nullptr, c->step);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -990,7 +1000,7 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 int gh_launch_vv_a(gh_ctx *c) {
   k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                     make_mass(c), c->type, c->pos, c->vel,
-                                                    c->acc, c->frc, c->xold, gh_maxd2(c));
+                                                    c->acc, c->frc, c->xold, c->status);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -998,14 +1008,14 @@ int gh_launch_vv_a(gh_ctx *c) {
 int gh_launch_vv_ba(gh_ctx *c) {
   k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                      make_mass(c), c->type, c->pos, c->vel,
-                                                     c->frc, c->xold, gh_maxd2(c));
+                                                     c->frc, c->xold, c->status);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
 
 int gh_launch_vv_b(gh_ctx *c) {
   k_vv_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c), c->type,
-                                                    c->vel, c->acc, c->frc);
+                                                    c->vel, c->acc, c->frc, c->status);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1013,7 +1023,7 @@ int gh_launch_vv_b(gh_ctx *c) {
 int gh_launch_ermak_a(gh_ctx *c) {
   k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
       c->n, c->ncap, make_ermak(c), make_mass(c), c->type, c->orig,
-      c->gauss, c->pos, c->vel, c->acc, c->xold, gh_maxd2(c));
+      c->gauss, c->pos, c->vel, c->acc, c->xold, c->status);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1021,7 +1031,8 @@ int gh_launch_ermak_a(gh_ctx *c) {
 int gh_launch_ermak_b(gh_ctx *c) {
   k_ermak_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, make_ermak(c),
                                                        make_mass(c), c->type, c->orig,
-                                                       c->gauss, c->vel, c->acc, c->frc);
+                                                       c->gauss, c->vel, c->acc, c->frc,
+                                                       c->status);
   GH_HIP(c, hipGetLastError());
   return 0;
 }

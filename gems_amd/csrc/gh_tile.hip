@@ -52,9 +52,10 @@ __device__ __forceinline__ double shift(const TileGeom &t, const Halo &H, int d,
 
 // Halo of brick b: atom ranges, image shifts and LDS slot bases of the
 // (bd+2)^3 halo cells.  Returns false, uniformly over the workgroup, if the
-// brick holds no atoms or (check) its halo exceeds slot_cap.
+// brick holds no atoms or (check) its halo exceeds slot_cap, which stops the
+// run at `step`.
 __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cell_start, Halo &H,
-                           GhStatus *st, bool check) {
+                           GhStatus *st, bool check, long long step = -1) {
   const int bx = b % t.nbr[0], by = (b / t.nbr[0]) % t.nbr[1], bz = b / (t.nbr[0] * t.nbr[1]);
   const int hd0 = t.bd[0] + 2, hd1 = t.bd[1] + 2, hd2 = t.bd[2] + 2;
   const int nh = hd0 * hd1 * hd2;
@@ -131,7 +132,10 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
   __syncthreads();
   if (H.a1 <= H.a0) return false;
   if (check && H.total > t.slot_cap) {
-    if (threadIdx.x == 0) st->tile_overflow = 1;
+    if (threadIdx.x == 0) {
+      st->tile_overflow = 1;
+      raise_halt(st, GH_HALT_TILE, step);
+    }
     return false;
   }
   return true;
@@ -466,7 +470,8 @@ __device__ __forceinline__ int tile_coord(double x, const TileGeom &t, int d, in
 // 444-470), copies the atom's state to the _t arrays (the gather reads it
 // there and writes the new order in place), computes the atom's key and
 // counts atoms per key.
-__device__ void cell_bin(const CellArgs &R, const double4 *__restrict__ pos, int i0, int stride) {
+__device__ void cell_bin(const CellArgs &R, const double4 *__restrict__ pos, int i0, int stride,
+                         long long step) {
   const TileGeom &t = R.t;
   const int ncap = R.ncap;
   for (int i = i0; i < R.n; i += stride) {
@@ -493,7 +498,10 @@ __device__ void cell_bin(const CellArgs &R, const double4 *__restrict__ pos, int
     const int cx = tile_coord(x[0], t, 0, &out);
     const int cy = tile_coord(x[1], t, 1, &out);
     const int cz = tile_coord(x[2], t, 2, &out);
-    if (out) R.st->outside = 1;
+    if (out) {
+      R.st->outside = 1;
+      raise_halt(R.st, GH_HALT_RANGE, step);
+    }
     const int key = grid_key(t, cx, cy, cz);
     R.cellid[i] = key;
     atomicAdd(&R.count[key], 1);
@@ -633,12 +641,13 @@ struct ChainArgs {
 #define GH_BINBLOCK 1024  // binning workgroup: its last workgroup scans the counts
 __global__ __launch_bounds__(GH_BINBLOCK) void k_cell_bin(GhStatus *st,
                                                         const ChainArgs *__restrict__ C,
-                                                        double4 *pos, double skin) {
-  const bool due = rebuild_due(st, skin);
+                                                        double4 *pos, double skin,
+                                                        long long step) {
+  const bool due = !st->halt && rebuild_due(st, skin);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rebuild = due ? 1 : 0;
   if (!due) return;
   const CellArgs &R = C->R;
-  cell_bin(R, pos, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  cell_bin(R, pos, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, step);
   __shared__ int last;
   __syncthreads();  // this workgroup's counts are issued
   if (threadIdx.x == 0) {
@@ -683,8 +692,8 @@ __device__ __forceinline__ void stage_local(const Halo &H, const TileGeom &t,
 // is free again on return).
 __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
                             const int *__restrict__ cellof, const int *__restrict__ cell_start,
-                            const BuildArgs &A, float *dynf, Halo &H, int b) {
-  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+                            const BuildArgs &A, float *dynf, Halo &H, int b, long long step) {
+  if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   int mmax = 0;
   const int pad = t.slot_cap;  // sentinel slot of the force passes
   const int cap = A.cap, lds_slots = A.lds_slots;
@@ -719,21 +728,25 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
   mmax = wave_imax(mmax);
   if ((threadIdx.x & 63) == 0 && mmax > 0) {
     atomicMax(&st->maxnn, mmax);
-    if (mmax > cap) st->overflow = 1;
+    if (mmax > cap) {
+      st->overflow = 1;
+      if (cap > 0) raise_halt(st, GH_HALT_OVERFLOW, step);  // cap 0: the sizing count
+    }
   }
   __syncthreads();
 }
 
 __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(GhStatus *st,
                                                            const ChainArgs *__restrict__ C,
-                                                           const double4 *__restrict__ pos) {
+                                                           const double4 *__restrict__ pos,
+                                                           long long step) {
   if (!st->rebuild) return;
   extern __shared__ float dynf[];
   __shared__ Halo H;
   const CellArgs &R = C->R;
   const int b = xcd_block();
   if (b >= R.t.nbrick) return;
-  build_brick(st, R.t, pos, R.cellof, R.start, C->A, dynf, H, b);
+  build_brick(st, R.t, pos, R.cellof, R.start, C->A, dynf, H, b, step);
 }
 
 // Largest halo and brick (atoms) over all bricks; sizes slot_cap.
@@ -766,19 +779,41 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
     }
 }
 
+// Per-step arguments of the force passes.  The first pass of a step runs
+// only if the run is not halted and the step's rebuild is not due (when the
+// host has not enqueued the rebuild chain for the step, a due rebuild stops
+// the run there; the host re-runs the step with the chain); its workgroup 0
+// publishes the step (step_publish).
+struct StepArgs {
+  long long step;
+  double skin;
+  GhProgress *prog;  // null: nothing published
+};
+__device__ __forceinline__ bool step_gate(GhStatus *st, const StepArgs &sa) {
+  const int h = halted(st);
+  const bool stop = h || rebuild_due(st, sa.skin);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (stop && !h) raise_halt(st, GH_HALT_REBUILD, sa.step);
+    step_publish(sa.prog, sa.step, h ? h : (stop ? GH_HALT_REBUILD : 0),
+                 h ? st->halt_step : sa.step, st->maxdisp2_bits);
+  }
+  return stop;
+}
+
 // ---------------------------------------------------------------------------
 // SMA-TB, mixed precision (fixed-point LDS, 2^t.fxk steps per Angstrom about
 // the brick centre; the FP32 pair forms of TbF in that unit).  Pass 1:
-// smatb_preinteraction (TB.F90:367-455); LDS per slot: qx, qy, qz in three
-// int arrays (one 16-B array reads with more bank conflicts), the slot map
-// and the type.  Stores rho (when asked) and d = 1/sqrt(rho) - e0 in FP32.
+// smatb_preinteraction (TB.F90:367-455); LDS per slot: {qx, qy, qz, type} as
+// one 16-B word (one address per neighbour), then the slot map.  Stores rho
+// (when asked) and d = 1/sqrt(rho) - e0 in FP32.
 // ---------------------------------------------------------------------------
 template <bool SINGLE>
 __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbF *__restrict__ tbf,
-    double *__restrict__ rho, float *__restrict__ ebd32, double e0, TbF p1) {
+    double *__restrict__ rho, float *__restrict__ ebd32, double e0, TbF p1, StepArgs sa) {
+  if (step_gate(st, sa)) return;
   extern __shared__ int qa[];
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
@@ -786,7 +821,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) sp[k] = tbf[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
-  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
   int *qxa = qa, *qya = qa + ns, *qza = qa + 2 * ns;
   unsigned char *scell = reinterpret_cast<unsigned char *>(qa + 3 * ns);
@@ -842,12 +877,13 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
 // velocity Verlet update of the atoms (no forces or energies stored, no
 // repulsion summed); otherwise forces and per-atom energies are stored.
 template <bool SINGLE, bool FUSED>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
-    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
+__device__ __forceinline__ void tb_force_fx_body(
+    GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
     const double *__restrict__ rho, const float *__restrict__ ebd32, double ev_ui,
-    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, TbF p1) {
+    double *__restrict__ frc, double *__restrict__ epa, const FuseVV &fv, const TbF &p1,
+    long long step) {
   extern __shared__ int4 q4[];
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
@@ -855,7 +891,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) sp[k] = tbf[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
-  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
   unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + ns);
   unsigned char *ty = scell + ns;
@@ -918,6 +954,17 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
   }
   if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
+template <bool SINGLE, bool FUSED>
+__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
+    const int *__restrict__ cell_start, const int *__restrict__ nn,
+    const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
+    const double *__restrict__ rho, const float *__restrict__ ebd32, double ev_ui,
+    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, TbF p1, StepArgs sa) {
+  if (!halted(st))
+    tb_force_fx_body<SINGLE, FUSED>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbf, rho,
+                                    ebd32, ev_ui, frc, epa, fv, p1, sa.step);
+}
 
 // ---------------------------------------------------------------------------
 // SMA-TB, plain FP64 (the reference's arithmetic; LDS holds FP64 positions
@@ -928,7 +975,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbDev *__restrict__ tbd, int nt,
-    double *__restrict__ rho, double *__restrict__ eband, float *__restrict__ eband32) {
+    double *__restrict__ rho, double *__restrict__ eband, float *__restrict__ eband32,
+    StepArgs sa) {
+  if (step_gate(st, sa)) return;
   extern __shared__ double dyn[];
   __shared__ Halo H;
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
@@ -939,7 +988,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
   }
   const int b = xcd_block();
   if (b >= t.nbrick) return;
-  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
   int *ty = reinterpret_cast<int *>(dyn + 3 * cap);
@@ -977,12 +1026,12 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
 }
 
 template <bool SINGLE>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
-    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
+__device__ __forceinline__ void tb_force64_body(
+    GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbDev *__restrict__ tbd,
     int nt, const double *__restrict__ rho, const double *__restrict__ eband, double ev_ui,
-    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv) {
+    double *__restrict__ frc, double *__restrict__ epa, const FuseVV &fv, long long step) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
   __shared__ TbDev sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];
@@ -993,7 +1042,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
   }
   const int b = xcd_block();
   if (b >= t.nbrick) return;
-  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap, *eb = dyn + 3 * cap;
   int *ty = reinterpret_cast<int *>(dyn + 4 * cap);
@@ -1047,17 +1096,29 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
   }
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
+template <bool SINGLE>
+__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
+    const int *__restrict__ cell_start, const int *__restrict__ nn,
+    const unsigned short *__restrict__ list16, int ld, int ncap, const TbDev *__restrict__ tbd,
+    int nt, const double *__restrict__ rho, const double *__restrict__ eband, double ev_ui,
+    double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, StepArgs sa) {
+  if (!halted(st))
+    tb_force64_body<SINGLE>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbd, nt, rho,
+                            eband, ev_ui, frc, epa, fv, sa.step);
+}
 
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
 // POT: GH_POT_LJ (eps, sigma as arguments) or a tabulated form (plj, slj,
 // wca: per type pair, types staged in LDS).
 template <bool MIXED, int POT>
-__global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
-    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
-    const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
-    double eps, double sigma, double rc2, double ev_ui, double *__restrict__ frc,
-    double *__restrict__ epa, const int *__restrict__ type, FuseVV fv,
-    const PljDev *__restrict__ pljd) {
+__device__ __forceinline__ void lj_tiled_body(
+    GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
+    const int *__restrict__ cell_start, const int *__restrict__ nn,
+    const unsigned short *__restrict__ list16, int ld, int ncap, double eps, double sigma,
+    double rc2, double ev_ui, double *__restrict__ frc, double *__restrict__ epa,
+    const int *__restrict__ type, const FuseVV &fv, const PljDev *__restrict__ pljd,
+    long long step) {
   extern __shared__ double dyn[];
   __shared__ Halo H;
   constexpr bool TAB = POT != GH_POT_LJ;
@@ -1066,7 +1127,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) pp[k] = pljd[k];
   const int b = xcd_block();
   if (b >= t.nbrick) return;
-  if (!halo_setup(t, b, cell_start, H, st, true)) return;
+  if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
   double *px = dyn, *py = dyn + cap, *pz = dyn + 2 * cap;
   unsigned char *scell = reinterpret_cast<unsigned char *>(dyn + 3 * cap);
@@ -1150,6 +1211,18 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
     }
   }
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
+}
+// One pass per step: gated (and publishing) as a first pass.
+template <bool MIXED, int POT>
+__global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
+    const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
+    double eps, double sigma, double rc2, double ev_ui, double *__restrict__ frc,
+    double *__restrict__ epa, const int *__restrict__ type, FuseVV fv,
+    const PljDev *__restrict__ pljd, StepArgs sa) {
+  if (!step_gate(st, sa))
+    lj_tiled_body<MIXED, POT>(st, t, pos, cell_start, nn, list16, ld, ncap, eps, sigma, rc2,
+                              ev_ui, frc, epa, type, fv, pljd, sa.step);
 }
 
 }  // namespace
@@ -1291,10 +1364,10 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   // grid-stride launches of at most 2048 workgroups: cheap when no rebuild is due
   const int ga = std::min((R.n + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
   const int gb = std::min((R.n + GH_BINBLOCK - 1) / GH_BINBLOCK, 1024);
-  k_cell_bin<<<gb, GH_BINBLOCK, 0, s>>>(c->status, C, c->pos, c->skin);
+  k_cell_bin<<<gb, GH_BINBLOCK, 0, s>>>(c->status, C, c->pos, c->skin, c->step);
   k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(c->status, C);
   k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(c->status, C, c->pos);
-  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos);
+  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos, c->step);
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1345,6 +1418,10 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   // a fused step stores no forces, per-atom energies or densities: the
   // steps whose state is observed run unfused and store them
   double *rho1 = fuse_vv ? nullptr : c->rho;
+  StepArgs sa;
+  sa.step = c->step;
+  sa.skin = c->skin;
+  sa.prog = c->prog_d;
   if (c->pot == GH_POT_SMATB) {
     const size_t lds1 = gh_tile_slot_bytes(0, c->precision, c->nt) * slots;
     const size_t lds2 = gh_tile_slot_bytes(1, c->precision, c->nt) * slots;
@@ -1354,16 +1431,18 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (single)
         k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                         c->cell_start, c->nn, c->list16, ld,
-                                                        c->tbf, rho1, c->eband32, e0, c->tbf0);
+                                                        c->tbf, rho1, c->eband32, e0, c->tbf0,
+                                                        sa);
       else
         k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                          c->cell_start, c->nn, c->list16, ld,
-                                                         c->tbf, rho1, c->eband32, e0, c->tbf0);
+                                                         c->tbf, rho1, c->eband32, e0, c->tbf0,
+                                                         sa);
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
 #define TB_FX(SG, FU)                                                                         \
   k_tb_force_fx<SG, FU><<<grid, GH_TBLOCK, lds2, s>>>(                                        \
       c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap, c->tbf, \
-      c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0)
+      c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0, sa)
       if (single) {
         if (fuse_vv) TB_FX(true, true);
         else TB_FX(true, false);
@@ -1376,20 +1455,20 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (single)
         k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, rho1, c->eband, c->eband32);
+            c->nt, rho1, c->eband, c->eband32, sa);
       else
         k_tb_rho64_tiled<false><<<grid, GH_TBLOCK, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, rho1, c->eband, c->eband32);
+            c->nt, rho1, c->eband, c->eband32, sa);
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
       if (single)
         k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv);
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa);
       else
         k_tb_force64_tiled<false><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv);
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa);
     }
   } else if (c->pot == GH_POT_LJ || gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
@@ -1399,7 +1478,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   k_lj_tiled<MX, PT><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,  \
                                                   c->nn, c->list16, ld, c->ncap, c->lj_eps, \
                                                   c->lj_sigma, rc2, ev_ui, c->frc, c->epa,  \
-                                                  c->type, fv, c->pljd)
+                                                  c->type, fv, c->pljd, sa)
 #define LJ_POTS(MX)                                                                         \
   switch (c->pot) {                                                                         \
     case GH_POT_PLJ: LJ_LAUNCH(MX, GH_POT_PLJ); break;                                      \
