@@ -33,6 +33,7 @@ struct gh_batch {
   bool have_pot = false;
   double rcut = 0.0, skin = 1.0;
   int cap = 0;
+  int cap_req = 0;  // row capacity asked for by the last overflow
   long long step = 0;
   bool lists_valid = false;
   bool forces_valid = false;
@@ -82,7 +83,7 @@ struct BatchArgs {
   const double *T;
   double *out;  // per system: [0] epot ui, [1] ekin ui, [2] rebuilds, [3] fmax eV/A,
                 // [4] iterations, [5] converged
-  int *flags;   // [0] list overflow
+  int *flags;   // [0] list overflow, [1] longest row asked for past the capacity
   double e0;    // eband reference of the FP32 pair forms
 };
 
@@ -201,7 +202,10 @@ __device__ void sys_build(const BatchArgs &B, const Sys &S) {
       }
     }
     row.finish(n);
-    if (row.m > B.cap) B.flags[0] = 1;
+    if (row.m > B.cap) {
+      B.flags[0] = 1;
+      atomicMax(&B.flags[1], row.m);
+    }
     B.nn[i] = min(row.m, B.cap);
     B.xold[i] = pi.x;
     B.xold[B.ld + i] = pi.y;
@@ -707,15 +711,25 @@ static BatchArgs bargs(const gh_batch *b) {
   return A;
 }
 
+// Reports a row overflow of the last call and re-arms the flags; the next
+// call re-sizes the rows to the longest one asked for (plus a quarter).
 static int check_flags(gh_batch *b) {
-  int f = 0;
-  BH(b, hipMemcpyAsync(&f, b->flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  int f[2] = {0, 0};
+  BH(b, hipMemcpyAsync(f, b->flags, sizeof(f), hipMemcpyDeviceToHost, b->stream));
   BH(b, hipStreamSynchronize(b->stream));
-  if (f) {
+  if (f[0]) {
+    const int zero[2] = {0, 0};
+    BH(b, hipMemcpyAsync(b->flags, zero, sizeof(zero), hipMemcpyHostToDevice, b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
     b->lists_valid = false;
-    return bfail(b, GH_EOVERFLOW, "a cluster's neighbour row exceeded the capacity; "
-                                  "results of this call are invalid; call again after "
-                                  "gh_batch_set_skin or with fewer atoms per system");
+    b->forces_valid = false;
+    b->cap_req = f[1] + std::max(16, f[1] / 4);
+    const int cap = b->cap;
+    b->cap = 0;
+    return bfail(b, GH_EOVERFLOW,
+                 "a cluster's neighbour row exceeded the capacity (" + std::to_string(f[1]) +
+                     " > " + std::to_string(cap) + "); results of this call are invalid; "
+                     "the next call re-sizes the rows");
   }
   return 0;
 }
@@ -743,7 +757,7 @@ static int batch_ready(gh_batch *b) {
       for (int j = 0; j < b->nt; ++j) rmin = std::min(rmin, 0.8 * b->tb[i * GH_MAXT + j].r0);
     const double dens = 1.4142135623730951 / (rmin * rmin * rmin);
     int est = (int)(4.18879 * rl * rl * rl * dens) + 16;
-    b->cap = std::max(8, std::min(nmax, est));
+    b->cap = std::max(8, std::min(nmax, std::max(est, b->cap_req)));
     b->cap = (b->cap + 7) / 8 * 8;
     BT(balloc(b, &b->list, (size_t)(b->cap + 4 * GH_LIST_PAD_QUADS) * b->ld));
     // LDS staging of the largest system (int4 + type byte per atom, and the
@@ -880,6 +894,7 @@ int gh_batch_set_systems(gh_batch *b, int nsys, const int *off, const double *x,
   BT(balloc(b, &b->out, (size_t)nsys * OUTW));
   BT(balloc(b, &b->scale, nsys));
   b->cap = 0;
+  b->cap_req = 0;
   if (b->list) {
     BH(b, hipFree(b->list));
     b->list = nullptr;

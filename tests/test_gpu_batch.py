@@ -33,8 +33,8 @@ def _batch_agcu(prec, off, x, t):
     return b
 
 
-@pytest.mark.parametrize("prec,tol,nmax", [(hip.GH_FP64, 1e-10, 400), (hip.GH_MIXED, 1e-4, 400),
-                                           (hip.GH_FP64, 1e-10, 1200), (hip.GH_MIXED, 1e-4, 1200)])
+@pytest.mark.parametrize("prec,tol,nmax", [(hip.GH_FP64, 1e-10, 400), (hip.GH_MIXED, 1e-5, 400),
+                                           (hip.GH_FP64, 1e-10, 1200), (hip.GH_MIXED, 1e-5, 1200)])
 def test_batch_forces_match_oracle(prec, tol, nmax):
     # nmax 1200: batches with a system over 512 atoms run 512-thread workgroups
     off, x, t, T, lad = _agcu_small(nmax=nmax, ngeom=8)
@@ -165,3 +165,113 @@ def test_fire_relaxations_match_oracle():
         if abs(ro["epot"] - r["epot"][k]) < 1e-6 * n:
             same += 1
     assert same >= 0.75 * (len(off) - 1)
+
+
+def _agcu_sizes(sizes, seed=21):
+    """Ag-Cu fcc spheres of the given sizes (50:50 seeded labels, grouped by
+    element), as the replica-exchange workload builds them."""
+    rng = np.random.default_rng(seed)
+    a = CL.AGCU["r0"] * np.sqrt(2.0)
+    xs, ts, off = [], [], [0]
+    for n in sizes:
+        x = CL.fcc_sphere(int(n), a, rng.uniform(-0.5, 0.5, 3) * a)
+        lab = np.zeros(int(n), np.int32)
+        lab[rng.permutation(int(n))[: int(n) // 2]] = 1
+        order = np.argsort(lab, kind="stable")
+        xs.append(x[order])
+        ts.append(lab[order])
+        off.append(off[-1] + int(n))
+    return np.array(off, np.int32), np.concatenate(xs), np.concatenate(ts)
+
+
+# the replica-exchange workload's sizes reach 2000 atoms: systems above 1024
+# atoms run 1024-thread workgroups on a second stream beside the rest
+BIG = [1878, 38, 1100, 640, 1350, 95]
+
+
+@pytest.mark.parametrize("prec,tol", [(hip.GH_FP64, 1e-10), (hip.GH_MIXED, 1e-5)],
+                         ids=["fp64", "mixed"])
+def test_batch_forces_with_systems_above_1024_atoms(prec, tol):
+    off, x, t = _agcu_sizes(BIG)
+    with _batch_agcu(prec, off, x, t) as b:
+        F, ep = b.compute()
+    s = _oracle_agcu()
+    box = O.make_box(None)
+    for k in range(len(off) - 1):
+        a0, a1 = off[k], off[k + 1]
+        nn, lst, mode = O.neighbours(x[a0:a1], box, O.smatb_rcut(s) + 1.0)
+        Fo, eo, epo, _ = O.smatb_force(x[a0:a1], t[a0:a1], box, s, nn, lst)
+        assert cm.force_err(F[a0:a1], Fo) <= tol, (k, a1 - a0)
+        assert ep[k] == pytest.approx(epo * cm.UI_EV, rel=1e-10 if prec == 0 else 1e-7)
+
+
+@pytest.mark.parametrize("n", [1100, 1878])
+def test_batch_langevin_above_1024_atoms_equals_single_system_path(n):
+    off, x, t = _agcu_sizes([n, 200])  # the big one on the second stream
+    one = np.array([0, n], np.int32)
+    with _batch_agcu(hip.GH_FP64, off, x, t) as b:
+        b.set_temperatures([600.0, 300.0])
+        ep, ek = b.langevin(100, 0.002, 1.0, 99)
+        xb, vb = b.state()
+    with hip.Context(0, hip.GH_FP64) as c:
+        c.set_box(None)
+        c.set_atoms(x[:n], t[:n])
+        c.set_masses([CL.MASS_AG, CL.MASS_CU])
+        c.set_smatb(2, *CL.agcu_params())
+        c.integrator_ermak(0.002, 600.0, 1.0, rng=hip.GH_RNG_PHILOX, seed=99)
+        c.run(100)
+        xc, vc, _, _ = c.state()
+        epc, ekc = c.energies()
+    assert np.max(np.abs(xb[:n] - xc)) < 1e-9
+    assert np.max(np.abs(vb[:n] - vc)) < 1e-7
+    assert ep[0] == pytest.approx(epc, rel=1e-11)
+    assert ek[0] == pytest.approx(ekc, rel=1e-9)
+
+
+def test_replica_exchange_with_systems_above_1024_atoms():
+    sizes = [1878, 1878, 1878, 1100, 1100, 1100, 60, 60, 60]  # three 3-rung ladders
+    off, x, t = _agcu_sizes(sizes)
+    T = np.array([300.0, 520.0, 900.0] * 3)
+    lad = np.arange(9, dtype=np.int32).reshape(3, 3)
+    with _batch_agcu(hip.GH_MIXED, off, x, t) as b:
+        b.set_temperatures(T)
+        b.langevin(300, 0.002, 1.0, 5)
+        x0, v0 = b.state()
+        ladders = lad.copy()
+        nacc = b.exchange(ladders, 0, 77)
+        x1, v1 = b.state()
+    for l in range(3):
+        assert sorted(ladders[l]) == sorted(lad[l])
+    assert np.array_equal(x0, x1)
+    swapped = np.where(ladders != lad)
+    assert nacc == len(swapped[0]) // 2
+    for l, r in zip(*swapped):
+        s = lad[l, r]
+        f = np.sqrt(T[lad[l, list(ladders[l]).index(s)]] / T[s])
+        a0, a1 = off[s], off[s + 1]
+        assert np.allclose(v1[a0:a1], v0[a0:a1] * f, rtol=1e-12, atol=0)
+
+
+def test_batch_row_overflow_is_reported_and_resized():
+    # a 300-atom Au cluster squeezed to 70 %: its rows outgrow the capacity the
+    # batch sizes from close packing; the flags re-arm and the next call
+    # re-sizes the rows and matches the oracle
+    off, x0 = CL.au_fire_batch(nclusters=2, nmin=300, nmax=300, sigma=0.0, seed=3)
+    x = x0 * 0.7
+    p = [CL.AU["A"], CL.AU["xi"], CL.AU["p"], CL.AU["q"], CL.AU["r0"], 4.078,
+         4.078 * np.sqrt(1.5)]
+    with hip.Batch(0, hip.GH_FP64) as b:
+        b.set_systems(off, x)
+        b.set_smatb(1, *p)
+        b.set_masses([CL.MASS_AU])
+        with pytest.raises(hip.GhError) as e:
+            b.compute()
+        assert e.value.code == -4
+        F, ep = b.compute()
+    s = O.smatb_params([(0, 0) + tuple(p)], 1)
+    box = O.make_box(None)
+    for k in range(2):
+        a0, a1 = off[k], off[k + 1]
+        nn, lst, _ = O.neighbours(x[a0:a1], box, O.smatb_rcut(s) + 1.0)
+        Fo, _, epo, _ = O.smatb_force(x[a0:a1], np.zeros(a1 - a0, np.int32), box, s, nn, lst)
+        assert cm.force_err(F[a0:a1], Fo) <= 1e-10
