@@ -815,7 +815,7 @@ int gh_set_atoms(gh_ctx *c, int n, const double *x, const int *type) {
     GH_TRY(dalloc(c, &c->perm, ncap));
     GH_TRY(dalloc(c, &c->nn, ncap));
     c->nred = std::min(1024, std::max(1, (ncap + 255) / 256));
-    GH_TRY(dalloc(c, &c->red, (size_t)c->nred * 5 + 16));
+    GH_TRY(dalloc(c, &c->red, (size_t)c->nred * 10 + 16));
     if (c->gauss) {
       GH_HIP(c, hipFree(c->gauss));
       c->gauss = nullptr;

@@ -19,26 +19,6 @@
 namespace {
 using namespace ghd;
 
-// Block-wide sum of NV doubles; deterministic order.  Result valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double part[GH_BLOCK / 64][NV];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) part[wid][k] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < GH_BLOCK / 64; ++w) s += part[w][k];
-      v[k] = s;
-    }
-  }
-}
 
 __global__ void k_fill_int(int *__restrict__ p, int v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -695,38 +675,91 @@ __global__ __launch_bounds__(GH_BLOCK) void k_ermak_b(
 }
 
 // ---------------------------------------------------------------------------
-// Energies (write_energy, Output.F90:824-831): deterministic two-stage sums.
+// Energies (write_energy, Output.F90:824-831): deterministic two-stage sums,
+// compensated: every running sum carries its rounding error beside it
+// (TwoSum, Knuth), partial sums travel as (sum, error) pairs and the result
+// is sum + error, so the total does not depend on the grid's order of
+// accumulation beyond the last bit.  (The reference sums per atom with an
+// OpenMP reduction, TB.F90:472-477.)
 // stage A: [epot, M, Px, Py, Pz] partials; stage B: sum m |v - vcm|^2.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void dd_add(double &s, double &c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (x - bp);
+  s = t;
+}
+
+// Block-wide compensated sums of NV values; deterministic order.  Result
+// (sum, error) valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum_dd(double (&s)[NV], double (&c)[NV]) {
+  __shared__ double part[GH_BLOCK / 64][NV][2];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    for (int o = 32; o > 0; o >>= 1) {
+      const double s2 = __shfl_xor(s[k], o, 64), c2 = __shfl_xor(c[k], o, 64);
+      dd_add(s[k], c[k], s2);
+      c[k] += c2;
+    }
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      part[wid][k][0] = s[k];
+      part[wid][k][1] = c[k];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double ss = 0.0, cc = 0.0;
+      for (int w = 0; w < GH_BLOCK / 64; ++w) {
+        dd_add(ss, cc, part[w][k][0]);
+        cc += part[w][k][1];
+      }
+      s[k] = ss;
+      c[k] = cc;
+    }
+  }
+}
+
 __global__ __launch_bounds__(GH_BLOCK) void k_red_a(int n, int ncap, MassArgs ma,
                                                     const int *__restrict__ type,
                                                     const double *__restrict__ epa,
                                                     const double *__restrict__ vel,
                                                     double *__restrict__ part) {
-  double v[5] = {0, 0, 0, 0, 0};
+  double v[5] = {0, 0, 0, 0, 0}, e[5] = {0, 0, 0, 0, 0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double m = 1.0 / ma.om[type[i]];
-    v[0] += epa[i];
-    v[1] += m;
-    v[2] += m * vel[i];
-    v[3] += m * vel[ncap + i];
-    v[4] += m * vel[2 * ncap + i];
+    dd_add(v[0], e[0], epa[i]);
+    dd_add(v[1], e[1], m);
+    dd_add(v[2], e[2], m * vel[i]);
+    dd_add(v[3], e[3], m * vel[ncap + i]);
+    dd_add(v[4], e[4], m * vel[2 * ncap + i]);
   }
-  block_sum<5>(v);
+  block_sum_dd<5>(v, e);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 5; ++k) part[blockIdx.x * 5 + k] = v[k];
+    for (int k = 0; k < 5; ++k) {
+      part[(blockIdx.x * 5 + k) * 2] = v[k];
+      part[(blockIdx.x * 5 + k) * 2 + 1] = e[k];
+    }
 }
 
+// One workgroup: out[k] = the compensated total of the nblk (sum, error)
+// partials of value k (nv <= 5).
 __global__ __launch_bounds__(GH_BLOCK) void k_red_final(int nblk, int nv,
                                                         const double *__restrict__ part,
                                                         double *__restrict__ out) {
-  // one workgroup; nv <= 5
-  double v[5] = {0, 0, 0, 0, 0};
+  double v[5] = {0, 0, 0, 0, 0}, e[5] = {0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-    for (int k = 0; k < nv; ++k) v[k] += part[b * nv + k];
-  block_sum<5>(v);
+    for (int k = 0; k < nv; ++k) {
+      dd_add(v[k], e[k], part[(b * nv + k) * 2]);
+      e[k] += part[(b * nv + k) * 2 + 1];
+    }
+  block_sum_dd<5>(v, e);
   if (threadIdx.x == 0)
-    for (int k = 0; k < nv; ++k) out[k] = v[k];
+    for (int k = 0; k < nv; ++k) out[k] = v[k] + e[k];
 }
 
 __global__ __launch_bounds__(GH_BLOCK) void k_red_b(int n, int ncap, MassArgs ma,
@@ -736,14 +769,17 @@ __global__ __launch_bounds__(GH_BLOCK) void k_red_b(int n, int ncap, MassArgs ma
                                                     double *__restrict__ part) {
   const double M = sums[1];
   const double cx = sums[2] / M, cy = sums[3] / M, cz = sums[4] / M;
-  double v[1] = {0};
+  double v[1] = {0}, e[1] = {0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double m = 1.0 / ma.om[type[i]];
     const double wx = vel[i] - cx, wy = vel[ncap + i] - cy, wz = vel[2 * ncap + i] - cz;
-    v[0] += (wx * wx + wy * wy + wz * wz) * m;
+    dd_add(v[0], e[0], (wx * wx + wy * wy + wz * wz) * m);
   }
-  block_sum<1>(v);
-  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+  block_sum_dd<1>(v, e);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = e[0];
+  }
 }
 
 // Wraps positions into [0, L) on periodic axes (used when the box or the
@@ -1042,8 +1078,8 @@ int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui) {
   hipStream_t s = c->stream;
   const int nblk = std::min(c->nred, std::max(1, nblocks(c->n)));
   const MassArgs ma = make_mass(c);
-  double *part = c->red;                 // [nblk*5]
-  double *sums = c->red + 5 * c->nred;   // [5]
+  double *part = c->red;                 // [nblk*5] (sum, error) pairs
+  double *sums = c->red + 10 * c->nred;  // [5]
   double *kin = sums + 8;                // [1]
   k_red_a<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->epa, c->vel, part);
   k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 5, part, sums);
