@@ -335,6 +335,18 @@ module gems_hip
       type(c_ptr), value :: b
       real(c_double), intent(out) :: x(*), v(*)
     end function
+
+    ! device arrays are passed as c_ptr (their device addresses); no allocation
+    integer(c_int) function gh_cluster_forces_static(device, precision, n, nt, x, type, a, xi, &
+        p, q, r0, rci, rce, skin, f, e_atom, epot, status, stream) &
+        bind(C, name='gh_cluster_forces_static')
+      import :: c_ptr, c_double, c_int
+      integer(c_int), value :: device, precision, n, nt
+      type(c_ptr), value :: x, type
+      real(c_double), intent(in) :: a(*), xi(*), p(*), q(*), r0(*), rci(*), rce(*)
+      real(c_double), value :: skin
+      type(c_ptr), value :: f, e_atom, epot, status, stream
+    end function
     integer(c_int) function gh_get_stats(ctx, st) bind(C, name='gh_get_stats')
       import :: c_ptr, c_int, gh_stats
       type(c_ptr), value :: ctx

@@ -12,6 +12,10 @@
 //          with the neighbour's periodic image at x_j + wd*L_d.
 // Wavefronts are 64 lanes; per-atom kernels use 256-thread workgroups.
 #include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
 
 #include "gh_device.h"
 
@@ -1112,4 +1116,212 @@ int gh_fill_int(gh_ctx *c, int *p, int v, int n) {
   k_fill_int<<<nblocks(n), GH_BLOCK, 0, c->stream>>>(p, v, n);
   GH_HIP(c, hipGetLastError());
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// gh_cluster_forces_static: gh_compute of one free cluster (no box: the
+// O(N^2) verlet_self list and the SMA-TB passes above) from the caller's
+// device arrays, every intermediate in one file-scope device array.  No
+// device memory is allocated; the array lives as long as the process, one
+// copy per device.  Calls that overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+#define GH_STATIC_MAXN 2048  // atoms per call
+#define GH_STATIC_CAP 160    // list row capacity (a free cluster's rows hold < 90)
+#define GH_STATIC_MAXT 2     // atom types per call
+namespace {
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kSOffPos = al256(sizeof(GhStatus));
+constexpr size_t kSOffList = kSOffPos + al256(sizeof(double4) * GH_STATIC_MAXN);
+constexpr size_t kSOffNn = kSOffList + al256(sizeof(unsigned) * GH_STATIC_CAP * GH_STATIC_MAXN);
+constexpr size_t kSOffXold = kSOffNn + al256(sizeof(int) * GH_STATIC_MAXN);
+constexpr size_t kSOffRho = kSOffXold + al256(sizeof(double) * 3 * GH_STATIC_MAXN);
+constexpr size_t kSOffEband = kSOffRho + al256(sizeof(double) * GH_STATIC_MAXN);
+constexpr size_t kSOffEb32 = kSOffEband + al256(sizeof(double) * GH_STATIC_MAXN);
+constexpr size_t kSOffFrc = kSOffEb32 + al256(sizeof(float) * GH_STATIC_MAXN);
+constexpr size_t kSOffTb = kSOffFrc + al256(sizeof(double) * 3 * GH_STATIC_MAXN);
+constexpr size_t kScratchBytes = kSOffTb + al256(sizeof(TbDev) * GH_MAXT * GH_MAXT);
+
+// the parameter blocks of up to GH_STATIC_MAXT types, by value
+struct StaticTb {
+  TbDev t[GH_STATIC_MAXT * GH_STATIC_MAXT];
+};
+
+// Re-arms the status for one build, stages positions (3n AoS -> double4)
+// and writes the parameter table.  Every field the later kernels read is
+// written here: nothing depends on an earlier call's contents.
+__global__ __launch_bounds__(GH_BLOCK) void k_static_init(unsigned char *scr, int n, int nt,
+                                                          const double *__restrict__ x,
+                                                          StaticTb tb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  GhStatus *st = reinterpret_cast<GhStatus *>(scr);
+  if (i == 0) {
+    st->maxdisp2_bits = 0ull;
+    st->rebuild = 1;
+    st->overflow = 0;
+    st->maxnn = 0;
+    st->outside = 0;
+    st->halt = 0;
+    st->halt_step = 0;
+  }
+  TbDev *tbd = reinterpret_cast<TbDev *>(scr + kSOffTb);
+  if (i < nt * nt) tbd[(i / nt) * GH_MAXT + i % nt] = tb.t[i];
+  if (i < n) {
+    double4 *pos = reinterpret_cast<double4 *>(scr + kSOffPos);
+    pos[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+  }
+}
+
+// Forces to the caller's AoS layout; one workgroup then sums the energies
+// (compensated) and reports the build's row overflow.
+__global__ __launch_bounds__(GH_BLOCK) void k_static_out(const unsigned char *scr, int n,
+                                                         double *__restrict__ F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *frc = reinterpret_cast<const double *>(scr + kSOffFrc);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[3 * i + d] = frc[d * GH_STATIC_MAXN + i];
+}
+__global__ __launch_bounds__(GH_BLOCK) void k_static_sum(const unsigned char *scr, int n,
+                                                         const double *__restrict__ e,
+                                                         double *__restrict__ epot,
+                                                         int *__restrict__ status) {
+  double v[1] = {0.0}, c[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dd_add(v[0], c[0], e[i]);
+  block_sum_dd<1>(v, c);
+  if (threadIdx.x == 0) {
+    *epot = v[0] + c[0];
+    const GhStatus *st = reinterpret_cast<const GhStatus *>(scr);
+    *status = st->overflow ? GH_EOVERFLOW : 0;
+  }
+}
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on each device, taken once per device with
+// that device current (the host-side name is only a stand-in).
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+extern "C" int gh_cluster_forces_static(int device, int precision, int n, int nt,
+                                        const double *x, const int *type, const double *A,
+                                        const double *xi, const double *p, const double *q,
+                                        const double *r0, const double *rci, const double *rce,
+                                        double skin, double *F, double *e_atom, double *epot,
+                                        int *status, void *stream) {
+  if (device < 0 || device >= 64 || (precision != GH_FP64 && precision != GH_MIXED))
+    return GH_EINVAL;
+  if (n < 1 || n > GH_STATIC_MAXN || nt < 1 || nt > GH_STATIC_MAXT || !(skin >= 0.0))
+    return GH_EINVAL;  // n <= GH_STATIC_MAXN atoms, nt <= GH_STATIC_MAXT types
+  if (!x || !type || !A || !xi || !p || !q || !r0 || !rci || !rce || !F || !e_atom || !epot ||
+      !status)
+    return GH_EINVAL;
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return GH_EHIP;
+  int rc = 0;
+  unsigned char *scr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void *ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) rc = GH_EHIP;
+      else g_scratch_ptr[device] = ptr;
+    }
+    scr = static_cast<unsigned char *>(g_scratch_ptr[device]);
+  }
+  // parameter blocks (TB.F90:106-170) as gh_set_smatb and gh_upload_tb build them
+  StaticTb tb;
+  memset(&tb, 0, sizeof(tb));
+  double rcut = 0.0;
+  const GhTbPair t00{A[0], xi[0], p[0], q[0], r0[0], rci[0], rce[0], 0, 0, 0, 0, 0, 0};
+  const double e0 = gh_tb_e0(&t00);
+  for (int a = 0; a < nt && rc == 0; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const int k = a * nt + b;
+      if (!(r0[k] > 0) || !(rci[k] > 0) || !(rce[k] > rci[k])) rc = GH_EINVAL;
+      GhTbPair t{A[k], xi[k], p[k], q[k], r0[k], rci[k], rce[k], 0, 0, 0, 0, 0, 0};
+      gh_tb_coeffs(t);
+      TbDev &d = tb.t[k];
+      d.ca = t.A;
+      d.eps2 = t.xi * t.xi;
+      d.p = t.p;
+      d.q = t.q;
+      d.r0 = t.r0;
+      d.one_r0 = 1.0 / t.r0;
+      d.rci = t.rci;
+      d.rce = t.rce;
+      d.rce2 = t.rce * t.rce;
+      d.b1 = t.b1;
+      d.b2 = t.b2;
+      d.b3 = t.b3;
+      d.r1 = t.r1;
+      d.r2 = t.r2;
+      d.r3 = t.r3;
+      d.p_r0 = t.p / t.r0;
+      d.q2_r0 = 2.0 * t.q / t.r0;
+      gh_tbf_fill(t, 0, e0, d.f);
+      rcut = std::max(rcut, t.rce);
+    }
+  if (rc == 0) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    GhStatus *st = reinterpret_cast<GhStatus *>(scr);
+    double4 *pos = reinterpret_cast<double4 *>(scr + kSOffPos);
+    unsigned *list = reinterpret_cast<unsigned *>(scr + kSOffList);
+    int *nn = reinterpret_cast<int *>(scr + kSOffNn);
+    double *xold = reinterpret_cast<double *>(scr + kSOffXold);
+    double *rho = reinterpret_cast<double *>(scr + kSOffRho);
+    double *eband = reinterpret_cast<double *>(scr + kSOffEband);
+    float *eb32 = reinterpret_cast<float *>(scr + kSOffEb32);
+    double *frc = reinterpret_cast<double *>(scr + kSOffFrc);
+    const TbDev *tbd = reinterpret_cast<const TbDev *>(scr + kSOffTb);
+    const int nb = nblocks(n), ld = GH_STATIC_MAXN;
+    const double L = 1e6, rl = rcut + skin;
+    GridArgs g;
+    memset(&g, 0, sizeof(g));
+    for (int d = 0; d < 3; ++d) {
+      g.nc[d] = g.gn[d] = 1;
+      g.L[d] = L;
+      g.one_cell[d] = 1.0 / L;
+    }
+    g.ncell = 1;
+    const double ev_ui = ghc::ev_ui();
+    const bool single = nt == 1;
+    k_static_init<<<std::max(nb, 1), GH_BLOCK, 0, s>>>(scr, n, nt, x, tb);
+    k_build_n2<<<nb, GH_BLOCK, 0, s>>>(st, pos, n, g, rl * rl, GH_STATIC_CAP, ld, 0, nn, list,
+                                        xold, ld, nullptr, nullptr, nullptr, 0);
+    if (precision == GH_MIXED) {
+      if (single) {
+        k_tb_rho_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, tbd, nt, L, L,
+                                                         L, rho, eband, eb32, e0);
+        k_tb_force_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, ld, tbd,
+                                                           nt, L, L, L, rho, eb32, ev_ui, frc,
+                                                           e_atom);
+      } else {
+        k_tb_rho_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, tbd, nt, L,
+                                                          L, L, rho, eband, eb32, e0);
+        k_tb_force_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, ld, tbd,
+                                                            nt, L, L, L, rho, eb32, ev_ui, frc,
+                                                            e_atom);
+      }
+    } else {
+      if (single) {
+        k_tb_rho64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, tbd, nt, L, L,
+                                                        L, rho, eband, eb32);
+        k_tb_force64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, ld, tbd, nt,
+                                                          L, L, L, rho, eband, ev_ui, frc,
+                                                          e_atom);
+      } else {
+        k_tb_rho64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, tbd, nt, L,
+                                                         L, L, rho, eband, eb32);
+        k_tb_force64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(pos, type, nn, list, ld, n, ld, tbd,
+                                                           nt, L, L, L, rho, eband, ev_ui, frc,
+                                                           e_atom);
+      }
+    }
+    k_static_out<<<nb, GH_BLOCK, 0, s>>>(scr, n, F);
+    k_static_sum<<<1, GH_BLOCK, 0, s>>>(scr, n, e_atom, epot, status);
+    if (hipGetLastError() != hipSuccess) rc = GH_EHIP;
+  }
+  (void)hipSetDevice(prev);
+  return rc;
 }

@@ -29,6 +29,7 @@ SYMBOLS = [
     "gh_batch_set_systems", "gh_batch_set_smatb", "gh_batch_set_masses", "gh_batch_set_skin",
     "gh_batch_set_velocities", "gh_batch_set_temperatures", "gh_batch_compute",
     "gh_batch_langevin", "gh_batch_exchange", "gh_batch_fire", "gh_batch_get_state",
+    "gh_cluster_forces_static",
 ]
 
 
@@ -114,6 +115,8 @@ def load():
     L.gh_batch_fire.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                 _ip, _ip, _dp, _dp]
     L.gh_batch_get_state.argtypes = [vp, _dp, _dp]
+    L.gh_cluster_forces_static.argtypes = ([C.c_int] * 4 + [vp, vp] + [_dp] * 7 +
+                                           [C.c_double] + [vp] * 5)
     _lib = L
     return L
 

@@ -300,6 +300,28 @@ int gh_batch_fire(gh_batch *b, int max_iter, double dt_max, double alpha0, doubl
 /* Copy back positions and velocities (3 per atom). */
 int gh_batch_get_state(gh_batch *b, double *x, double *v);
 
+/* ------------------------------------------------------------------------
+ * Forces without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  The SMA-TB forces of one free cluster (no box), computed
+ * as gh_compute computes them on a context without a box (the O(N^2)
+ * verlet_self list, both passes, `precision` GH_FP64 or GH_MIXED), from the
+ * caller's DEVICE arrays: x (3n, AoS, A), type (n, values 0..nt-1) in, F
+ * (3n, AoS, ui/A), e_atom (n, ui), epot (1, ui) and status (1: 0, or
+ * GH_EOVERFLOW when a list row exceeded its capacity) out.  The parameter
+ * matrices (nt x nt, row-major, as gh_set_smatb) are host arrays.  Every
+ * intermediate lives in one static device array of the library (about
+ * 1.6 MB per device, held until the process ends); nothing is allocated.
+ * Limits: n <= 2048, nt <= 2 (GH_EINVAL beyond).  Work is enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream) and nothing waits for it.
+ * Calls that overlap on one device share the static array and race.
+ * ------------------------------------------------------------------------ */
+int gh_cluster_forces_static(int device, int precision, int n, int nt, const double *x,
+                             const int *type, const double *A, const double *xi,
+                             const double *p, const double *q, const double *r0,
+                             const double *rci, const double *rce, double skin, double *F,
+                             double *e_atom, double *epot, int *status, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
