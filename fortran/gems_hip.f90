@@ -6,6 +6,7 @@
 ! Entry points and the GEMS routines they replace:
 !   gh_compute          interact()            src/Interaction.f90:135-241
 !   gh_set_positions    ig%at(i)%o%pos        src/Neighbour.F90:114
+!   gh_set_accelerations atom%acel            src/Atoms.F90:70
 !   gh_set_smatb        smatb_set/readprm     src/TB.F90:106-170, 291-365
 !   gh_set_lj           lj_set                src/Pairs.F90:198-205
 !   gh_set_plj          plj_set               src/Pairs.F90:282-291
@@ -115,6 +116,12 @@ module gems_hip
       import :: c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: v(*)
+    end function
+
+    integer(c_int) function gh_set_accelerations(ctx, a) bind(C, name='gh_set_accelerations')
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: a(*)
     end function
 
     integer(c_int) function gh_set_smatb(ctx, nt, A, xi, p, q, r0, rci, rce) &

@@ -935,6 +935,13 @@ int gh_set_velocities(gh_ctx *c, const double *v) {
   return put_soa3(c, c->vel, orig, v);
 }
 
+int gh_set_accelerations(gh_ctx *c, const double *a) {
+  if (!c || !c->have_atoms) return gh_fail(c, GH_ESTATE, "no atoms");
+  std::vector<int> orig;
+  GH_TRY(fetch_orig(c, orig));
+  return put_soa3(c, c->acc, orig, a);
+}
+
 int gh_set_smatb(gh_ctx *c, int nt, const double *A, const double *xi, const double *p,
                  const double *q, const double *r0, const double *rci, const double *rce) {
   if (!c || nt <= 0 || nt > GH_MAXT || !A || !xi || !p || !q || !r0 || !rci || !rce)

@@ -19,7 +19,8 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "EOVERFLOW", -5: "ERANGE"}
 # every symbol declared in include/gems_hip.h
 SYMBOLS = [
     "gh_create", "gh_destroy", "gh_last_error", "gh_set_precision", "gh_set_box", "gh_set_images", "gh_set_pair_masks", "gh_set_sm1",
-    "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities", "gh_set_smatb",
+    "gh_set_atoms", "gh_set_positions", "gh_set_masses", "gh_set_velocities",
+    "gh_set_accelerations", "gh_set_smatb",
     "gh_set_lj", "gh_set_plj", "gh_set_slj", "gh_set_wca", "gh_get_lcg_state",
     "gh_set_lcg_state", "gh_lbfgs",
     "gh_set_skin", "gh_compute", "gh_neighbors", "gh_prng_lcg", "gh_set_gdist",
@@ -72,6 +73,7 @@ def load():
     L.gh_set_atoms.argtypes = [vp, C.c_int, _dp, _ip]
     L.gh_set_masses.argtypes = [vp, C.c_int, _dp]
     L.gh_set_velocities.argtypes = [vp, _dp]
+    L.gh_set_accelerations.argtypes = [vp, _dp]
     L.gh_set_positions.argtypes = [vp, _dp]
     L.gh_set_smatb.argtypes = [vp, C.c_int] + [_dp] * 7
     L.gh_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double]
@@ -199,6 +201,10 @@ class Context:
     def set_velocities(self, v=None):
         vv = None if v is None else np.ascontiguousarray(v, np.float64)
         self._ck(self.L.gh_set_velocities(self.h, _p(vv)))
+
+    def set_accelerations(self, a=None):
+        aa = None if a is None else np.ascontiguousarray(a, np.float64)
+        self._ck(self.L.gh_set_accelerations(self.h, _p(aa)))
 
     def set_positions(self, x):
         """New positions (n x 3, caller order) for the next compute; lists kept."""

@@ -103,6 +103,11 @@ int gh_set_atoms(gh_ctx *ctx, int n, const double *x, const int *type);
 int gh_set_masses(gh_ctx *ctx, int nt, const double *mass);
 /* Velocities, 3n (A/ps); NULL sets them to zero. */
 int gh_set_velocities(gh_ctx *ctx, const double *v);
+/* Accelerations F/m, 3n (A/ps^2 in ui): atom%acel (Atoms.F90:70), which
+ * ermak_a (Integration.F90:963-1011) reads from the previous step's ermak_b;
+ * NULL sets them to zero (as gh_set_atoms does).  Velocity Verlet recomputes
+ * them from the forces. */
+int gh_set_accelerations(gh_ctx *ctx, const double *a);
 
 /* SMA-TB parameters for every type pair, row-major nt*nt arrays (eV, A).
  * Replaces smatb_set / smatb_readprm (TB.F90:106-170, 291-365); the quintic
