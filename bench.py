@@ -101,7 +101,7 @@ def timed_run(c, steps, warmup, timing=False):
     tm = c.timing()
     c.set_timing(False)
     st1 = c.stats()
-    return t1 - t0, tm, st1["nupd"] - st0["nupd"]
+    return t1 - t0, tm, st1["nupd"] - st0["nupd"], st1["launches"] - st0["launches"]
 
 
 def cpu_baseline(x, L, skin, budget_s=20.0):
@@ -128,6 +128,153 @@ def cpu_baseline(x, L, skin, budget_s=20.0):
                 sample="%d VV steps of the same %d-atom Au workload (oracle/gems_oracle.c, "
                        "OpenMP, %d threads; reference semantics incl. its rebuild rule: "
                        "%d rebuilds)" % (steps, len(x), cores, md.nupd - 1))
+
+
+def _pool_map(fn, items, cores):
+    """fn over items on host threads (the C oracle runs without the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        return list(ex.map(fn, items))
+
+
+def _cores():
+    cores = len(os.sched_getaffinity(0))
+    return min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+
+
+def cpu_md_baseline(x, L, pot, mass, dt, T, skin, budget_s, max_steps, what):
+    """The oracle's velocity-Verlet loop (OpenMP on the host cores) over as
+    many steps of a bulk workload as fit the budget."""
+    from oracle import oracle as O
+    cores = _cores()
+    box = O.make_box([L] * 3, (1, 1, 1))
+    m = np.full(len(x), mass)
+    v = O.set_gdist(O.LCG(4242 if pot == "smatb" else 777), T, m)
+    kw = dict(pot="smatb", tb=O.smatb_params([(0, 0, AU["A"], AU["xi"], AU["p"], AU["q"],
+                                                AU["r0"], A_LAT, A_LAT * np.sqrt(1.5))], 1),
+              maxnb=96) if pot == "smatb" else dict(pot="lj", lj=(1.0, 1.0, 2.5), maxnb=96)
+    md = O.MDRun(x, v, m, box, types=np.zeros(len(x)), skin=skin, integ="vv", dt=dt,
+                 nthreads=cores, **kw)
+    md.interact()  # the first force evaluation (not timed, as in dinamic)
+    steps, t = 0, 0.0
+    while t < budget_s and steps < max_steps:
+        t0 = time.perf_counter()
+        md.run(10, each=0)
+        t += time.perf_counter() - t0
+        steps += 10
+    return dict(value=len(x) * steps / t, unit="atom-steps/s", cores=cores, kind="port",
+                sample="%d of the %s's VV steps (oracle/gems_oracle.c, OpenMP, %d threads, "
+                       "the reference's rebuild rule: %d rebuilds)"
+                       % (steps, what, cores, md.nupd - 1))
+
+
+def cpu_re_baseline(budget_s):
+    """The oracle's Ermak Langevin (ermak_a/b, Integration.F90:963-1031) of a
+    stratified sample of the replica-exchange batch (one cluster per
+    geometry, the rungs in turn), each cluster on one host thread."""
+    from gems_amd import clusters as CL
+    from oracle import oracle as O
+    cores = _cores()
+    off, x, t, T, lad = CL.agcu_re_batch()
+    A, xi, p, q, r0, rci, rce = CL.agcu_params()
+    s = O.smatb_params([(i, j, A[i, j], xi[i, j], p[i, j], q[i, j], r0[i, j], rci[i, j],
+                         rce[i, j]) for i in range(2) for j in range(2)], 2)
+    box = O.make_box(None)
+    sample = [int(lad[g, g % lad.shape[1]]) for g in range(lad.shape[0])]
+    runs = []
+    for k in sample:
+        a0, a1 = off[k], off[k + 1]
+        m = np.where(t[a0:a1] == 0, CL.MASS_AG, CL.MASS_CU)
+        md = O.MDRun(x[a0:a1], np.zeros((a1 - a0, 3)), m, box, pot="smatb", tb=s,
+                     types=t[a0:a1], skin=1.0, integ="ermak", dt=0.002, T=float(T[k]),
+                     gamma=1.0, rng=O.LCG(1000 + k), nthreads=1)
+        md.interact()
+        runs.append(md)
+    chunk, steps, wall = 20, 0, 0.0
+    while wall < budget_s and steps < 2000:
+        t0 = time.perf_counter()
+        _pool_map(lambda md: md.run(chunk, each=0), runs, cores)
+        wall += time.perf_counter() - t0
+        steps += chunk
+    atoms = sum(md.n for md in runs)
+    return dict(value=len(runs) * steps / wall, unit="cluster-steps/s", cores=cores, kind="port",
+                atom_steps_per_s=atoms * steps / wall,
+                sample="%d of the 512 replica clusters (one per geometry, %d atoms in all, "
+                       "every rung represented), %d Ermak steps each, one host thread per "
+                       "cluster over %d threads" % (len(runs), atoms, steps, cores))
+
+
+def cpu_fire_sample(x, off, sample, cores):
+    """The oracle's FP64 FIRE of the sampled clusters on host threads; returns
+    (results, wall seconds)."""
+    from gems_amd import clusters as CL
+    from oracle import oracle as O
+    s = O.smatb_params([(0, 0, AU["A"], AU["xi"], AU["p"], AU["q"], AU["r0"], A_LAT,
+                         A_LAT * np.sqrt(1.5))], 1)
+
+    def one(k):
+        a0, a1 = off[k], off[k + 1]
+        return O.fire(x[a0:a1], np.zeros(a1 - a0), s, np.full(a1 - a0, CL.MASS_AU),
+                      max_iter=20000, dt_max=0.005, alpha0=0.1, ftol=1e-4)
+
+    t0 = time.perf_counter()
+    res = _pool_map(one, sample, cores)
+    return res, time.perf_counter() - t0
+
+
+def gems_script(nx, nsteps, each):
+    """The Au NVE run as a GEMS script (fcc nx^3 cells, the workload's jitter,
+    `prng lcg` 4242, `set tempgdist 300`, velocity Verlet dt 2 fs, skin 1 A,
+    energy every `each` steps), with its input files; returns (files, script)."""
+    x, L = workload(nx)
+    xyz = "%d\n\n" % len(x) + "".join("Au %.17e %.17e %.17e\n" % tuple(r) for r in x)
+    prm = "GUPTA\nAu Au " + " ".join("%.17e" % v for v in (
+        AU["A"], AU["xi"], AU["p"], AU["q"], AU["r0"], A_LAT, A_LAT * np.sqrt(1.5))) + "\n"
+    s = ("dimension 3\nprng lcg\nprng seed 4242\nout float 25.16\n>< read x.xyz\nsys add\n"
+         "box size %.17e %.17e %.17e\n> sys\nset pbc T T T\n> sys\ngroup 1 add\n"
+         "interact 1 under tb read p.prm\ntime step 0.002\n> sys\nset tempgdist 300.\n"
+         "outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each %d\n"
+         "> sys\nevolve v_verlet\ndinamica %d\n" % (L, L, L, each, nsteps))
+    return {"x.xyz": xyz, "p.prm": prm}, s
+
+
+def gems_driven(nx=20, nsteps=2000, each=100):
+    """GEMS's own binary (fortran/_build/gems, GEMS_BACKEND=hip) on the Au 32k
+    NVE script: GEMS reads the script, builds the system and draws the
+    velocities on the host; its `dinamica` loop runs on the device in chunks
+    between the energy rows.  Rate = atom-steps / wall time of the loop as
+    GEMS's log reports it (state written back at every row)."""
+    import re
+    import subprocess
+    import tempfile
+    exe = os.path.join(REPO, "fortran", "_build", "gems")
+    if not os.path.exists(exe):
+        return None
+    files, script = gems_script(nx, nsteps, each)
+    out = {"workload": "GEMS script: Au SMA-TB fcc %d^3 cells = %d atoms, 300 K (lcg 4242), VV "
+                       "dt 2 fs, skin 1 A, `dinamica %d`, energy row every %d steps"
+                       % (nx, 4 * nx ** 3, nsteps, each)}
+    for prec in ("fp64", "mixed"):
+        with tempfile.TemporaryDirectory() as wd:
+            for k, v in files.items():
+                with open(os.path.join(wd, k), "w") as f:
+                    f.write(v)
+            with open(os.path.join(wd, "in.gms"), "w") as f:
+                f.write(script)
+            env = dict(os.environ, GEMS_BACKEND="hip", GEMS_HIP_PRECISION=prec)
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "in.gms"], cwd=wd, capture_output=True, text=True,
+                               timeout=300, env=env)
+            wall = time.perf_counter() - t0
+            log = open(os.path.join(wd, "in.log")).read()
+            m = re.search(r"dinamica done in ([0-9.]+) s \(\s*([0-9.Ee+-]+) atom-steps/s", log)
+            if r.returncode != 0 or m is None:
+                out[prec] = {"error": log[-500:]}
+                continue
+            out[prec] = {"atom_steps_per_s": float(m.group(2)), "loop_s": float(m.group(1)),
+                         "process_s": wall,
+                         "rows": len(np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3))}
+    return out
 
 
 def nve_drift(hip, precision, x, L, skin, nsteps, tile=None, each=100):
@@ -207,10 +354,11 @@ def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72):
         c.integrator_vv(0.005 * tau)
         shape, _ = c.autotune(reps=3)
         F[name] = c.compute()[0]
-        dt, _, nupd = timed_run(c, steps, warmup)
+        dt, _, nupd, launches = timed_run(c, steps, warmup)
         c.close()
         out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
-                     "rebuilds": nupd, "brick_cells": shape}
+                     "rebuilds": nupd, "launches_per_step": launches / steps,
+                     "brick_cells": shape}
     if jitter > 0:  # on the perfect lattice the forces vanish and the ratio means nothing
         rms = float(np.sqrt(np.mean(F["fp64"] ** 2)))
         out["force_err_mixed_vs_fp64_rms_over_rms"] = float(
@@ -227,10 +375,11 @@ def au_32k(hip, steps=2000, warmup=20):
     for name, prec in (("mixed", hip.GH_MIXED), ("fp64", hip.GH_FP64)):
         c = make_ctx(hip, prec, x, L, 1.0)
         shape, _ = c.autotune(reps=3)
-        dt, _, nupd = timed_run(c, steps, warmup)
+        dt, _, nupd, launches = timed_run(c, steps, warmup)
         c.close()
         out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
-                     "rebuilds": nupd, "brick_cells": shape}
+                     "rebuilds": nupd, "launches_per_step": launches / steps,
+                     "brick_cells": shape}
     return out
 
 
@@ -265,10 +414,13 @@ def batch_re(hip, rounds, swap_every=500):
                         "exchange every %d steps, mixed precision" % swap_every}
 
 
-def batch_fire(hip, nclusters=4096):
+def batch_fire(hip, nclusters=4096, sample_every=64, cpu=True):
     """FIRE relaxation of 4096 Au clusters (13..309 atoms): FP32 pair terms
-    until max|F| < 1e-2 eV/A, FP64 polish to 1e-4 eV/A; the same batch relaxed
-    in plain FP64 gives the final-energy error."""
+    until max|F| < 1e-2 eV/A, FP64 polish to 1e-4 eV/A.  The final energies
+    of every `sample_every`-th cluster are compared with the oracle's FP64
+    FIRE from the same start (timed on the host cores: the CPU baseline of
+    this workload); the same batch relaxed in plain FP64 on the GPU is timed
+    beside it."""
     from gems_amd import clusters as CL
     off, x = CL.au_fire_batch(nclusters)
     out = {}
@@ -284,19 +436,32 @@ def batch_fire(hip, nclusters=4096):
     dt, r = out[hip.GH_MIXED]
     dt64, r64 = out[hip.GH_FP64]
     n = np.diff(off)
-    same = np.abs(r["epot"] - r64["epot"]) < 1e-6 * n
-    return {"clusters": nclusters, "atoms": int(off[-1]), "seconds": dt,
-            "clusters_relaxed_per_s": nclusters / dt,
-            "converged": float(np.mean(r["converged"])),
-            "iters_mean": float(np.mean(r["iters"])), "iters_max": int(np.max(r["iters"])),
-            "fp64_seconds": dt64, "fp64_clusters_relaxed_per_s": nclusters / dt64,
-            "same_minimum_as_fp64": float(np.mean(same)),
-            "final_energy_err_ev_per_atom_same_minimum":
-                float(np.max(np.abs(r["epot"] - r64["epot"])[same] / n[same])) if same.any()
-                else None,
-            "workload": "4096 Au clusters, sizes uniform 13..309, half fcc spheres / half "
-                        "Mackay fragments, 0.1 A Gaussian perturbation; FIRE dt_max 5 fs, "
-                        "alpha0 0.1, FP32 pair terms to 1e-2 eV/A then FP64 to 1e-4 eV/A"}
+    res = {"clusters": nclusters, "atoms": int(off[-1]), "seconds": dt,
+           "clusters_relaxed_per_s": nclusters / dt,
+           "converged": float(np.mean(r["converged"])),
+           "iters_mean": float(np.mean(r["iters"])), "iters_max": int(np.max(r["iters"])),
+           "fp64_gpu_seconds": dt64, "fp64_gpu_clusters_relaxed_per_s": nclusters / dt64,
+           "workload": "4096 Au clusters, sizes uniform 13..309, half fcc spheres / half "
+                       "Mackay fragments, 0.1 A Gaussian perturbation; FIRE dt_max 5 fs, "
+                       "alpha0 0.1, FP32 pair terms to 1e-2 eV/A then FP64 to 1e-4 eV/A"}
+    if cpu:
+        sample = list(range(0, nclusters, sample_every))
+        cores = _cores()
+        ro, wall = cpu_fire_sample(x, off, sample, cores)
+        eo = np.array([q["epot"] for q in ro])
+        de = np.abs(r["epot"][sample] - eo) / n[sample]
+        same = de < 1e-6
+        res["vs_oracle_fp64_fire"] = {
+            "clusters": len(sample), "same_minimum": float(np.mean(same)),
+            "max_final_energy_err_ev_per_atom_same_minimum":
+                float(de[same].max()) if same.any() else None,
+            "note": "every %d-th cluster relaxed by the oracle's FP64 FIRE from the same "
+                    "start; same minimum = |dE| < 1e-6 eV/atom" % sample_every}
+        res["cpu_baseline"] = dict(
+            value=len(sample) / wall, unit="clusters relaxed/s", cores=cores, kind="port",
+            sample="the same %d clusters relaxed by oracle/gems_oracle.c's FP64 FIRE, one "
+                   "host thread per cluster over %d threads" % (len(sample), cores))
+    return res
 
 
 def copy_probe(nbytes=1 << 30, reps=5):
@@ -314,7 +479,7 @@ def copy_probe(nbytes=1 << 30, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=63)
     ap.add_argument("--skin", type=float, default=1.0)
@@ -326,6 +491,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-gems", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--re-rounds", type=int, default=4)
     ap.add_argument("--fire-clusters", type=int, default=4096)
@@ -351,9 +517,9 @@ def main():
     tile_mx = fixed_tile(c, args.tile) if args.tile else c.autotune(reps=3)
     F_mx, _, et_mx = c.compute()
     e0 = c.energies()
-    dt_mx, _, nupd_mx = timed_run(c, args.steps, args.warmup)
+    dt_mx, _, nupd_mx, launches_mx = timed_run(c, args.steps, args.warmup)
     e1 = c.energies()
-    dt_ev, tm_mx, _ = timed_run(c, args.steps, 0, timing=True)
+    dt_ev, tm_mx, _, _ = timed_run(c, args.steps, 0, timing=True)
     c.close()
 
     # plain FP64 GPU form, same input
@@ -361,8 +527,8 @@ def main():
     c = make_ctx(hip, hip.GH_FP64, x, L, args.skin)
     tile_64 = fixed_tile(c, args.tile64) if args.tile64 else c.autotune(reps=3)
     F_64, _, et_64 = c.compute()
-    dt_64, _, nupd_64 = timed_run(c, k64, max(2, args.warmup // 2))
-    _, tm_64, _ = timed_run(c, k64, 0, timing=True)
+    dt_64, _, nupd_64, launches_64 = timed_run(c, k64, max(2, args.warmup // 2))
+    _, tm_64, _, _ = timed_run(c, k64, 0, timing=True)
     c.close()
 
     value = n * args.steps / dt_mx
@@ -426,7 +592,7 @@ def main():
         "fp64": {"value": value64, "steps": k64, "ms_per_step": dt_64 / k64 * 1e3,
                  "force_pass_us": tm_64["force"][0] / tm_64["force"][1] * 1e3,
                  "rho_pass_us": tm_64["rho"][0] / tm_64["rho"][1] * 1e3,
-                 "rebuilds": nupd_64},
+                 "rebuilds": nupd_64, "launches_per_step": launches_64 / k64},
         "mixed_vs_fp64": value / value64,
         "mixed_vs_fp64_force": {
             "force_evaluation": (None if tile_mx[1] is None or tile_64[1] is None
@@ -442,6 +608,10 @@ def main():
         "epot_rel_err_vs_fp64": abs(et_mx / et_64 - 1.0),
         "energy_change_ev_per_atom": (e1[0] + e1[1] - e0[0] - e0[1]) / n,
         "rebuilds": nupd_mx,
+        "launches_per_step": launches_mx / args.steps,
+        "rebuild_note": "the timed window holds %d neighbour rebuilds (one every %.0f steps); "
+                        "the rebuild chain is enqueued only at steps where one is expected"
+                        % (nupd_mx, args.steps / max(nupd_mx, 1)),
         "tiles": {"mixed": {"brick_cells": tile_mx[0], "force_eval_us": tile_mx[1]},
                   "fp64": {"brick_cells": tile_64[0], "force_eval_us": tile_64[1]}},
         "step_breakdown_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in tm_mx.items()},
@@ -453,12 +623,26 @@ def main():
             "workload": "the same 1,000,188-atom Au NVE run, energy every 100 steps",
             "vs_reference_32k": drift_vs_reference(hip),
         }
+    if not args.no_gems:
+        out["gems_driven"] = gems_driven()
     if not args.no_batch:
         out["lj_4000"] = lj_bulk(hip, 10, 1000)
         out["lj_1m"] = lj_bulk(hip, 64, 100, jitter=0.01, Tstar=1.44)
         out["au_32k"] = au_32k(hip)
         out["batched_re"] = batch_re(hip, args.re_rounds)
-        out["batched_fire"] = batch_fire(hip, args.fire_clusters)
+        out["batched_fire"] = batch_fire(hip, args.fire_clusters, cpu=not args.no_cpu)
+        if not args.no_cpu:
+            b = args.cpu_budget / 2
+            alj = (4.0 / 0.8442) ** (1.0 / 3.0)
+            xl = np.mod(fcc(10, alj), 10 * alj)
+            out["lj_4000"]["cpu_baseline"] = cpu_md_baseline(
+                xl, 10 * alj, "lj", 1.0, 0.005 * 0.01018047, 0.72 / 8.617330337217213e-05,
+                0.3, b, 1000, "LJ 4000-atom workload")
+            x32, L32 = workload(20)
+            out["au_32k"]["cpu_baseline"] = cpu_md_baseline(
+                x32, L32, "smatb", MASS_AU, 0.002, 300.0, 1.0, b, 2000,
+                "Au 32,000-atom workload")
+            out["batched_re"]["cpu_baseline"] = cpu_re_baseline(b)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, L, args.skin, args.cpu_budget)
     print(json.dumps(out))

@@ -70,6 +70,7 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
   integer(c_int) :: rc, lcg(3)
   integer(8) :: m0, m, s, nxt, se, p10, k, kdone, s0, nupd0
   integer :: i, j, kind, nat, l, ist, tries, ngr
+  integer(8) :: c0, c1, crate
   logical :: periodic, ok
 
   done = .false.
@@ -153,6 +154,7 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
   call wlog('HIP', trim(msg))
 
   ! ---- the loop, in chunks between the steps GEMS looks at
+  call system_clock(c0, crate)
   m0 = nint(dm_steps, 8)
   m = m0
   do while (m < m0 + steps)
@@ -234,6 +236,11 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
     endif
     if (b_time) call timer_dump(int(m - m0), time, nupd_vlist)
   enddo
+  call system_clock(c1)
+  write(msg, '(a,f0.3,a,es10.3,a)') 'dinamica done in ', real(c1 - c0, dp) / real(crate, dp), &
+      ' s (', real(steps, dp) * nat * real(crate, dp) / real(max(1_8, c1 - c0), dp), &
+      ' atom-steps/s with the state written back where GEMS looks at it)'
+  call wlog('HIP', trim(msg))
   done = .true.
 
 contains

@@ -1594,6 +1594,7 @@ int gh_get_stats(gh_ctx *c, gh_stats *s) {
   for (int d = 0; d < 3; ++d) s->ncells[d] = c->list_mode ? c->nc[d] : 0;
   s->list_mode = c->list_mode;
   s->steps = c->step;
+  s->launches = c->launches;
   return 0;
 }
 

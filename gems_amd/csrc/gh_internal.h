@@ -202,6 +202,7 @@ struct gh_ctx {
   GhLcg lcg;
   bool first_force = true;   // forces not yet valid (interact not called)
   long long step = 0;
+  long long launches = 0;  // kernels enqueued on the context's stream
   // a run stopped at step `step` + 1 after its stepa: the forces and stepb of
   // that step are still due (the next gh_run applies them first)
   bool mid_step = false;

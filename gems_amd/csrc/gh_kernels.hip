@@ -928,17 +928,17 @@ int gh_launch_rebuild(gh_ctx *c, bool force) {
   }
   const int n = c->n, nb = nblocks(n);
   if (c->list_mode == 1) return gh_launch_rebuild_cells(c);  // decides in its first kernels
-  k_decide<<<1, 1, 0, s>>>(c->status, c->skin);
+  { k_decide<<<1, 1, 0, s>>>(c->status, c->skin); ++c->launches; }
   if (c->list_mode == 0 && c->boxed && (c->pbc[0] || c->pbc[1] || c->pbc[2]))
-    k_wrap_rebuild<<<nb, GH_BLOCK, 0, s>>>(c->status, n, make_wrap(c), c->pos);
+    { k_wrap_rebuild<<<nb, GH_BLOCK, 0, s>>>(c->status, n, make_wrap(c), c->pos); ++c->launches; }
   const GridArgs g = make_grid(c);
   const double rl = c->rcut + c->skin;
   const double rl2 = rl * rl;
   const int ld = c->ncap;
-  k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
+  { k_build_n2<<<nb, GH_BLOCK, 0, s>>>(c->status, c->pos, n, g, rl2, c->cap, ld, 0, c->nn,
                                      (unsigned *)c->list, c->xold, c->ncap, c->orig,
                                      c->masked ? c->mfeel : nullptr,
-                                     c->masked ? c->mfelt : nullptr, c->step);
+                                     c->masked ? c->mfelt : nullptr, c->step); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -962,60 +962,64 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   if (c->pot == GH_POT_SMATB) {
     if (mixed) {
       if (single)
-        k_tb_rho_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        { k_tb_rho_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                   c->nt, L0, L1, L2, c->rho, c->eband,
-                                                  c->eband32, c->tb_e0);
+                                                  c->eband32, c->tb_e0); ++c->launches; }
       else
-        k_tb_rho_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        { k_tb_rho_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                    c->nt, L0, L1, L2, c->rho, c->eband,
-                                                   c->eband32, c->tb_e0);
+                                                   c->eband32, c->tb_e0); ++c->launches; }
     } else {
       if (single)
-        k_tb_rho64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        { k_tb_rho64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                  c->nt, L0, L1, L2, c->rho, c->eband,
-                                                 c->eband32);
+                                                 c->eband32); ++c->launches; }
       else
-        k_tb_rho64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
+        { k_tb_rho64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->tbd,
                                                   c->nt, L0, L1, L2, c->rho, c->eband,
-                                                  c->eband32);
+                                                  c->eband32); ++c->launches; }
     }
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     if (mixed) {
       if (single)
-        k_tb_force_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        { k_tb_force_mx<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                     c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
-                                                    c->eband32, ev_ui, c->frc, c->epa);
+                                                    c->eband32, ev_ui, c->frc, c->epa); ++c->launches; }
       else
-        k_tb_force_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        { k_tb_force_mx<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                      c->ncap, c->tbd, c->nt, L0, L1, L2,
                                                      c->rho, c->eband32, ev_ui, c->frc,
-                                                     c->epa);
+                                                     c->epa); ++c->launches; }
     } else {
       if (single)
-        k_tb_force64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        { k_tb_force64<true><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                    c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
-                                                   c->eband, ev_ui, c->frc, c->epa);
+                                                   c->eband, ev_ui, c->frc, c->epa); ++c->launches; }
       else
-        k_tb_force64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
+        { k_tb_force64<false><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,
                                                     c->ncap, c->tbd, c->nt, L0, L1, L2, c->rho,
-                                                    c->eband, ev_ui, c->frc, c->epa);
+                                                    c->eband, ev_ui, c->frc, c->epa); ++c->launches; }
     }
   } else if (c->pot == GH_POT_LJ) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
     if (mixed)
-      k_lj_mx<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, (float)c->lj_eps,
+      { k_lj_mx<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, (float)c->lj_eps,
                                       (float)(c->lj_sigma * c->lj_sigma), rc2, L0, L1, L2,
-                                      ev_ui, c->frc, c->epa);
+                                      ev_ui, c->frc, c->epa); ++c->launches; }
     else
-      k_lj64<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
-                                     c->lj_sigma, rc2, L0, L1, L2, ev_ui, c->frc, c->epa);
+      { k_lj64<<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->nn, list, ld, n, c->ncap, c->lj_eps,
+                                     c->lj_sigma, rc2, L0, L1, L2, ev_ui, c->frc, c->epa); ++c->launches; }
   } else if (gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
 #define PLJ_LAUNCH(MX, PT)                                                                  \
-  k_plj<MX, PT><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n, c->ncap, \
-                                               c->pljd, rc2, L0, L1, L2, ev_ui, c->frc, c->epa)
+  do {                                                                                      \
+    k_plj<MX, PT><<<xgrid(nb), GH_BLOCK, 0, s>>>(c->pos, c->type, c->nn, list, ld, n,        \
+                                                 c->ncap, c->pljd, rc2, L0, L1, L2, ev_ui,  \
+                                                 c->frc, c->epa);                           \
+    ++c->launches;                                                                          \
+  } while (0)
 #define PLJ_POTS(MX)                                                                        \
   switch (c->pot) {                                                                         \
     case GH_POT_SLJ: PLJ_LAUNCH(MX, GH_POT_SLJ); break;                                     \
@@ -1038,41 +1042,41 @@ int gh_launch_forces(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 }
 
 int gh_launch_vv_a(gh_ctx *c) {
-  k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
+  { k_vv_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                     make_mass(c), c->type, c->pos, c->vel,
-                                                    c->acc, c->frc, c->xold, c->status);
+                                                    c->acc, c->frc, c->xold, c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
 
 int gh_launch_vv_ba(gh_ctx *c) {
-  k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
+  { k_vv_ba<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt,
                                                      make_mass(c), c->type, c->pos, c->vel,
-                                                     c->frc, c->xold, c->status);
+                                                     c->frc, c->xold, c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
 
 int gh_launch_vv_b(gh_ctx *c) {
-  k_vv_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c), c->type,
-                                                    c->vel, c->acc, c->frc, c->status);
+  { k_vv_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, c->dt, make_mass(c), c->type,
+                                                    c->vel, c->acc, c->frc, c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
 
 int gh_launch_ermak_a(gh_ctx *c) {
-  k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
+  { k_ermak_a<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
       c->n, c->ncap, make_ermak(c), make_mass(c), c->type, c->orig,
-      c->gauss, c->pos, c->vel, c->acc, c->xold, c->status);
+      c->gauss, c->pos, c->vel, c->acc, c->xold, c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
 
 int gh_launch_ermak_b(gh_ctx *c) {
-  k_ermak_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, make_ermak(c),
+  { k_ermak_b<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, c->ncap, make_ermak(c),
                                                        make_mass(c), c->type, c->orig,
                                                        c->gauss, c->vel, c->acc, c->frc,
-                                                       c->status);
+                                                       c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1085,10 +1089,10 @@ int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui) {
   double *part = c->red;                 // [nblk*5] (sum, error) pairs
   double *sums = c->red + 10 * c->nred;  // [5]
   double *kin = sums + 8;                // [1]
-  k_red_a<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->epa, c->vel, part);
-  k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 5, part, sums);
-  k_red_b<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->vel, sums, part);
-  k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 1, part, kin);
+  { k_red_a<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->epa, c->vel, part); ++c->launches; }
+  { k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 5, part, sums); ++c->launches; }
+  { k_red_b<<<nblk, GH_BLOCK, 0, s>>>(c->n, c->ncap, ma, c->type, c->vel, sums, part); ++c->launches; }
+  { k_red_final<<<1, GH_BLOCK, 0, s>>>(nblk, 1, part, kin); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   GH_HIP(c, hipMemcpyAsync(c->red_h, sums, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
   GH_HIP(c, hipStreamSynchronize(s));
@@ -1098,14 +1102,14 @@ int gh_launch_energies(gh_ctx *c, double *epot_ui, double *ekin_ui) {
 }
 
 int gh_launch_set_positions(gh_ctx *c, const double *x_dev) {
-  k_set_positions<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
-      c->n, c->ncap, make_wrap(c), c->orig, x_dev, c->pos, c->xold, gh_maxd2(c));
+  { k_set_positions<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(
+      c->n, c->ncap, make_wrap(c), c->orig, x_dev, c->pos, c->xold, gh_maxd2(c)); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
 
 int gh_launch_wrap(gh_ctx *c) {
-  k_wrap<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, make_wrap(c), c->pos);
+  { k_wrap<<<nblocks(c->n), GH_BLOCK, 0, c->stream>>>(c->n, make_wrap(c), c->pos); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1113,7 +1117,7 @@ int gh_launch_wrap(gh_ctx *c) {
 // Fill helper for host code (no hipMemset in code that may be captured).
 int gh_fill_int(gh_ctx *c, int *p, int v, int n) {
   if (n <= 0) return 0;
-  k_fill_int<<<nblocks(n), GH_BLOCK, 0, c->stream>>>(p, v, n);
+  { k_fill_int<<<nblocks(n), GH_BLOCK, 0, c->stream>>>(p, v, n); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }

@@ -1364,10 +1364,10 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   // grid-stride launches of at most 2048 workgroups: cheap when no rebuild is due
   const int ga = std::min((R.n + GH_TBLOCK - 1) / GH_TBLOCK, 2048);
   const int gb = std::min((R.n + GH_BINBLOCK - 1) / GH_BINBLOCK, 1024);
-  k_cell_bin<<<gb, GH_BINBLOCK, 0, s>>>(c->status, C, c->pos, c->skin, c->step);
-  k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(c->status, C);
-  k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(c->status, C, c->pos);
-  k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos, c->step);
+  { k_cell_bin<<<gb, GH_BINBLOCK, 0, s>>>(c->status, C, c->pos, c->skin, c->step); ++c->launches; }
+  { k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(c->status, C); ++c->launches; }
+  { k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(c->status, C, c->pos); ++c->launches; }
+  { k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos, c->step); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1379,7 +1379,7 @@ int gh_tile_occupancy(gh_ctx *c, int *max_halo, int *max_brick) {
   h.max_halo = 0;
   h.max_brick = 0;
   GH_HIP(c, hipMemcpyAsync(c->status, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-  k_tile_occupancy<<<c->tg.nbrick, GH_TBLOCK, 0, c->stream>>>(c->tg, c->cell_start, c->status);
+  { k_tile_occupancy<<<c->tg.nbrick, GH_TBLOCK, 0, c->stream>>>(c->tg, c->cell_start, c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   GH_HIP(c, hipMemcpyAsync(&h, c->status, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   GH_HIP(c, hipStreamSynchronize(c->stream));
@@ -1389,9 +1389,9 @@ int gh_tile_occupancy(gh_ctx *c, int *max_halo, int *max_brick) {
 }
 
 int gh_tiled_to_global(gh_ctx *c, int *dst) {
-  k_tiled_to_global<<<c->tg.nbrick, GH_TBLOCK, 0, c->stream>>>(c->tg, c->cell_start, c->nn,
+  { k_tiled_to_global<<<c->tg.nbrick, GH_TBLOCK, 0, c->stream>>>(c->tg, c->cell_start, c->nn,
                                                                c->list16, c->ncap, dst,
-                                                               c->status);
+                                                               c->status); ++c->launches; }
   GH_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1429,20 +1429,23 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       if (c->tbf_k != c->tg.fxk) GH_TRY(gh_upload_tbf(c, c->tg.fxk));
       const double e0 = c->tb_e0;
       if (single)
-        k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
+        { k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                         c->cell_start, c->nn, c->list16, ld,
                                                         c->tbf, rho1, c->eband32, e0, c->tbf0,
-                                                        sa);
+                                                        sa); ++c->launches; }
       else
-        k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
+        { k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
                                                          c->cell_start, c->nn, c->list16, ld,
                                                          c->tbf, rho1, c->eband32, e0, c->tbf0,
-                                                         sa);
+                                                         sa); ++c->launches; }
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
 #define TB_FX(SG, FU)                                                                         \
-  k_tb_force_fx<SG, FU><<<grid, GH_TBLOCK, lds2, s>>>(                                        \
-      c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap, c->tbf, \
-      c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0, sa)
+  do {                                                                                        \
+    k_tb_force_fx<SG, FU><<<grid, GH_TBLOCK, lds2, s>>>(                                      \
+        c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,      \
+        c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0, sa);                  \
+    ++c->launches;                                                                            \
+  } while (0)
       if (single) {
         if (fuse_vv) TB_FX(true, true);
         else TB_FX(true, false);
@@ -1453,32 +1456,35 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 #undef TB_FX
     } else {
       if (single)
-        k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
+        { k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, rho1, c->eband, c->eband32, sa);
+            c->nt, rho1, c->eband, c->eband32, sa); ++c->launches; }
       else
-        k_tb_rho64_tiled<false><<<grid, GH_TBLOCK, lds1, s>>>(
+        { k_tb_rho64_tiled<false><<<grid, GH_TBLOCK, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, rho1, c->eband, c->eband32, sa);
+            c->nt, rho1, c->eband, c->eband32, sa); ++c->launches; }
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
       if (single)
-        k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
+        { k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa);
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa); ++c->launches; }
       else
-        k_tb_force64_tiled<false><<<grid, GH_TBLOCK, lds2, s>>>(
+        { k_tb_force64_tiled<false><<<grid, GH_TBLOCK, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa);
+            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa); ++c->launches; }
     }
   } else if (c->pot == GH_POT_LJ || gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
     const size_t lds = gh_pair_slot_bytes(c->pot) * slots;
 #define LJ_LAUNCH(MX, PT)                                                                   \
-  k_lj_tiled<MX, PT><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,  \
-                                                  c->nn, c->list16, ld, c->ncap, c->lj_eps, \
-                                                  c->lj_sigma, rc2, ev_ui, c->frc, c->epa,  \
-                                                  c->type, fv, c->pljd, sa)
+  do {                                                                                      \
+    k_lj_tiled<MX, PT><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start, \
+                                                    c->nn, c->list16, ld, c->ncap, c->lj_eps, \
+                                                    c->lj_sigma, rc2, ev_ui, c->frc, c->epa, \
+                                                    c->type, fv, c->pljd, sa);              \
+    ++c->launches;                                                                          \
+  } while (0)
 #define LJ_POTS(MX)                                                                         \
   switch (c->pot) {                                                                         \
     case GH_POT_PLJ: LJ_LAUNCH(MX, GH_POT_PLJ); break;                                      \

@@ -231,7 +231,8 @@ int gh_lbfgs(gh_ctx *ctx, int m, double eps, int max_iter, int *iters, double *e
 int gh_get_state(gh_ctx *ctx, double *x, double *v, double *F, double *e);
 
 /* Counters: nupd = neighbour rebuilds (nupd_vlist, Neighbour.F90:238),
- * maxnn = longest list row at the last build, cap = row capacity. */
+ * maxnn = longest list row at the last build, cap = row capacity, steps =
+ * MD steps taken, launches = kernels enqueued so far. */
 typedef struct {
   int64_t nupd;
   int32_t maxnn;
@@ -239,6 +240,7 @@ typedef struct {
   int32_t ncells[3];
   int32_t list_mode; /* 0 = O(N^2) (verlet_self), 1 = cells (linkedcell_self) */
   int64_t steps;
+  int64_t launches;  /* kernels the context has enqueued (no counterpart) */
 } gh_stats;
 int gh_get_stats(gh_ctx *ctx, gh_stats *s);
 

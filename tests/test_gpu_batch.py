@@ -137,9 +137,31 @@ def test_replica_exchange_bookkeeping():
         assert np.allclose(v1[a0:a1], v0[a0:a1] * f, rtol=1e-12, atol=0)
 
 
-def test_fire_relaxations_match_oracle():
-    off, x = CL.au_fire_batch(nclusters=12, seed=8)
-    with hip.Batch(0, hip.GH_MIXED) as b:
+def _oracle_fire_all(off, x, s, workers=16):
+    """The oracle's FP64 FIRE of every system, on host threads (the C oracle
+    runs without the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(k):
+        a0, a1 = off[k], off[k + 1]
+        n = a1 - a0
+        return O.fire(x[a0:a1], np.zeros(n), s, np.full(n, CL.MASS_AU))
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, range(len(off) - 1)))
+
+
+@pytest.mark.parametrize("precision", [hip.GH_MIXED, hip.GH_FP64], ids=["mixed", "fp64"])
+def test_fire_relaxations_match_oracle(precision):
+    """64 Au clusters (13..309 atoms, fcc spheres and Mackay fragments, 0.1 A
+    perturbation): the batched FIRE (FP32 pair terms to 1e-2 eV/A, then FP64
+    to 1e-4 eV/A; or FP64 throughout) against the oracle's FP64 FIRE from the
+    same start.  Bars: every relaxation converges to a minimum (the oracle's
+    FP64 forces there below 1.01e-4 eV/A, the reported energy the oracle's to
+    1e-10); at least 95% reach the oracle's minimum, and within that basin the
+    final energies agree to 1e-8 eV/atom."""
+    off, x = CL.au_fire_batch(nclusters=64, seed=8)
+    with hip.Batch(0, precision) as b:
         b.set_systems(off, x)
         b.set_smatb(1, CL.AU["A"], CL.AU["xi"], CL.AU["p"], CL.AU["q"], CL.AU["r0"], 4.078,
                     4.078 * np.sqrt(1.5))
@@ -150,21 +172,25 @@ def test_fire_relaxations_match_oracle():
     s = O.smatb_params([(0, 0, CL.AU["A"], CL.AU["xi"], CL.AU["p"], CL.AU["q"], CL.AU["r0"],
                          4.078, 4.078 * np.sqrt(1.5))], 1)
     box = O.make_box(None)
-    same = 0
-    for k in range(len(off) - 1):
+    ro = _oracle_fire_all(off, x, s)
+    n = np.diff(off)
+    eo = np.array([q["epot"] for q in ro])
+    assert all(q["converged"] for q in ro)
+    for k in range(len(n)):
         a0, a1 = off[k], off[k + 1]
-        n = a1 - a0
         # the GPU minimum is a minimum: FP64 oracle forces there are below tol
         nn, lst, _ = O.neighbours(xf[a0:a1], box, O.smatb_rcut(s) + 1.0)
-        Fo, eo, epo, _ = O.smatb_force(xf[a0:a1], np.zeros(n), box, s, nn, lst)
+        Fo, _, epo, _ = O.smatb_force(xf[a0:a1], np.zeros(n[k]), box, s, nn, lst)
         assert np.sqrt((Fo ** 2).sum(axis=1)).max() * cm.UI_EV < 1.01e-4
         assert r["epot"][k] == pytest.approx(epo * cm.UI_EV, rel=1e-10)
-        # and the FP64 FIRE of the oracle from the same start reaches the same energy
-        ro = O.fire(x[a0:a1], np.zeros(n), s, np.full(n, CL.MASS_AU))
-        assert ro["converged"]
-        if abs(ro["epot"] - r["epot"][k]) < 1e-6 * n:
-            same += 1
-    assert same >= 0.75 * (len(off) - 1)
+    de = np.abs(r["epot"] - eo) / n  # eV/atom
+    same = de < 1e-6
+    print("FIRE %s: %d/%d reach the oracle's minimum; within it max |dE| %.2e eV/atom; "
+          "iterations mean %.0f (oracle %.0f)"
+          % ("mixed" if precision == hip.GH_MIXED else "fp64", same.sum(), len(n),
+             de[same].max(), np.mean(r["iters"]), np.mean([q["iters"] for q in ro])))
+    assert same.mean() >= 0.95
+    assert de[same].max() <= 1e-8
 
 
 def _agcu_sizes(sizes, seed=21):
