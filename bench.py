@@ -497,9 +497,16 @@ def main():
     ap.add_argument("--fire-clusters", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM counters)")
+    ap.add_argument("--lj-only", action="store_true",
+                    help="only run the Lennard-Jones legs (profiling runs of k_lj_tiled)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.lj_only:
+        from gems_amd import hip
+        print(json.dumps({"lj_4000": lj_bulk(hip, 10, 1000),
+                          "lj_1m": lj_bulk(hip, 64, 100, jitter=0.01, Tstar=1.44)}))
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "

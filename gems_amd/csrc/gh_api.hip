@@ -232,6 +232,25 @@ static int max_slots(const gh_ctx *c) {
   return (int)std::min<size_t>((GH_LDS_BYTES - stat) / per - 1, 65534);  // - the sentinel
 }
 
+// Threads per workgroup of the force passes, two candidates: one per atom of
+// the fullest brick at sizing (plus 1/16 for the atoms' motion), so that no
+// brick's atom loop runs a second round (spare wavefronts of emptier bricks
+// end at once); or one per atom of the mean brick (fewer wave slots held,
+// a few bricks run a short second round).  Rounded up to whole wavefronts,
+// at least one thread per halo cell (halo_setup), at most 1024.  The
+// autotuner times both; the fullest-brick rule is the default.
+static int force_pass_threads(const TileGeom &t, int atoms) {
+  const int nh = (t.bd[0] + 2) * (t.bd[1] + 2) * (t.bd[2] + 2);
+  const int need = std::max(nh, atoms);
+  return std::min(1024, (need + 63) / 64 * 64);
+}
+static void set_force_pass_threads(gh_ctx *c, int max_brick) {
+  TileGeom &t = c->tg;
+  t.tpb_max = force_pass_threads(t, max_brick + max_brick / 16 + 1);
+  t.tpb_avg = force_pass_threads(t, (int)((c->n + t.nbrick - 1) / t.nbrick));
+  t.tpb = c->tpb_mean ? t.tpb_avg : t.tpb_max;
+}
+
 // Brick edges (cells) for the tiled passes: the largest brick whose expected
 // halo fits the LDS slot budget with a 30% margin; ties go to the smaller
 // halo.  bd_req (gh_set_tile) overrides.
@@ -362,6 +381,7 @@ static int ensure_ready(gh_ctx *c) {
       slot_cap = roundup(slot_cap, 32);
       if (slot_cap <= max_slots(c) && !h.tile_overflow) {
         c->tg.slot_cap = slot_cap;
+        set_force_pass_threads(c, max_brick);
         maxnn = h.maxnn;
         break;
       }
@@ -652,6 +672,7 @@ int gh_set_tile(gh_ctx *c, const int *bd) {
   for (int d = 0; d < 3; ++d)
     if (bd && (bd[d] < 0 || bd[d] > 4)) return gh_fail(c, GH_EINVAL, "brick edges must be 0..4");
   for (int d = 0; d < 3; ++d) c->bd_req[d] = bd ? bd[d] : 0;
+  c->tpb_mean = 0;
   c->ready = false;
   return 0;
 }
@@ -673,7 +694,7 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
   hipEvent_t e0, e1;
   GH_HIP(c, hipEventCreate(&e0));
   GH_HIP(c, hipEventCreate(&e1));
-  int best[3] = {0, 0, 0};
+  int best[3] = {0, 0, 0}, best_mean = 0;
   double best_us = 1e30;
   for (int bz = 1; bz <= 4; ++bz)
     for (int by = bz; by <= 4; ++by)
@@ -690,23 +711,30 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
         bool same = true;  // the sizing may have shrunk the request
         for (int d = 0; d < 3; ++d) same = same && c->tg.bd[d] == b[d];
         if (!same) continue;
-        GH_TRY(gh_launch_forces(c, nullptr));
-        GH_HIP(c, hipEventRecord(e0, c->stream));
-        for (int r = 0; r < reps; ++r) GH_TRY(gh_launch_forces(c, nullptr));
-        GH_HIP(c, hipEventRecord(e1, c->stream));
-        GH_HIP(c, hipEventSynchronize(e1));
-        float ms = 0.f;
-        GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-        if (check_status(c) != 0) continue;
-        const double us = 1e3 * ms / reps;
-        if (us < best_us) {
-          best_us = us;
-          for (int d = 0; d < 3; ++d) best[d] = b[d];
+        const int cand[2] = {c->tg.tpb_max, c->tg.tpb_avg};
+        for (int m = 0; m < 2; ++m) {
+          if (m == 1 && cand[1] == cand[0]) break;
+          c->tg.tpb = cand[m];
+          GH_TRY(gh_launch_forces(c, nullptr));
+          GH_HIP(c, hipEventRecord(e0, c->stream));
+          for (int r = 0; r < reps; ++r) GH_TRY(gh_launch_forces(c, nullptr));
+          GH_HIP(c, hipEventRecord(e1, c->stream));
+          GH_HIP(c, hipEventSynchronize(e1));
+          float ms = 0.f;
+          GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+          if (check_status(c) != 0) break;
+          const double us = 1e3 * ms / reps;
+          if (us < best_us) {
+            best_us = us;
+            best_mean = m;
+            for (int d = 0; d < 3; ++d) best[d] = b[d];
+          }
         }
       }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (int d = 0; d < 3; ++d) c->bd_req[d] = best[d];
+  c->tpb_mean = best_mean;
   c->ready = false;
   c->err.clear();
   GH_TRY(ensure_ready(c));

@@ -120,6 +120,8 @@ struct TileGeom {
   int nbrick;     // bricks
   int nkey;       // nbrick * cpb
   int slot_cap;   // LDS slots per workgroup (halo atoms)
+  int tpb;        // threads per workgroup of the force passes (a multiple of 64)
+  int tpb_max, tpb_avg;  // its two candidates: the fullest brick, the mean brick
   int fxk;        // fixed-point steps per Angstrom: 2^fxk (mixed SMA-TB)
   double fxs;     // 2^fxk
   double L[3];
@@ -216,6 +218,7 @@ struct gh_ctx {
   int ncell = 1;
   TileGeom tg;                // brick grouping (cell mode)
   int bd_req[3] = {0, 0, 0};  // requested brick edge (0 = automatic)
+  int tpb_mean = 0;           // 1: force-pass workgroups sized to the mean brick (autotuned)
   int cap = 0;                // list row capacity
   int cap_req = 0;
 

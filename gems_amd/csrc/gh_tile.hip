@@ -26,7 +26,8 @@
 
 #include "gh_device.h"
 
-#define GH_TBLOCK 256
+#define GH_TBLOCK 256   // list build, occupancy and layout kernels
+#define GH_TBMAX 1024   // force passes: TileGeom::tpb threads, at most this
 
 namespace {
 using namespace ghd;
@@ -36,7 +37,7 @@ struct Halo {
   double org[3];                  // brick centre: origin of the fixed-point frame
   int delta[GH_MAXHALO];          // atom in slot s of halo cell h: s + delta[h]
   int hbase[GH_MAXHALO + 1];      // first slot of halo cell h
-  int wsum[GH_TBLOCK / 64];       // halo_setup's per-wavefront count totals
+  int wsum[GH_TBMAX / 64];        // halo_setup's per-wavefront count totals
   signed char shc[3][GH_MAXHALO]; // periodic image shift of halo cell h (box lengths)
 };
 
@@ -105,7 +106,8 @@ __device__ bool halo_setup(const TileGeom &t, int b, const int *__restrict__ cel
   if (lane == 63) H.wsum[wv] = inc;
   __syncthreads();
   int woff = 0, total = 0;
-  for (int w = 0; w < GH_TBLOCK / 64; ++w) {
+  const int nw = (int)(blockDim.x >> 6);
+  for (int w = 0; w < nw; ++w) {
     const int x = H.wsum[w];
     if (w < wv) woff += x;
     total += x;
@@ -187,12 +189,13 @@ __device__ __forceinline__ double fused_vv(const FuseVV &f, int a, const double4
 // Workgroup maximum of d2, stored at dst[b] (all threads call it).
 // Workgroup maximum of |x - x_old|^2 folded into the device-wide maximum.
 __device__ __forceinline__ void block_max_atomic(double d2, unsigned long long *maxd2) {
-  __shared__ double part[GH_TBLOCK / 64];
+  __shared__ double part[GH_TBMAX / 64];
   d2 = wave_max(d2);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d2;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < GH_TBLOCK / 64; ++w) d2 = fmax(d2, part[w]);
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 1; w < nw; ++w) d2 = fmax(d2, part[w]);
     atomicMax(maxd2, (unsigned long long)__double_as_longlong(d2));
   }
 }
@@ -803,18 +806,42 @@ __device__ __forceinline__ bool step_gate(GhStatus *st, const StepArgs &sa) {
 // ---------------------------------------------------------------------------
 // SMA-TB, mixed precision (fixed-point LDS, 2^t.fxk steps per Angstrom about
 // the brick centre; the FP32 pair forms of TbF in that unit).  Pass 1:
-// smatb_preinteraction (TB.F90:367-455); LDS per slot: {qx, qy, qz, type} as
-// one 16-B word (one address per neighbour), then the slot map.  Stores rho
-// (when asked) and d = 1/sqrt(rho) - e0 in FP32.
+// smatb_preinteraction (TB.F90:367-455); LDS per slot: qx, qy, qz in three
+// arrays, then the slot map (and the type).  Stores rho (when asked) and
+// d = 1/sqrt(rho) - e0 in FP32.
+// S > 0: the slot arrays are static LDS of S slots (S >= slot_cap + 1), so a
+// neighbour's three coordinates are one address (the slot shifted) and
+// compile-time offsets; S = 0: dynamic LDS sized to the halo.
 // ---------------------------------------------------------------------------
-template <bool SINGLE>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
+// A copy of x held in a vector register: the empty asm makes the value opaque,
+// so the compiler keeps one VGPR copy instead of re-materialising it from its
+// scalar register before every instruction that takes two scalar operands.
+__device__ __forceinline__ float in_vgpr(float x) {
+  float v;
+  asm("" : "=v"(v) : "0"(x));
+  return v;
+}
+// The one-type parameters with the addends of the FP32 pair forms in VGPRs.
+__device__ __forceinline__ TbF tbf_vgpr(const TbF &p) {
+  TbF v = p;
+  v.rce = in_vgpr(p.rce);
+  v.b2f = in_vgpr(p.b2f);
+  v.b2x4 = in_vgpr(p.b2x4);
+  v.r2m8 = in_vgpr(p.r2m8);
+  v.c0 = in_vgpr(p.c0);
+  return v;
+}
+
+template <bool SINGLE, int S>
+__global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbF *__restrict__ tbf,
     double *__restrict__ rho, float *__restrict__ ebd32, double e0, TbF p1, StepArgs sa) {
   if (step_gate(st, sa)) return;
-  extern __shared__ int qa[];
+  extern __shared__ int qa_dyn[];
+  __shared__ int qa_st[S > 0 ? 3 * S + (S + 3) / 4 * (SINGLE ? 1 : 2) : 1];
+  int *qa = S > 0 ? qa_st : qa_dyn;
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
   if (!SINGLE)
@@ -823,8 +850,9 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
-  int *qxa = qa, *qya = qa + ns, *qza = qa + 2 * ns;
-  unsigned char *scell = reinterpret_cast<unsigned char *>(qa + 3 * ns);
+  const int st3 = S ? S : ns;     // stride of the three coordinate arrays
+  int *qxa = qa, *qya = qa + st3, *qza = qa + 2 * st3;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(qa + 3 * st3);
   unsigned char *ty = scell + ns;
   slot_map(H, scell);
   const double fxs = t.fxs;
@@ -842,6 +870,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
+  const TbF pv = tbf_vgpr(p1);
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
@@ -851,7 +880,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
     auto pair = [&](int jx, int jy, int jz, int s) {
       const double dx = fxd(jx, qx), dy = fxd(jy, qy), dz = fxd(jz, qz);
       float band;
-      if (tb_rho_fx(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]], fma(dx, dx, fma(dy, dy, dz * dz)),
+      if (tb_rho_fx(SINGLE ? pv : sp[ti * GH_MAXT + ty[s]], fma(dx, dx, fma(dy, dy, dz * dz)),
                     band))
         acc += (double)band;
     };
@@ -867,16 +896,21 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho_fx(
       pair(x3, y3, z3, s3);
     });
     if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
-    ebd32[a] = (float)(1.0 / sqrt(acc) - e0);
+    double yr = __builtin_amdgcn_rsq(acc);
+    const double hr = 0.5 * acc;
+    yr = yr * fma(-hr * yr, yr, 1.5);
+    yr = yr * fma(-hr * yr, yr, 1.5);
+    ebd32[a] = (float)(yr - e0);
   }
 }
 
 // Pass 2: smatb_interact (TB.F90:457-586); LDS slot = {qx, qy, qz, d_j}
 // (one 16-B read per neighbour: here fewer instructions outweigh the bank
-// conflicts that make the density pass prefer separate arrays).  FUSED: the
+// conflicts that make the density pass prefer separate arrays), in static
+// LDS of S slots as in pass 1 (S = 0: dynamic).  FUSED: the
 // velocity Verlet update of the atoms (no forces or energies stored, no
 // repulsion summed); otherwise forces and per-atom energies are stored.
-template <bool SINGLE, bool FUSED>
+template <bool SINGLE, bool FUSED, int S>
 __device__ __forceinline__ void tb_force_fx_body(
     GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -884,7 +918,9 @@ __device__ __forceinline__ void tb_force_fx_body(
     const double *__restrict__ rho, const float *__restrict__ ebd32, double ev_ui,
     double *__restrict__ frc, double *__restrict__ epa, const FuseVV &fv, const TbF &p1,
     long long step) {
-  extern __shared__ int4 q4[];
+  extern __shared__ int4 q4_dyn[];
+  __shared__ int4 q4_st[S > 0 ? S + (S + 15) / 16 * (SINGLE ? 1 : 2) : 1];
+  int4 *q4 = S > 0 ? q4_st : q4_dyn;
   __shared__ Halo H;
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
   if (!SINGLE)
@@ -893,8 +929,8 @@ __device__ __forceinline__ void tb_force_fx_body(
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
-  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + ns);
-  unsigned char *ty = scell + ns;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(q4 + (S > 0 ? S : ns));
+  unsigned char *ty = scell + (S > 0 ? S : ns);
   slot_map(H, scell);
   const double fxs = t.fxs;
   for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
@@ -914,6 +950,7 @@ __device__ __forceinline__ void tb_force_fx_body(
   // tb_force_fx returns factor / r per unit of the pair vector, so the sums
   // over integer pair vectors come out in eV/A
   double dmax2 = 0.0;
+  const TbF pv = tbf_vgpr(p1);
   for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
@@ -924,7 +961,7 @@ __device__ __forceinline__ void tb_force_fx_body(
     auto pair = [&](const int4 &q, int s) {
       const double dx = fxd(q.x, qx), dy = fxd(q.y, qy), dz = fxd(q.z, qz);
       float sf, repul;
-      if (tb_force_fx<!FUSED>(SINGLE ? p1 : sp[ti * GH_MAXT + ty[s]],
+      if (tb_force_fx<!FUSED>(SINGLE ? pv : sp[ti * GH_MAXT + ty[s]],
                               fma(dx, dx, fma(dy, dy, dz * dz)), di + __int_as_float(q.w), sf,
                               repul)) {
         const double sfd = (double)sf;
@@ -954,15 +991,15 @@ __device__ __forceinline__ void tb_force_fx_body(
   }
   if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
-template <bool SINGLE, bool FUSED>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
+template <bool SINGLE, bool FUSED, int S>
+__global__ __launch_bounds__(GH_TBMAX) void k_tb_force_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbF *__restrict__ tbf,
     const double *__restrict__ rho, const float *__restrict__ ebd32, double ev_ui,
     double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, TbF p1, StepArgs sa) {
   if (!halted(st))
-    tb_force_fx_body<SINGLE, FUSED>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbf, rho,
+    tb_force_fx_body<SINGLE, FUSED, S>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbf, rho,
                                     ebd32, ev_ui, frc, epa, fv, p1, sa.step);
 }
 
@@ -971,7 +1008,7 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tb_force_fx(
 // with the image shift applied).
 // ---------------------------------------------------------------------------
 template <bool SINGLE>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_rho64_tiled(
+__global__ __launch_bounds__(GH_TBMAX) void k_tb_rho64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, const TbDev *__restrict__ tbd, int nt,
@@ -1097,7 +1134,7 @@ __device__ __forceinline__ void tb_force64_body(
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
 template <bool SINGLE>
-__global__ __launch_bounds__(GH_TBLOCK) void k_tb_force64_tiled(
+__global__ __launch_bounds__(GH_TBMAX) void k_tb_force64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
     const unsigned short *__restrict__ list16, int ld, int ncap, const TbDev *__restrict__ tbd,
@@ -1214,7 +1251,7 @@ __device__ __forceinline__ void lj_tiled_body(
 }
 // One pass per step: gated (and publishing) as a first pass.
 template <bool MIXED, int POT>
-__global__ __launch_bounds__(GH_TBLOCK) void k_lj_tiled(
+__global__ __launch_bounds__(GH_TBMAX) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
     double eps, double sigma, double rc2, double ev_ui, double *__restrict__ frc,
@@ -1428,23 +1465,52 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     if (mixed) {
       if (c->tbf_k != c->tg.fxk) GH_TRY(gh_upload_tbf(c, c->tg.fxk));
       const double e0 = c->tb_e0;
-      if (single)
-        { k_tb_rho_fx<true><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
-                                                        c->cell_start, c->nn, c->list16, ld,
-                                                        c->tbf, rho1, c->eband32, e0, c->tbf0,
-                                                        sa); ++c->launches; }
-      else
-        { k_tb_rho_fx<false><<<grid, GH_TBLOCK, lds1, s>>>(c->status, c->tg, c->pos, c->type,
-                                                         c->cell_start, c->nn, c->list16, ld,
-                                                         c->tbf, rho1, c->eband32, e0, c->tbf0,
-                                                         sa); ++c->launches; }
+#define RHO_FX(SG, SS)                                                                      \
+  do {                                                                                      \
+    const size_t lds = SS ? 0 : lds1;                                                        \
+    k_tb_rho_fx<SG, SS><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->type,      \
+                                                     c->cell_start, c->nn, c->list16, ld,    \
+                                                     c->tbf, rho1, c->eband32, e0, c->tbf0,  \
+                                                     sa);                                    \
+    ++c->launches;                                                                          \
+  } while (0)
+#define RHO_S(SG)                                                                           \
+  do {                                                                                      \
+    const int need = c->tg.slot_cap + 1;                                                    \
+    if (need <= 768) RHO_FX(SG, 768);                                                       \
+    else if (need <= 1024) RHO_FX(SG, 1024);                                                \
+    else if (need <= 1280) RHO_FX(SG, 1280);                                                \
+    else if (need <= 1536) RHO_FX(SG, 1536);                                                \
+    else if (need <= 1792) RHO_FX(SG, 1792);                                                \
+    else if (need <= 2048) RHO_FX(SG, 2048);                                                \
+    else if (need <= 2560) RHO_FX(SG, 2560);                                                \
+    else if (need <= 3072) RHO_FX(SG, 3072);                                                \
+    else RHO_FX(SG, 0);                                                                     \
+  } while (0)
+      if (single) RHO_S(true);
+      else RHO_S(false);
+#undef RHO_S
+#undef RHO_FX
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
-#define TB_FX(SG, FU)                                                                         \
-  do {                                                                                        \
-    k_tb_force_fx<SG, FU><<<grid, GH_TBLOCK, lds2, s>>>(                                      \
-        c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,      \
-        c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0, sa);                  \
-    ++c->launches;                                                                            \
+#define TB_FX3(SG, FU, SS)                                                                  \
+  do {                                                                                      \
+    k_tb_force_fx<SG, FU, SS><<<grid, c->tg.tpb, SS ? 0 : lds2, s>>>(                        \
+        c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,    \
+        c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0, sa);                \
+    ++c->launches;                                                                          \
+  } while (0)
+#define TB_FX(SG, FU)                                                                       \
+  do {                                                                                      \
+    const int need = c->tg.slot_cap + 1;                                                    \
+    if (need <= 768) TB_FX3(SG, FU, 768);                                                   \
+    else if (need <= 1024) TB_FX3(SG, FU, 1024);                                            \
+    else if (need <= 1280) TB_FX3(SG, FU, 1280);                                            \
+    else if (need <= 1536) TB_FX3(SG, FU, 1536);                                            \
+    else if (need <= 1792) TB_FX3(SG, FU, 1792);                                            \
+    else if (need <= 2048) TB_FX3(SG, FU, 2048);                                            \
+    else if (need <= 2560) TB_FX3(SG, FU, 2560);                                            \
+    else if (need <= 3072) TB_FX3(SG, FU, 3072);                                            \
+    else TB_FX3(SG, FU, 0);                                                                 \
   } while (0)
       if (single) {
         if (fuse_vv) TB_FX(true, true);
@@ -1454,22 +1520,23 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
         else TB_FX(false, false);
       }
 #undef TB_FX
+#undef TB_FX3
     } else {
       if (single)
-        { k_tb_rho64_tiled<true><<<grid, GH_TBLOCK, lds1, s>>>(
+        { k_tb_rho64_tiled<true><<<grid, c->tg.tpb, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
             c->nt, rho1, c->eband, c->eband32, sa); ++c->launches; }
       else
-        { k_tb_rho64_tiled<false><<<grid, GH_TBLOCK, lds1, s>>>(
+        { k_tb_rho64_tiled<false><<<grid, c->tg.tpb, lds1, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
             c->nt, rho1, c->eband, c->eband32, sa); ++c->launches; }
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
       if (single)
-        { k_tb_force64_tiled<true><<<grid, GH_TBLOCK, lds2, s>>>(
+        { k_tb_force64_tiled<true><<<grid, c->tg.tpb, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
             c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa); ++c->launches; }
       else
-        { k_tb_force64_tiled<false><<<grid, GH_TBLOCK, lds2, s>>>(
+        { k_tb_force64_tiled<false><<<grid, c->tg.tpb, lds2, s>>>(
             c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
             c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa); ++c->launches; }
     }
@@ -1479,7 +1546,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     const size_t lds = gh_pair_slot_bytes(c->pot) * slots;
 #define LJ_LAUNCH(MX, PT)                                                                   \
   do {                                                                                      \
-    k_lj_tiled<MX, PT><<<grid, GH_TBLOCK, lds, s>>>(c->status, c->tg, c->pos, c->cell_start, \
+    k_lj_tiled<MX, PT><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start, \
                                                     c->nn, c->list16, ld, c->ncap, c->lj_eps, \
                                                     c->lj_sigma, rc2, ev_ui, c->frc, c->epa, \
                                                     c->type, fv, c->pljd, sa);              \
