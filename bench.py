@@ -66,11 +66,11 @@ def make_ctx(hip, precision, x, L, skin, lcg_seed=4242, T=300.0, dt=0.002):
     return c
 
 
-def profiled(field, kernel="k_tb_force_fx"):
+def profiled(field, kernel="k_tb_force_fx", summary="k_tb_force_fx_au1m_counters.md"):
     """A per-launch figure of `kernel` from the committed rocprofv3 summary:
     traffic_bytes_per_launch (FETCH_SIZE and WRITE_SIZE, calibrated against a
     1 GiB device copy) or valu_insts_per_launch (SQ_INSTS_VALU)."""
-    path = os.path.join(REPO, "profiles", "k_tb_force_fx_au1m_counters.md")
+    path = os.path.join(REPO, "profiles", summary)
     try:
         for ln in open(path):
             if ln.startswith("%s %s:" % (field, kernel)):
@@ -355,10 +355,21 @@ def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72):
         shape, _ = c.autotune(reps=3)
         F[name] = c.compute()[0]
         dt, _, nupd, launches = timed_run(c, steps, warmup)
+        _, tm, _, _ = timed_run(c, steps, 0, timing=True)
         c.close()
         out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
                      "rebuilds": nupd, "launches_per_step": launches / steps,
+                     "force_pass_us": tm["force"][0] / max(tm["force"][1], 1) * 1e3,
                      "brick_cells": shape}
+    # the fused pass moves x 24 + v 48 + the next x 24 = 96 algorithmic B/atom
+    t_us = out["mixed"]["force_pass_us"]
+    ach = 96.0 * len(x) / (t_us * 1e-6) / 1e9
+    traffic, src = profiled("traffic_bytes_per_launch", "k_lj_tiled",
+                            "k_lj_tiled_lj1m_counters.md") if len(x) > 10 ** 6 else (None, None)
+    out["roofline"] = {"kernel": "k_lj_tiled (mixed, velocity Verlet fused)", "bound": "hbm",
+                       "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": src,
+                       "bytes_per_atom": 96, "avg_launch_us": t_us}
     if jitter > 0:  # on the perfect lattice the forces vanish and the ratio means nothing
         rms = float(np.sqrt(np.mean(F["fp64"] ** 2)))
         out["force_err_mixed_vs_fp64_rms_over_rms"] = float(
