@@ -19,7 +19,10 @@ Workloads: the scripts that wrote the fixtures with the reference binary
 and agcu201_ermak (`evolve ermak 300|600 1`, 100 steps; the Ag-Cu cluster in
 no box); nve_plj_script for examples/NVE/v_verlet.gms (100 He + 100 Ne from
 `fill`, three plj groups, two of them self and one `with` group, 5000
-steps); drift_main's script for the 32,000-atom Au NVE trace (10^4 steps).
+steps) and the NVT example (the same gas under `evolve ermak 300 10`, 10^4
+steps); examples/SMA-TB/bulk.gms and md.gms restated (GEMS builds the 4000-atom
+Ag crystal itself; md.gms runs GEMS's `lbfgs` first); drift_main's script for
+the 32,000-atom Au NVE trace (10^4 steps).
 
 Bars: the energy columns within 1e-9 relative of the reference's 16-digit
 columns (Ermak: of each column's largest entry) and final positions within
@@ -181,6 +184,98 @@ def test_gems_nve_example_three_plj_groups(precision):
         assert np.all(np.abs(E[:18, 2] - kat[:18, 2]) <= 1e-6 * np.abs(kat[:18, 2]))
         assert np.all(np.abs(E[:18, 0] - kat[:18, 0]) <= 1e-5)
     assert abs(np.median(E[-10:, 2]) - kat[0, 2]) <= 2e-5 * abs(kat[0, 2])
+
+
+def _sma_tb_example(name, wd):
+    """The reference's examples/SMA-TB/{bulk,md}.gms restated: the same commands
+    with the script's expressions evaluated into literals, the parameter file
+    and Ag13 start (tests/golden/baletto_2003.prm, ag13_coords.xyz) beside
+    them, the energy file renamed E.dat."""
+    shutil.copy(os.path.join(cm.GOLDEN, "baletto_2003.prm"), wd)
+    head = "dimension 3\nprng lcg\nprng seed 123456\n"
+    out = "outfile :e name E.dat\noutfile :e cols energy 1\noutfile :e each %d\n"
+    if name == "bulk":
+        fact = 2.89 / np.sqrt(0.5)
+        s = head + ">< atom 0 0 0\n+< atom 0.5 0.5 0\n+< atom 0 0.5 0.5\n+< atom 0.5 0 0.5\n"
+        s += "+< reply 1 0 0 10\n+< reply 0 1 0 10\n+< reply 0 0 1 10\n"
+        s += "sys add Ag\nbox size 12 12 12\n> sys\nset move 1 1 1\n"
+        s += "set expand %s %s %s\nbox expand %s %s %s\n" % ((mg.fmt(fact),) * 6)
+        s += "group 1 add\ninteract 1 under tb read baletto_2003.prm\n" + out % 20
+        s += "evolve ermak 1500 1.\ndinamica 220\n"
+        return s
+    shutil.copy(os.path.join(cm.GOLDEN, "ag13_coords.xyz"), os.path.join(wd, "coords.xyz"))
+    s = head + ">< read coords.xyz\nsys add Ag\n> element Ag\ngroup 1 add\ntime step 0.001d0\n"
+    s += "interact 1 under tb read baletto_2003.prm\n" + out % 100
+    s += "lbfgs\n> sys\nevolve ermak 300 1\ndinamica 1000\n"
+    return s
+
+
+@pytest.mark.parametrize("name", ["bulk", "md"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gems_sma_tb_examples_known_answer(name, precision):
+    """examples/SMA-TB/bulk.gms (4000 Ag atoms built by GEMS's own `atom`,
+    `reply`, `expand` commands, Ermak 1500 K, 220 steps) and md.gms (Ag13:
+    GEMS's own `lbfgs` with device forces, then Ermak 300 K, 1000 steps)
+    through GEMS's binary with the loop on the device, against the
+    reference's ref/Energy.bulk.dat and Energy.md.dat (7 digits): FP64 every
+    entry within half a unit of the 7th digit (measured: every digit equal);
+    mixed bulk within 2e-6 relative (measured 9e-7); mixed md within 1e-3 of
+    the thermal scale (the largest ekin; measured 2.9e-4): the 13 atoms under
+    light friction (1/ps, 1 fs steps) amplify the mixed form's 1e-6 force
+    differences from 2e-5 eV at step 100 to 1e-4 eV at step 1000, where the
+    FP64 form keeps every printed digit."""
+    ref = np.loadtxt(cm.GOLDEN + "/ref_Energy.%s.dat" % name)
+    wd = tempfile.mkdtemp(prefix="gems_hip_")
+    try:
+        print("gems_hip run: examples/SMA-TB", name, precision, wd, flush=True)
+        log, _ = _run(wd, _sma_tb_example(name, wd), precision)
+        E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3)
+    finally:
+        shutil.rmtree(wd)
+    assert E.shape == ref.shape
+    rel = np.max(np.abs(E - ref) / np.abs(ref))
+    therm = np.max(np.abs(E - ref)) / np.abs(ref[:, 1]).max()
+    print("%s %s: max rel diff to the 7-digit file %.2e; max |dE| / max ekin %.2e"
+          % (name, precision, rel, therm))
+    if precision == "fp64":
+        assert rel <= 5.01e-7
+    elif name == "bulk":
+        assert rel <= 2e-6
+    else:
+        assert therm <= 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gems_nvt_ermak_example(precision):
+    """examples/NVT/ermak.gms through GEMS: the NVE example's 200-atom gas
+    (three plj groups in one device context) under `evolve ermak 300 10` for
+    10^4 steps, energy and temperature every 100 (100 device chunks, the lcg
+    stream handed to the device and back at each), against the reference's
+    ref/Energy.ermak.dat: FP64 every entry within 0.6 of the file's last
+    digit, mixed every entry within 2e-6 of the thermal scale (as the direct
+    test, test_gpu_plj::test_nvt_ermak_example_known_answer)."""
+    kat = np.loadtxt(cm.GOLDEN + "/ref_Energy.ermak.dat")
+    wd = tempfile.mkdtemp(prefix="gems_hip_")
+    try:
+        s, _, _ = mg.nve_plj_script(None, 10000, "outfile :e name E.dat\n"
+                                                 "outfile :e cols energy 1 temp 1\n"
+                                                 "outfile :e each 100\n")
+        s = s.replace("evolve v_verlet\n", "evolve ermak 300 10\n")
+        print("gems_hip run: examples/NVT/ermak.gms", precision, wd, flush=True)
+        _run(wd, s, precision)
+        E = np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 4)
+    finally:
+        shutil.rmtree(wd)
+    assert E.shape == kat.shape
+    u = cm.last_digit_units(E, kat)
+    scale = np.array([np.abs(kat[:, 1]).max()] * 3 + [np.abs(kat[:, 3]).max()])
+    dev = np.abs(E - kat).max(axis=0) / scale
+    print("nvt ermak via GEMS %s: max %.2f units of the last digit; per column max |dE| / "
+          "thermal scale %s" % (precision, u.max(), dev))
+    if precision == "fp64":
+        assert np.all(u <= 0.6)
+    else:
+        assert np.all(dev <= 2e-6)
 
 
 def au32k_script(wd, nsteps=10000, each=100):
