@@ -274,6 +274,12 @@ def gems_driven(nx=20, nsteps=2000, each=100):
             out[prec] = {"atom_steps_per_s": float(m.group(2)), "loop_s": float(m.group(1)),
                          "process_s": wall,
                          "rows": len(np.loadtxt(os.path.join(wd, "E.dat")).reshape(-1, 3))}
+            t = re.search(r"device steps ([0-9.]+) s, state copies ([0-9.]+) s, state into the "
+                          r"atoms ([0-9.]+) s, GEMS output ([0-9.]+) s", log)
+            if t:
+                out[prec]["loop_parts_s"] = {k: float(v) for k, v in zip(
+                    ("device_steps", "state_copies", "state_into_atoms", "gems_output"),
+                    t.groups())}
     return out
 
 

@@ -34,7 +34,7 @@ use gems_hip
 use gems_plugin_hip, only: gems_hip_on, hip_system_ctx
 use gems_constants, only: dp, dm
 use gems_program_types, only: dm_steps, time, dt, box, nlocal, ghost
-use gems_atoms, only: atom_ap, atom_dclist
+use gems_atoms, only: atom, atom_ap, atom_dclist
 use gems_neighbour, only: intergroup, igr_vop, pbcghost_move, nupd_vlist
 use gems_integration, only: its, its_hip_kind
 use gems_random, only: lcg_hip_state
@@ -61,6 +61,7 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
   character(len=160) :: msg
   type(atom_ap), allocatable :: ap(:)
   type(atom_dclist), pointer :: la
+  class(atom), pointer :: o
   class(intergroup), pointer :: ig
   type(outfile), pointer :: of
   type(c_ptr) :: ctx
@@ -70,7 +71,8 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
   integer(c_int) :: rc, lcg(3)
   integer(8) :: m0, m, s, nxt, se, p10, k, kdone, s0, nupd0
   integer :: i, j, kind, nat, l, ist, tries, ngr
-  integer(8) :: c0, c1, crate
+  integer(8) :: c0, c1, crate, ca, cb, tdev, tcopy, tback, tgems
+  real(dp) :: ibox(3)
   logical :: periodic, ok
 
   done = .false.
@@ -155,6 +157,11 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
 
   ! ---- the loop, in chunks between the steps GEMS looks at
   call system_clock(c0, crate)
+  tdev = 0
+  tcopy = 0
+  tback = 0
+  tgems = 0
+  ibox = 1.0_dp / box(1:3)
   m0 = nint(dm_steps, 8)
   m = m0
   do while (m < m0 + steps)
@@ -182,6 +189,7 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
     k = nxt - m
     ! a stop for a list or halo overflow re-sizes; the next call completes the
     ! stopped step and goes on
+    call system_clock(ca)
     call check(gh_get_stats(ctx, st))
     s0 = st%steps
     kdone = 0
@@ -194,19 +202,23 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
       tries = tries + 1
       if (rc /= -4 .or. tries > 20) call check(rc)
     enddo
+    call system_clock(cb)
+    tdev = tdev + (cb - ca)
     ! ---- the state where the loop looks at it
     call check(gh_get_state(ctx, x, vel, F, e))
+    call system_clock(ca)
+    tcopy = tcopy + (ca - cb)
     tepot = 0.0_dp
     do i = 1, nat
+      o => ap(i)%o
       do j = 1, 3
-        if (ap(i)%o%pbc(j)) ap(i)%o%boxcr(j) = ap(i)%o%boxcr(j) + &
-            nint((ap(i)%o%pos(j) - x(j, i)) / box(j))
+        if (o%pbc(j)) o%boxcr(j) = o%boxcr(j) + nint((o%pos(j) - x(j, i)) * ibox(j))
+        o%pos(j) = x(j, i)
+        o%vel(j) = vel(j, i)
+        o%force(j) = F(j, i)
+        o%acel(j) = F(j, i) * o%one_mass
       enddo
-      ap(i)%o%pos(1:3) = x(:, i)
-      ap(i)%o%vel(1:3) = vel(:, i)
-      ap(i)%o%force(1:3) = F(:, i)
-      ap(i)%o%acel(1:3) = F(:, i) * ap(i)%o%one_mass
-      ap(i)%o%epot = e(i)
+      o%epot = e(i)
       tepot = tepot + e(i)
     enddo
     cepot = tepot
@@ -225,6 +237,9 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
     call epot_changed()
     nupd_vlist = nupd_vlist + int(st%nupd - nupd0)
     nupd0 = st%nupd
+    call system_clock(cb)
+    tback = tback + (cb - ca)
+    ca = cb
     do s = m + 1, nxt
       time = time + dt
     enddo
@@ -235,11 +250,18 @@ logical function hip_dinamic(steps, b_out, b_time) result(done)
       if (mod(m, int(max(1, chpeach), 8)) == 0) call write_chp(int(m - m0), steps)
     endif
     if (b_time) call timer_dump(int(m - m0), time, nupd_vlist)
+    call system_clock(cb)
+    tgems = tgems + (cb - ca)
   enddo
   call system_clock(c1)
   write(msg, '(a,f0.3,a,es10.3,a)') 'dinamica done in ', real(c1 - c0, dp) / real(crate, dp), &
       ' s (', real(steps, dp) * nat * real(crate, dp) / real(max(1_8, c1 - c0), dp), &
       ' atom-steps/s with the state written back where GEMS looks at it)'
+  call wlog('HIP', trim(msg))
+  write(msg, '(a,f0.3,a,f0.3,a,f0.3,a,f0.3,a)') 'dinamica time: device steps ', &
+      real(tdev, dp) / real(crate, dp), ' s, state copies ', real(tcopy, dp) / real(crate, dp), &
+      ' s, state into the atoms ', real(tback, dp) / real(crate, dp), ' s, GEMS output ', &
+      real(tgems, dp) / real(crate, dp), ' s'
   call wlog('HIP', trim(msg))
   done = .true.
 
