@@ -616,6 +616,10 @@ int gh_create(gh_ctx **out, int device, int precision) {
   if (!c) return GH_EINVAL;
   c->device = device;
   c->precision = precision;
+  // the per-thread list build alone (diagnostics: it writes the same rows as
+  // the cooperative build, which is used wherever its LDS fits)
+  const char *lb = getenv("GEMS_HIP_LIST_BUILD");
+  c->list_rows = lb && strcmp(lb, "rows") == 0 ? 1 : 0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     delete c;
@@ -1623,6 +1627,7 @@ int gh_get_stats(gh_ctx *c, gh_stats *s) {
   s->list_mode = c->list_mode;
   s->steps = c->step;
   s->launches = c->launches;
+  s->list_build = c->list_build;
   return 0;
 }
 

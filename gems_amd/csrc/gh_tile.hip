@@ -237,6 +237,9 @@ struct BuildArgs {
   float clsf[3];     // the same in FP32
   float clsm;        // within clsm of a bound the FP64 r^2 decides the class
   int cap, ld, ncap, lds_slots, lds_rows;
+  // cooperative build (k_build_coop): chunks per brick, staged slots,
+  // candidate-list entries and row-buffer entries per wavefront in its LDS
+  int chunks, nsl, ncand, rstride;
   int *nn;
   unsigned short *list16;
   double *xold;
@@ -750,6 +753,298 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(GhStatus *st,
   const int b = xcd_block();
   if (b >= R.t.nbrick) return;
   build_brick(st, R.t, pos, R.cellof, R.start, C->A, dynf, H, b, step);
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative list build: a wavefront builds one atom's row at a time, its 64
+// lanes testing 64 consecutive candidates per step.  The candidates of a
+// brick cell (the slots of its nine runs, in the (dz, dy) order of the
+// stencil, slots ascending within a run) are laid out once per workgroup as
+// one LDS list, so a step reads 64 consecutive list entries and the halo
+// positions they name (float4, brick-local FP32) without bank conflicts.
+// Kept candidates go to the wavefront's row buffer in candidate order (ballot
+// and lane prefix count); a second sweep makes the exact tests and tags each
+// entry with its class, a third writes every entry to its place in class
+// order.  The rows are the ones build_rows_lds writes: same prefilter, same
+// exact tests, same order.  A workgroup takes a range of the brick's cells
+// (several chunks per brick where bricks are fewer than the CUs), in batches
+// whose candidate lists fit its LDS (a single cell always fits: its lists
+// hold at most the halo's slots).
+// ---------------------------------------------------------------------------
+#define GH_CBLOCK 1024  // workgroup of the cooperative build: 16 wavefronts
+
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Halo cell of brick cell X (0 <= X < cpb).
+__device__ __forceinline__ int halo_cell(const TileGeom &t, const Halo &H, int X) {
+  const int lx = X % t.bd[0], ly = (X / t.bd[0]) % t.bd[1], lz = X / (t.bd[0] * t.bd[1]);
+  return (lx + 1) + H.hd0 * ((ly + 1) + H.hd1 * (lz + 1));
+}
+
+__global__ __launch_bounds__(GH_CBLOCK) void k_build_coop(GhStatus *st,
+                                                          const ChainArgs *__restrict__ C,
+                                                          const double4 *__restrict__ pos,
+                                                          long long step) {
+  if (!st->rebuild) return;
+  extern __shared__ float4 cl4[];
+  __shared__ Halo H;
+  __shared__ int coff[65];  // candidate-list offsets of the batch's cells (cpb <= 64)
+  __shared__ int cst[65];   // their first atoms (cst[nfit]: one past the last)
+  __shared__ int bend;      // the batch's end cell
+  const CellArgs &R = C->R;
+  const BuildArgs &A = C->A;
+  const TileGeom &t = R.t;
+  const int K = A.chunks;
+  const int lin = xcd_block();
+  const int b = lin / K, kc = lin - b * K;
+  if (b >= t.nbrick) return;
+  if (!halo_setup(t, b, R.start, H, st, true, step)) return;
+  float4 *xyz = cl4;
+  unsigned short *cand = reinterpret_cast<unsigned short *>(cl4 + A.nsl);
+  unsigned short *rows = cand + A.ncand;
+  unsigned char *scell = reinterpret_cast<unsigned char *>(rows + (GH_CBLOCK / 64) * A.rstride);
+  slot_map(H, scell);
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const double4 p = pos[s + H.delta[h]];
+    xyz[s] = make_float4((float)(p.x - (H.org[0] - shift(t, H, 0, h))),
+                         (float)(p.y - (H.org[1] - shift(t, H, 1, h))),
+                         (float)(p.z - (H.org[2] - shift(t, H, 2, h))), __int_as_float(s));
+  }
+  // the padding slot of the candidate lists: beyond every cut, never kept
+  const int sent = A.nsl - 1;
+  if (threadIdx.x == 0) xyz[sent] = make_float4(1e30f, 1e30f, 1e30f, __int_as_float(-1));
+  // wavefront-uniform values go through readfirstlane, so that the loops
+  // over a wavefront's atoms and candidates run on scalar registers
+  const int lane = threadIdx.x & 63, nw = (int)(blockDim.x >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  unsigned short *rw = rows + wv * A.rstride;
+  const int cap = A.cap, pad = t.slot_cap;
+  const float hi2f = A.hi2f;
+  const size_t qstride = (size_t)A.ld << 2;
+  const int X1 = (kc + 1) * t.cpb / K;
+  int mmax = 0;
+  for (int xb = kc * t.cpb / K; xb < X1;) {
+    __syncthreads();  // the staging, or the previous batch, is complete
+    if (wv == 0) {
+      // candidate counts of cells xb.. and the cells whose lists fit
+      const int X = xb + lane;
+      int T = 0;
+      if (X < X1) {
+        const int ha = halo_cell(t, H, X);
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int h = ha + H.hd0 * (dy + H.hd1 * dz);
+            T += H.hbase[h + 2] - H.hbase[h - 1];
+          }
+      }
+      T = (T + 127) & ~127;  // padded to whole iterations of the candidate loop
+      int inc = T;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+      }
+      const int nfit = max(__popcll(__ballot(X < X1 && inc <= A.ncand)), 1);
+      if (lane < nfit) coff[lane + 1] = inc;
+      if (lane <= nfit) cst[lane] = R.start[H.key0 + xb + lane];  // the cells' first atoms
+      if (lane == 0) {
+        coff[0] = 0;
+        bend = xb + nfit;
+      }
+    }
+    __syncthreads();
+    const int xe = __builtin_amdgcn_readfirstlane(bend);
+    for (int i = wv; i < xe - xb; i += nw) {  // one wavefront per cell
+      const int ha = halo_cell(t, H, xb + i);
+      int off = coff[i];
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int h = ha + H.hd0 * (dy + H.hd1 * dz);
+          const int s0 = H.hbase[h - 1], len = H.hbase[h + 2] - s0;
+          for (int j = lane; j < len; j += 64)
+            if (off + j < A.ncand) cand[off + j] = (unsigned short)(s0 + j);
+          off += len;
+        }
+      for (int j = off + lane; j < coff[i + 1]; j += 64)
+        if (j < A.ncand) cand[j] = (unsigned short)sent;
+    }
+    __syncthreads();
+    // the batch's atoms (contiguous: sorted by cell) split among the wavefronts
+    const int aa0 = __builtin_amdgcn_readfirstlane(cst[0]);
+    const int aa1 = __builtin_amdgcn_readfirstlane(cst[xe - xb]);
+    const int per = (aa1 - aa0 + nw - 1) / nw;
+    const int ab = aa0 + wv * per, ae = min(aa1, ab + per);
+    int i = 0, dl = 0, cb = 0, tot = 0;  // the cell of atom a (batch index) and its lists
+    int cnext = ab;                      // first atom past cell i
+    for (int a = ab; a < ae; ++a) {
+      if (a >= cnext) {
+        while (__builtin_amdgcn_readfirstlane(cst[i + 1]) <= a) ++i;
+        cnext = __builtin_amdgcn_readfirstlane(cst[i + 1]);
+        dl = __builtin_amdgcn_readfirstlane(H.delta[halo_cell(t, H, xb + i)]);
+        cb = __builtin_amdgcn_readfirstlane(coff[i]);
+        tot = __builtin_amdgcn_readfirstlane(coff[i + 1]) - cb;
+      }
+      const int own = a - dl;
+      const float4 pi = xyz[own];
+      // sweep 1: FP32 prefilter of every candidate, kept ones in order;
+      // two steps of 64 per iteration, so their LDS reads overlap
+      // (lists padded with the far slot: no bounds test; the wave masks of
+      // the compares are the ballots)
+      int m = 0;
+      const unsigned short *cl = cand + cb + lane;
+      for (int c0 = 0; c0 < tot; c0 += 128) {
+        const int sa = cl[c0], sc = cl[c0 + 64];
+        const float4 pa4 = xyz[sa], pc4 = xyz[sc];
+        float ux = pa4.x - pi.x, uy = pa4.y - pi.y, uz = pa4.z - pi.z;
+        const float r2a = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+        ux = pc4.x - pi.x, uy = pc4.y - pi.y, uz = pc4.z - pi.z;
+        const float r2c = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+        // (w holds the slot: the own test reads it, so each position is one
+        // 16-byte read, conflict-free over consecutive slots)
+        const unsigned long long ma =
+            __builtin_amdgcn_fcmpf(r2a, hi2f, 5 /* ordered <= */) &
+            __builtin_amdgcn_uicmp(__float_as_uint(pa4.w), (unsigned)own, 33 /* != */);
+        const unsigned long long mc = __builtin_amdgcn_fcmpf(r2c, hi2f, 5) &
+                                      __builtin_amdgcn_uicmp(__float_as_uint(pc4.w), (unsigned)own, 33);
+        const int da = m + lane_rank(ma);
+        if (__builtin_amdgcn_inverse_ballot_w64(ma) && da < cap) rw[da] = (unsigned short)sa;
+        m += __popcll(ma);
+        const int dc = m + lane_rank(mc);
+        if (__builtin_amdgcn_inverse_ballot_w64(mc) && dc < cap) rw[dc] = (unsigned short)sc;
+        m += __popcll(mc);
+      }
+      __builtin_amdgcn_wave_barrier();
+      // entry k after the exact test near the cut (the reference's rounding
+      // decides; a failure gives 0xFFFF), tagged with its class in bits 14-15
+      auto classify = [&](int k) -> int {
+        const int s = rw[k];
+        const float4 pj = xyz[s];
+        const float vx = pj.x - pi.x, vy = pj.y - pi.y, vz = pj.z - pi.z;
+        const float r2f = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+        int cls;
+        if (r2f >= A.lo2f) {
+          const double rd = pair_rd(H, t, pos, pos[a], s, scell[s]);
+          if (!(rd < A.rl2)) return 0xFFFF;
+          cls = row_class(rd, A);
+        } else {
+          cls = row_class_f(r2f, A);
+          if (cls < 0) cls = row_class(pair_rd(H, t, pos, pos[a], s, scell[s]), A);
+        }
+        return s | (cls << 14);
+      };
+      unsigned short *row = A.list16 + ((size_t)a << 2);
+      auto put = [&](int e, int d) {
+        if (e != 0xFFFF) row[(size_t)(d >> 2) * qstride + (d & 3)] = (unsigned short)(e & 0x3FFF);
+      };
+      const int mk = min(m, cap);
+      int mw;
+      if (mk <= 64) {
+        // one chunk: classify, count by class (wave masks of the class
+        // bits), place
+        const int e = lane < mk ? classify(lane) : 0xFFFF;
+        const unsigned long long ok = __builtin_amdgcn_uicmp((unsigned)e, 0xFFFFu, 33);
+        const unsigned long long hb = ok & __builtin_amdgcn_uicmp((unsigned)e & 0x8000u, 0u, 33);
+        const unsigned long long lb = ok & __builtin_amdgcn_uicmp((unsigned)e & 0x4000u, 0u, 33);
+        const unsigned long long q0 = ok & ~(hb | lb), q1 = lb & ~hb, q2 = hb & ~lb, q3 = hb & lb;
+        const int n0 = __popcll(q0), n1 = __popcll(q1), n2 = __popcll(q2);
+        mw = __popcll(ok);
+        const int cls = e >> 14;
+        const int d = cls == 0   ? lane_rank(q0)
+                      : cls == 1 ? n0 + lane_rank(q1)
+                      : cls == 2 ? n0 + n1 + lane_rank(q2)
+                                 : n0 + n1 + n2 + lane_rank(q3);
+        put(e, d);
+      } else if (mk <= 128) {
+        // two chunks held in registers: classify, count, place
+        const int e0 = classify(lane);
+        const int e1 = lane + 64 < mk ? classify(lane + 64) : 0xFFFF;
+        const int c0 = e0 >> 14, c1 = e1 >> 14;
+        const bool o0 = e0 != 0xFFFF, o1 = e1 != 0xFFFF;
+        unsigned long long b0[4], b1[4];
+        b0[0] = __ballot(o0 && c0 == 0);
+        b0[1] = __ballot(o0 && c0 == 1);
+        b0[2] = __ballot(o0 && c0 == 2);
+        b0[3] = __ballot(o0 && c0 == 3);
+        b1[0] = __ballot(o1 && c1 == 0);
+        b1[1] = __ballot(o1 && c1 == 1);
+        b1[2] = __ballot(o1 && c1 == 2);
+        b1[3] = __ballot(o1 && c1 == 3);
+        int base[4], run = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          base[q] = run;
+          run += __popcll(b0[q]) + __popcll(b1[q]);
+        }
+        mw = run;
+        int d0 = 0, d1 = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (c0 == q) d0 = base[q] + lane_rank(b0[q]);
+          if (c1 == q) d1 = base[q] + __popcll(b0[q]) + lane_rank(b1[q]);
+        }
+        put(e0, d0);
+        put(e1, d1);
+      } else {
+        // longer rows: the entries go through the row buffer twice
+        int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+        for (int k0 = 0; k0 < mk; k0 += 64) {
+          const int k = k0 + lane;
+          const int e = k < mk ? classify(k) : 0xFFFF;
+          if (k < mk) rw[k] = (unsigned short)e;
+          const bool ok = e != 0xFFFF;
+          n0 += __popcll(__ballot(ok && (e >> 14) == 0));
+          n1 += __popcll(__ballot(ok && (e >> 14) == 1));
+          n2 += __popcll(__ballot(ok && (e >> 14) == 2));
+          n3 += __popcll(__ballot(ok && (e >> 14) == 3));
+        }
+        __builtin_amdgcn_wave_barrier();
+        int d0 = 0, d1 = n0, d2 = n0 + n1, d3 = n0 + n1 + n2;
+        mw = d3 + n3;
+        for (int k0 = 0; k0 < mk; k0 += 64) {
+          const int k = k0 + lane;
+          const int e = k < mk ? (int)rw[k] : 0xFFFF;
+          const bool ok = e != 0xFFFF;
+          const int cls = e >> 14;
+          const unsigned long long q0 = __ballot(ok && cls == 0), q1 = __ballot(ok && cls == 1),
+                                   q2 = __ballot(ok && cls == 2), q3 = __ballot(ok && cls == 3);
+          const int d = cls == 0   ? d0 + lane_rank(q0)
+                        : cls == 1 ? d1 + lane_rank(q1)
+                        : cls == 2 ? d2 + lane_rank(q2)
+                                   : d3 + lane_rank(q3);
+          put(e, d);
+          d0 += __popcll(q0);
+          d1 += __popcll(q1);
+          d2 += __popcll(q2);
+          d3 += __popcll(q3);
+        }
+      }
+      if (lane < ((4 - (mw & 3)) & 3)) {
+        const int d = mw + lane;
+        row[(size_t)(d >> 2) * qstride + (d & 3)] = (unsigned short)pad;
+      }
+      if (lane == 0) A.nn[a] = mw;
+      mmax = max(mmax, m);  // before the exact drops: an overflow may have lost entries
+      __builtin_amdgcn_wave_barrier();
+    }
+    // the build positions of the batch's atoms for the skin test
+    for (int a = aa0 + (int)threadIdx.x; a < aa1; a += blockDim.x) {
+      const double4 p = pos[a];
+      A.xold[a] = p.x;
+      A.xold[A.ncap + a] = p.y;
+      A.xold[2 * A.ncap + a] = p.z;
+    }
+    xb = xe;
+  }
+  if (lane == 0 && mmax > 0) {
+    atomicMax(&st->maxnn, mmax);
+    if (mmax > cap) {
+      st->overflow = 1;
+      raise_halt(st, GH_HALT_OVERFLOW, step);
+    }
+  }
 }
 
 // Largest halo and brick (atoms) over all bricks; sizes slot_cap.
@@ -1351,8 +1646,37 @@ static BuildArgs build_args(const gh_ctx *c, int cap, size_t *lds_out) {
   A.nn = c->nn;
   A.list16 = c->list16;
   A.xold = c->xold;
+  A.chunks = A.nsl = A.ncand = A.rstride = 0;
   *lds_out = lds;
   return A;
+}
+
+// LDS of the cooperative build: the staged halo and a far slot (float4 + slot
+// map, 17 B per slot), the candidate lists, one row buffer of cap entries per
+// wavefront.
+// Sized like the per-thread build's, so that two workgroups share a CU.
+#define GH_COOP_LDS GH_BUILD_LDS
+
+// Sets the cooperative build's arguments; false where it does not apply: the
+// sizing count (cap 0, any halo), slot numbers beyond the 14 bits a tagged
+// row entry leaves, or LDS too small for one cell's candidate lists beside
+// the staged halo.
+static bool coop_args(const gh_ctx *c, BuildArgs &A, size_t *lds_out) {
+  const int slot_cap = c->tg.slot_cap;
+  if (c->list_rows || A.cap <= 0 || slot_cap >= 16000) return false;
+  const int nsl = (slot_cap + 4) / 4 * 4;  // + the far slot that pads the lists
+  const int rstride = (A.cap + 1) / 2 * 2;
+  const size_t fixed = (size_t)nsl * 17 + (size_t)(GH_CBLOCK / 64) * rstride * 2;
+  // one cell's padded lists: at most the halo's slots + 127
+  if (fixed + 2 * ((size_t)nsl + 128) > GH_COOP_LDS) return false;
+  A.nsl = nsl;
+  A.rstride = rstride;
+  A.ncand = (int)((GH_COOP_LDS - fixed) / 2);
+  // chunks of cells per brick: about two workgroups per CU when bricks are few
+  const int nb = c->tg.nbrick;
+  A.chunks = std::min(c->tg.cpb, std::max(1, (512 + nb - 1) / nb));
+  *lds_out = fixed + 2 * (size_t)A.ncand;
+  return true;
 }
 
 
@@ -1381,8 +1705,9 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   R.count = c->cell_count;
   R.start = c->cell_start;
   R.perm = c->perm;
-  size_t lds = 0;
+  size_t lds = 0, lds_coop = 0;
   h.A = build_args(c, c->cap, &lds);
+  const bool coop = coop_args(c, h.A, &lds_coop);
   hipStream_t s = c->stream;
   // upload on change only, ordered on the stream; the pinned staging copy is
   // rewritten only after the stream has drained
@@ -1404,7 +1729,17 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   { k_cell_bin<<<gb, GH_BINBLOCK, 0, s>>>(c->status, C, c->pos, c->skin, c->step); ++c->launches; }
   { k_cell_scatter<<<ga, GH_TBLOCK, 0, s>>>(c->status, C); ++c->launches; }
   { k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(c->status, C, c->pos); ++c->launches; }
-  { k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos, c->step); ++c->launches; }
+  c->list_build = coop ? 2 : 1;
+  if (coop) {
+    // about 24 atoms per wavefront of a workgroup's mean share, 4 to 16 wavefronts
+    const long long share = (long long)c->n / ((long long)c->tg.nbrick * h.A.chunks) + 1;
+    const int waves = (int)std::min<long long>(GH_CBLOCK / 64, std::max<long long>(4, (share + 23) / 24));
+    const int grid = xgrid(c->tg.nbrick * h.A.chunks);
+    k_build_coop<<<grid, 64 * waves, lds_coop, s>>>(c->status, C, c->pos, c->step);
+  } else {
+    k_build_tiled<<<xgrid(c->tg.nbrick), GH_TBLOCK, lds, s>>>(c->status, C, c->pos, c->step);
+  }
+  ++c->launches;
   GH_HIP(c, hipGetLastError());
   return 0;
 }

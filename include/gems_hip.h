@@ -241,6 +241,8 @@ typedef struct {
   int32_t list_mode; /* 0 = O(N^2) (verlet_self), 1 = cells (linkedcell_self) */
   int64_t steps;
   int64_t launches;  /* kernels the context has enqueued (no counterpart) */
+  int32_t list_build; /* kernel of the last enqueued list build: 0 none, 1 one
+                         thread per atom, 2 cooperative (no counterpart) */
 } gh_stats;
 int gh_get_stats(gh_ctx *ctx, gh_stats *s);
 
