@@ -200,6 +200,12 @@ __device__ __forceinline__ void block_max_atomic(double d2, unsigned long long *
   }
 }
 
+// Lanes of the wavefront below this one set in mask.
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
 __device__ __forceinline__ size_t qidx(int k, int a, int ld) {
   return (((size_t)(k >> 2) * ld + a) << 2) + (k & 3);
 }
@@ -510,7 +516,15 @@ __device__ void cell_bin(const CellArgs &R, const double4 *__restrict__ pos, int
     }
     const int key = grid_key(t, cx, cy, cz);
     R.cellid[i] = key;
-    atomicAdd(&R.count[key], 1);
+    // one atomic per distinct key of the wavefront (atoms sorted at the last
+    // build share cells: a wavefront holds a few keys)
+    for (unsigned long long todo = __ballot(1); todo;) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int lk = __shfl(key, leader, 64);
+      const unsigned long long grp = __ballot(key == lk) & todo;
+      if ((int)(threadIdx.x & 63) == leader) atomicAdd(&R.count[lk], __popcll(grp));
+      todo &= ~grp;
+    }
   }
 }
 
@@ -595,9 +609,23 @@ __device__ void cell_scan(const CellArgs &R) {
 // zero (ready for the next binning); the order within a cell is the atomics'
 // and is fixed by the rank gather.
 __device__ void cell_scatter(const CellArgs &R, int i0, int stride) {
+  const int lane = threadIdx.x & 63;
   for (int i = i0; i < R.n; i += stride) {
     const int key = R.cellid[i];
-    R.perm[R.start[key] + atomicSub(&R.count[key], 1) - 1] = i;
+    // one atomic per distinct key of the wavefront: the group's lanes take
+    // the slots below the count the atomic returns, one each
+    int slot = 0;
+    for (unsigned long long todo = __ballot(1); todo;) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int lk = __shfl(key, leader, 64);
+      const unsigned long long grp = __ballot(key == lk) & todo;
+      int old = 0;
+      if (lane == leader) old = atomicSub(&R.count[lk], __popcll(grp));
+      old = __shfl(old, leader, 64);
+      if (key == lk) slot = R.start[lk] + old - 1 - lane_rank(grp);
+      todo &= ~grp;
+    }
+    R.perm[slot] = i;
   }
 }
 
@@ -773,10 +801,6 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_build_tiled(GhStatus *st,
 // ---------------------------------------------------------------------------
 #define GH_CBLOCK 1024  // workgroup of the cooperative build: 16 wavefronts
 
-__device__ __forceinline__ int lane_rank(unsigned long long mask) {
-  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-}
 
 // Halo cell of brick cell X (0 <= X < cpb).
 __device__ __forceinline__ int halo_cell(const TileGeom &t, const Halo &H, int X) {
