@@ -616,10 +616,11 @@ int gh_create(gh_ctx **out, int device, int precision) {
   if (!c) return GH_EINVAL;
   c->device = device;
   c->precision = precision;
-  // the per-thread list build alone (diagnostics: it writes the same rows as
-  // the cooperative build, which is used wherever its LDS fits)
+  // list-build kernel (both write the same rows): "rows" one thread per
+  // atom, "coop" a wavefront per atom wherever its LDS fits; unset, the
+  // choice of coop_args
   const char *lb = getenv("GEMS_HIP_LIST_BUILD");
-  c->list_rows = lb && strcmp(lb, "rows") == 0 ? 1 : 0;
+  c->list_rows = !lb ? 0 : strcmp(lb, "rows") == 0 ? 1 : strcmp(lb, "coop") == 0 ? 2 : 0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     delete c;

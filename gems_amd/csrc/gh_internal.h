@@ -219,7 +219,7 @@ struct gh_ctx {
   TileGeom tg;                // brick grouping (cell mode)
   int bd_req[3] = {0, 0, 0};  // requested brick edge (0 = automatic)
   int tpb_mean = 0;           // 1: force-pass workgroups sized to the mean brick (autotuned)
-  int list_rows = 0;          // 1: list build by one thread per atom only (GEMS_HIP_LIST_BUILD=rows)
+  int list_rows = 0;          // GEMS_HIP_LIST_BUILD: 1 = rows (one thread per atom), 2 = coop, 0 auto
   int list_build = 0;         // kernel of the last enqueued build: 1 per-thread, 2 cooperative
   int cap = 0;                // list row capacity
   int cap_req = 0;

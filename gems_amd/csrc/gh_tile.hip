@@ -1684,10 +1684,15 @@ static BuildArgs build_args(const gh_ctx *c, int cap, size_t *lds_out) {
 // Sets the cooperative build's arguments; false where it does not apply: the
 // sizing count (cap 0, any halo), slot numbers beyond the 14 bits a tagged
 // row entry leaves, or LDS too small for one cell's candidate lists beside
-// the staged halo.
+// the staged halo.  Automatic choice: the per-thread build with its rows in
+// LDS where bricks fill the chip (1024 and more: 1M-atom SMA-TB, 526 us
+// against 563 us per rebuild), the cooperative build elsewhere (few bricks,
+// or rows too long for LDS: the 1M-atom LJ liquid 824 us against 1241 us,
+// 4000 atoms 37 us against 235 us).
 static bool coop_args(const gh_ctx *c, BuildArgs &A, size_t *lds_out) {
   const int slot_cap = c->tg.slot_cap;
-  if (c->list_rows || A.cap <= 0 || slot_cap >= 16000) return false;
+  if (c->list_rows == 1 || A.cap <= 0 || slot_cap >= 16000) return false;
+  if (c->list_rows == 0 && A.lds_rows && c->tg.nbrick >= 1024) return false;
   const int nsl = (slot_cap + 4) / 4 * 4;  // + the far slot that pads the lists
   const int rstride = (A.cap + 1) / 2 * 2;
   const size_t fixed = (size_t)nsl * 17 + (size_t)(GH_CBLOCK / 64) * rstride * 2;

@@ -1,5 +1,6 @@
 """The cooperative list build (a wavefront per atom row, 64 candidates per
-step) against the per-thread build (GEMS_HIP_LIST_BUILD=rows): both write the
+step; GEMS_HIP_LIST_BUILD=coop) against the per-thread build
+(GEMS_HIP_LIST_BUILD=rows), each forced on one context: both write the
 same rows in the same order, so forces, energies and whole trajectories
 through rebuilds agree bit for bit in both precisions.  Row order decides
 the FP64 summation order, so a reordered row shows up in the last bits of
@@ -24,8 +25,7 @@ hip = pytest.importorskip("gems_amd.hip")
 
 
 def _make(x, types, L, pot, precision, rows, skin):
-    if rows:
-        os.environ["GEMS_HIP_LIST_BUILD"] = "rows"
+    os.environ["GEMS_HIP_LIST_BUILD"] = "rows" if rows else "coop"
     try:
         c = hip.Context(0, precision)
     finally:
