@@ -221,10 +221,10 @@ static const int GH_LDS_BYTES = 65536;
 // the static tables, over the larger per-slot need of the two passes.
 size_t gh_tile_slot_bytes(int pass, int precision, int nt);
 size_t gh_tile_static_bytes(int precision, int nt);
-size_t gh_pair_slot_bytes(int pot);
+size_t gh_pair_slot_bytes(int pot, int precision);
 static int max_slots(const gh_ctx *c) {
   const bool pair = c->pot == GH_POT_LJ || gh_is_table_pot(c->pot);
-  const size_t per = pair ? gh_pair_slot_bytes(c->pot)
+  const size_t per = pair ? gh_pair_slot_bytes(c->pot, c->precision)
                           : std::max(gh_tile_slot_bytes(0, c->precision, c->nt),
                                      gh_tile_slot_bytes(1, c->precision, c->nt));
   const size_t stat = pair ? gh_tile_static_bytes(GH_FP64, 1) + sizeof(PljDev) * GH_MAXT * GH_MAXT
@@ -295,7 +295,7 @@ static void finish_tiles(gh_ctx *c) {
   }
   t.nkey = t.nbrick * t.cpb;
   c->ncell = t.nkey;
-  // fixed-point unit of the mixed SMA-TB tiles: a staged atom lies within half
+  // fixed-point unit of the mixed tiles (SMA-TB, LJ): a staged atom lies within half
   // a brick plus its halo cell of the brick centre, plus half the skin (the
   // largest displacement between rebuilds); 1 A of margin
   double ext = 0.0;
@@ -696,11 +696,29 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
   double cells = 1.0;
   for (int d = 0; d < 3; ++d) cells *= c->nc[d];
   const double apc = c->n / cells;
+  // Expected rebuilds per step: one falls due when an atom has moved half the
+  // skin, so at most 2 v_max dt / skin (ballistic motion of the fastest
+  // atom).  Each shape is scored by its force evaluation plus that share of
+  // its rebuild chain; before velocities and a time step are set the rate is
+  // 0 and the force evaluation alone decides.
+  double rate = 0.0;
+  if (c->dt > 0.0 && c->skin > 0.0) {
+    std::vector<double> v(3 * (size_t)c->ncap);
+    GH_HIP(c, hipMemcpyAsync(v.data(), c->vel, sizeof(double) * v.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    double v2 = 0.0;
+    for (int i = 0; i < c->n; ++i) {
+      const double a = v[i], b = v[c->ncap + i], d = v[2 * (size_t)c->ncap + i];
+      v2 = std::max(v2, a * a + b * b + d * d);
+    }
+    rate = std::min(1.0, 2.0 * sqrt(v2) * c->dt / c->skin);
+  }
   hipEvent_t e0, e1;
   GH_HIP(c, hipEventCreate(&e0));
   GH_HIP(c, hipEventCreate(&e1));
   int best[3] = {0, 0, 0}, best_mean = 0;
-  double best_us = 1e30;
+  double best_us = 1e30, best_score = 1e30;
   for (int bz = 1; bz <= 4; ++bz)
     for (int by = bz; by <= 4; ++by)
       for (int bx = by; bx <= 4; ++bx) {
@@ -716,6 +734,18 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
         bool same = true;  // the sizing may have shrunk the request
         for (int d = 0; d < 3; ++d) same = same && c->tg.bd[d] == b[d];
         if (!same) continue;
+        double chain_us = 0.0;
+        if (rate > 0.0) {
+          GH_TRY(gh_request_rebuild(c));
+          GH_HIP(c, hipEventRecord(e0, c->stream));
+          GH_TRY(gh_launch_rebuild(c, false));
+          GH_HIP(c, hipEventRecord(e1, c->stream));
+          GH_HIP(c, hipEventSynchronize(e1));
+          float ms = 0.f;
+          GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+          if (check_status(c) != 0) continue;
+          chain_us = 1e3 * ms;
+        }
         const int cand[2] = {c->tg.tpb_max, c->tg.tpb_avg};
         for (int m = 0; m < 2; ++m) {
           if (m == 1 && cand[1] == cand[0]) break;
@@ -729,7 +759,8 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
           GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
           if (check_status(c) != 0) break;
           const double us = 1e3 * ms / reps;
-          if (us < best_us) {
+          if (us + rate * chain_us < best_score) {
+            best_score = us + rate * chain_us;
             best_us = us;
             best_mean = m;
             for (int d = 0; d < 3; ++d) best[d] = b[d];

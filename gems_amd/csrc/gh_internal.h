@@ -328,6 +328,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv);
 int gh_fill_int(gh_ctx *c, int *p, int v, int n);
 int gh_launch_wrap(gh_ctx *c);
 int gh_tile_occupancy(gh_ctx *c, int *max_halo, int *max_brick);
+int gh_request_rebuild(gh_ctx *c);  // the next rebuild chain rebuilds whatever the displacement
 int gh_tiled_to_global(gh_ctx *c, int *dst);
 int gh_upload_tb(gh_ctx *c);
 int gh_upload_tbf(gh_ctx *c, int k);

@@ -916,16 +916,20 @@ ErmakArgs make_ermak(const gh_ctx *c) {
 // sequence can be enqueued every step without a host round trip.
 using namespace ghd;
 
+int gh_request_rebuild(gh_ctx *c) {
+  hipStream_t s = c->stream;
+  GhStatus h;
+  GH_HIP(c, hipMemcpyAsync(&h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, s));
+  GH_HIP(c, hipStreamSynchronize(s));
+  h.force_rebuild = 1;
+  GH_HIP(c, hipMemcpyAsync(c->status, &h, sizeof(GhStatus), hipMemcpyHostToDevice, s));
+  GH_HIP(c, hipStreamSynchronize(s));
+  return 0;
+}
+
 int gh_launch_rebuild(gh_ctx *c, bool force) {
   hipStream_t s = c->stream;
-  if (force) {
-    GhStatus h;
-    GH_HIP(c, hipMemcpyAsync(&h, c->status, sizeof(GhStatus), hipMemcpyDeviceToHost, s));
-    GH_HIP(c, hipStreamSynchronize(s));
-    h.force_rebuild = 1;
-    GH_HIP(c, hipMemcpyAsync(c->status, &h, sizeof(GhStatus), hipMemcpyHostToDevice, s));
-    GH_HIP(c, hipStreamSynchronize(s));
-  }
+  if (force) GH_TRY(gh_request_rebuild(c));
   const int n = c->n, nb = nblocks(n);
   if (c->list_mode == 1) return gh_launch_rebuild_cells(c);  // decides in its first kernels
   { k_decide<<<1, 1, 0, s>>>(c->status, c->skin); ++c->launches; }

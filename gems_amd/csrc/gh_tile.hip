@@ -1467,7 +1467,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_force64_tiled(
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
 // POT: GH_POT_LJ (eps, sigma as arguments) or a tabulated form (plj, slj,
 // wca: per type pair, types staged in LDS).
-template <bool MIXED, int POT>
+template <bool MIXED, int POT, bool FUSED>
 __device__ __forceinline__ void lj_tiled_body(
     GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1520,13 +1520,13 @@ __device__ __forceinline__ void lj_tiled_body(
             if (r2 > P.rc2) return;  // this pair's cut
             float p, f_r;
             table_pair32<POT>(P, (float)r2, p, f_r);
-            ei += (double)p;
+            if (!FUSED) ei += (double)p;
             sf = (double)f_r;
           } else {
             const float ir2 = __builtin_amdgcn_rcpf((float)r2);
             const float s2 = sig2f * ir2;
             const float s6 = s2 * s2 * s2;
-            ei += (double)(4.0f * epsf * (s6 * s6 - s6));
+            if (!FUSED) ei += (double)(4.0f * epsf * (s6 * s6 - s6));
             sf = (double)(epsf * fmaf(48.0f, s6, -24.0f) * s6 * ir2);
           }
           fx = fma(-sf, vx, fx);
@@ -1542,12 +1542,12 @@ __device__ __forceinline__ void lj_tiled_body(
             if (r2 > P.rc2) return;  // this pair's cut
             double p;
             table_pair64<POT>(P, r2, p, sf);
-            ei += p;
+            if (!FUSED) ei += p;
           } else {
             const double dr = sqrt(r2);
             const double tt = sigma / dr;
             const double t2 = tt * tt, t4 = t2 * t2, s6 = t4 * t2;
-            ei += 4.0 * eps * (s6 * s6 - s6);
+            if (!FUSED) ei += 4.0 * eps * (s6 * s6 - s6);
             sf = eps * (48.0 * s6 - 24.0) * s6 / dr / dr;
           }
           fx -= sf * vx;
@@ -1557,7 +1557,7 @@ __device__ __forceinline__ void lj_tiled_body(
       }
     });
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
-    if (fv.on) {
+    if (FUSED) {
       dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
     } else {
       frc[a] = Fx;
@@ -1566,10 +1566,10 @@ __device__ __forceinline__ void lj_tiled_body(
       epa[a] = ei * 0.5 * ev_ui;
     }
   }
-  if (fv.on) block_max_atomic(dmax2, fv.maxd2);
+  if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
 // One pass per step: gated (and publishing) as a first pass.
-template <bool MIXED, int POT>
+template <bool MIXED, int POT, bool FUSED>
 __global__ __launch_bounds__(GH_TBMAX) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
@@ -1577,8 +1577,88 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_tiled(
     double *__restrict__ epa, const int *__restrict__ type, FuseVV fv,
     const PljDev *__restrict__ pljd, StepArgs sa) {
   if (!step_gate(st, sa))
-    lj_tiled_body<MIXED, POT>(st, t, pos, cell_start, nn, list16, ld, ncap, eps, sigma, rc2,
+    lj_tiled_body<MIXED, POT, FUSED>(st, t, pos, cell_start, nn, list16, ld, ncap, eps, sigma, rc2,
                               ev_ui, frc, epa, type, fv, pljd, sa.step);
+}
+
+// Lennard-Jones, mixed precision (lj_interact, Pairs.F90:207-279), on the
+// fixed-point tiles of the SMA-TB passes: LDS slot = {qx, qy, qz, -} in steps
+// of 2^-t.fxk A about the brick centre, one 16-byte read per neighbour; the
+// pair vector is the exact integer difference and r^2 is exact in FP64, the
+// pair terms are FP32 in the same unit (sig2 = sigma^2 and rc2 = rc^2 in
+// steps^2), so f/r per step^2 summed over integer vectors, times 2^t.fxk,
+// is the force in eV/A.  FUSED: the velocity Verlet update, no energies.
+struct LjF {
+  double rc2;      // rc^2 in steps^2
+  float sig2, e4;  // sigma^2 in steps^2, 4 eps
+  float e24, e48;  // 24 eps, 48 eps
+};
+template <bool FUSED>
+__global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
+    GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
+    const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
+    LjF p, double ev_ui, double *__restrict__ frc, double *__restrict__ epa,
+    const int *__restrict__ type, FuseVV fv, StepArgs sa) {
+  if (step_gate(st, sa)) return;
+  extern __shared__ int4 lq4[];
+  __shared__ Halo H;
+  const int b = xcd_block();
+  if (b >= t.nbrick) return;
+  if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
+  const int ns = t.slot_cap + 1;  // slots + the sentinel
+  unsigned char *scell = reinterpret_cast<unsigned char *>(lq4 + ns);
+  slot_map(H, scell);
+  const double fxs = t.fxs;
+  for (int s = threadIdx.x; s < H.total; s += blockDim.x) {
+    const int h = scell[s];
+    const double4 q = pos[s + H.delta[h]];
+    lq4[s] = make_int4(fxq(q.x - (H.org[0] - shift(t, H, 0, h)), fxs),
+                       fxq(q.y - (H.org[1] - shift(t, H, 1, h)), fxs),
+                       fxq(q.z - (H.org[2] - shift(t, H, 2, h)), fxs), 0);
+  }
+  if (threadIdx.x == 0) lq4[t.slot_cap] = make_int4(GH_FXFAR, GH_FXFAR, GH_FXFAR, 0);
+  __syncthreads();
+  const float sig2 = in_vgpr(p.sig2), e24 = p.e24, e48 = p.e48, e4 = p.e4;
+  const double scale = fxs * ev_ui;
+  double dmax2 = 0.0;
+  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+    const double4 pa = pos[a];
+    const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
+              qz = fxq(pa.z - H.org[2], fxs);
+    double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
+    auto pair = [&](const int4 &q) {
+      const double dx = fxd(q.x, qx), dy = fxd(q.y, qy), dz = fxd(q.z, qz);
+      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      if (r2 <= p.rc2) {
+        const float ir2 = __builtin_amdgcn_rcpf((float)r2);
+        const float s2 = sig2 * ir2;
+        const float s6 = s2 * s2 * s2;
+        if (!FUSED) ei += (double)(e4 * (s6 * s6 - s6));
+        const double sf = (double)(fmaf(e48, s6, -e24) * s6 * ir2);
+        fx = fma(-sf, dx, fx);
+        fy = fma(-sf, dy, fy);
+        fz = fma(-sf, dz, fz);
+      }
+    };
+    for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
+      const int4 q0 = lq4[GH_QSLOT(w, 0)], q1 = lq4[GH_QSLOT(w, 1)], q2 = lq4[GH_QSLOT(w, 2)],
+                 q3 = lq4[GH_QSLOT(w, 3)];
+      pair(q0);
+      pair(q1);
+      pair(q2);
+      pair(q3);
+    });
+    const double Fx = fx * scale, Fy = fy * scale, Fz = fz * scale;
+    if (FUSED) {
+      dmax2 = fmax(dmax2, fused_vv(fv, a, pa, type[a], Fx, Fy, Fz));
+    } else {
+      frc[a] = Fx;
+      frc[ncap + a] = Fy;
+      frc[2 * ncap + a] = Fz;
+      epa[a] = ei * 0.5 * ev_ui;
+    }
+  }
+  if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
 
 }  // namespace
@@ -1595,8 +1675,12 @@ size_t gh_tile_slot_bytes(int pass, int precision, int nt) {
   if (pass == 0) return 24 + 1 + (multi ? 4 : 0);  // rho: xyz + map (+ type)
   return 32 + 1 + (multi ? 4 : 0);                 // force: xyz + eband + map (+ type)
 }
-// Pair potentials: FP64 xyz + map per slot (+ the type for the tabulated forms).
-size_t gh_pair_slot_bytes(int pot) { return 25 + (gh_is_table_pot(pot) ? 1 : 0); }
+// Pair potentials: FP64 xyz + map per slot (+ the type for the tabulated
+// forms); mixed-precision LJ: fixed-point int4 + map.
+size_t gh_pair_slot_bytes(int pot, int precision) {
+  if (pot == GH_POT_LJ && precision == GH_MIXED) return 16 + 1;
+  return 25 + (gh_is_table_pot(pot) ? 1 : 0);
+}
 size_t gh_tile_static_bytes(int precision, int nt) {
   const size_t params = nt > 1 ? (precision == GH_MIXED ? sizeof(TbF) : sizeof(TbDev)) *
                                      GH_MAXT * GH_MAXT
@@ -1907,13 +1991,17 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   } else if (c->pot == GH_POT_LJ || gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
-    const size_t lds = gh_pair_slot_bytes(c->pot) * slots;
+    const size_t lds = gh_pair_slot_bytes(c->pot, c->precision) * slots;
 #define LJ_LAUNCH(MX, PT)                                                                   \
   do {                                                                                      \
-    k_lj_tiled<MX, PT><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start, \
-                                                    c->nn, c->list16, ld, c->ncap, c->lj_eps, \
-                                                    c->lj_sigma, rc2, ev_ui, c->frc, c->epa, \
-                                                    c->type, fv, c->pljd, sa);              \
+    if (fuse_vv)                                                                            \
+      k_lj_tiled<MX, PT, true><<<grid, c->tg.tpb, lds, s>>>(                                \
+          c->status, c->tg, c->pos, c->cell_start, c->nn, c->list16, ld, c->ncap, c->lj_eps, \
+          c->lj_sigma, rc2, ev_ui, c->frc, c->epa, c->type, fv, c->pljd, sa);               \
+    else                                                                                    \
+      k_lj_tiled<MX, PT, false><<<grid, c->tg.tpb, lds, s>>>(                               \
+          c->status, c->tg, c->pos, c->cell_start, c->nn, c->list16, ld, c->ncap, c->lj_eps, \
+          c->lj_sigma, rc2, ev_ui, c->frc, c->epa, c->type, fv, c->pljd, sa);               \
     ++c->launches;                                                                          \
   } while (0)
 #define LJ_POTS(MX)                                                                         \
@@ -1924,7 +2012,24 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     case GH_POT_SM1: LJ_LAUNCH(MX, GH_POT_SM1); break;                                      \
     default: LJ_LAUNCH(MX, GH_POT_LJ);                                                      \
   }
-    if (mixed) {
+    if (mixed && c->pot == GH_POT_LJ) {
+      LjF p;
+      const double f2 = c->tg.fxs * c->tg.fxs;
+      p.rc2 = rc2 * f2;
+      p.sig2 = (float)(c->lj_sigma * c->lj_sigma * f2);
+      p.e4 = (float)(4.0 * c->lj_eps);
+      p.e24 = (float)(24.0 * c->lj_eps);
+      p.e48 = (float)(48.0 * c->lj_eps);
+      if (fuse_vv)
+        k_lj_fx<true><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
+                                                   c->nn, c->list16, ld, c->ncap, p, ev_ui,
+                                                   c->frc, c->epa, c->type, fv, sa);
+      else
+        k_lj_fx<false><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
+                                                    c->nn, c->list16, ld, c->ncap, p, ev_ui,
+                                                    c->frc, c->epa, c->type, fv, sa);
+      ++c->launches;
+    } else if (mixed) {
       LJ_POTS(true)
     } else {
       LJ_POTS(false)

@@ -160,10 +160,12 @@ int gh_set_skin(gh_ctx *ctx, double skin);
  * workgroup per brick.  No counterpart in the reference. */
 int gh_set_tile(gh_ctx *ctx, const int bd[3]);
 /* Autotuner: times `reps` force evaluations for each brick shape whose halo
- * fits the LDS budget and keeps the fastest.  Writes the chosen shape and its
- * time per force evaluation (us); bd_out = {0,0,0} when the O(N^2) list is in
- * use (nothing to tune).  Positions and velocities are unchanged.  No
- * counterpart in the reference. */
+ * fits the LDS budget and keeps the fastest.  With velocities and a time step
+ * set, each shape's time is that of a force evaluation plus its rebuild
+ * chain times the expected rebuilds per step (2 v_max dt / skin, at most 1).
+ * Writes the chosen shape and its time per force evaluation (us); bd_out =
+ * {0,0,0} when the O(N^2) list is in use (nothing to tune).  Positions and
+ * velocities are unchanged.  No counterpart in the reference. */
 int gh_autotune(gh_ctx *ctx, int reps, int bd_out[3], double *us_out);
 
 /* New positions (3n, A, caller order) for the next gh_compute when the
