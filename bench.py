@@ -331,12 +331,13 @@ def drift_vs_reference(hip):
     return out
 
 
-def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72):
+def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72, tiles=None):
     """Lennard-Jones (lj_interact, Pairs.F90:207-279) on the LJ-liquid geometry
     of the LJ NVE configuration: fcc nx^3 cells at rho* = 0.8442 (4 nx^3
     atoms), rc = 2.5 sigma (unshifted), skin 0.3 sigma, eps = 1 eV, sigma =
     1 A, m = 1 amu, T* from the lcg stream (`set tempgdist`), velocity Verlet
-    dt = 0.005 tau; mixed and FP64, each with its autotuned brick shape."""
+    dt = 0.005 tau; mixed and FP64, each with its autotuned brick shape (or the
+    shape tiles[name] names)."""
     a = (4.0 / 0.8442) ** (1.0 / 3.0)
     x = fcc(nx, a)
     if jitter > 0:
@@ -358,7 +359,11 @@ def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72):
         c.prng_lcg(4242)
         c.set_gdist(T)
         c.integrator_vv(0.005 * tau)
-        shape, _ = c.autotune(reps=3)
+        if tiles and tiles.get(name):
+            shape = tiles[name]
+            c.set_tile(shape)
+        else:
+            shape, _ = c.autotune(reps=3)
         F[name] = c.compute()[0]
         dt, _, nupd, launches = timed_run(c, steps, warmup)
         _, tm, _, _ = timed_run(c, steps, 0, timing=True)
@@ -370,9 +375,9 @@ def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72):
     # the fused pass moves x 24 + v 48 + the next x 24 = 96 algorithmic B/atom
     t_us = out["mixed"]["force_pass_us"]
     ach = 96.0 * len(x) / (t_us * 1e-6) / 1e9
-    traffic, src = profiled("traffic_bytes_per_launch", "k_lj_tiled",
-                            "k_lj_tiled_lj1m_counters.md") if len(x) > 10 ** 6 else (None, None)
-    out["roofline"] = {"kernel": "k_lj_tiled (mixed, velocity Verlet fused)", "bound": "hbm",
+    traffic, src = profiled("traffic_bytes_per_launch", "k_lj_fx",
+                            "k_lj_fx_lj1m_counters.md") if len(x) > 10 ** 6 else (None, None)
+    out["roofline"] = {"kernel": "k_lj_fx (mixed, velocity Verlet fused)", "bound": "hbm",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": src,
                        "bytes_per_atom": 96, "avg_launch_us": t_us}
@@ -514,16 +519,24 @@ def main():
     ap.add_argument("--fire-clusters", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM counters)")
+    ap.add_argument("--lj-tile", default=None,
+                    help="with --lj-only: brick shapes 'bx,by,bz[:bx,by,bz]' of the 1M LJ leg "
+                         "(mixed[:FP64]) instead of the autotuner")
     ap.add_argument("--lj-only", action="store_true",
-                    help="only run the Lennard-Jones legs (profiling runs of k_lj_tiled)")
+                    help="only run the Lennard-Jones legs (profiling runs of k_lj_fx, k_lj_tiled)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
         return
     if args.lj_only:
         from gems_amd import hip
+        tiles = None
+        if args.lj_tile:  # 'bx,by,bz:bx,by,bz' = the mixed and FP64 shapes of the 1M leg
+            mx, _, f64 = args.lj_tile.partition(":")
+            tiles = {"mixed": [int(v) for v in mx.split(",")],
+                     "fp64": [int(v) for v in f64.split(",")] if f64 else None}
         print(json.dumps({"lj_4000": lj_bulk(hip, 10, 1000),
-                          "lj_1m": lj_bulk(hip, 64, 100, jitter=0.01, Tstar=1.44)}))
+                          "lj_1m": lj_bulk(hip, 64, 100, jitter=0.01, Tstar=1.44, tiles=tiles)}))
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
