@@ -7,9 +7,10 @@ the FP64 summation order, so a reordered row shows up in the last bits of
 the forces.
 
 Cases: the golden Au 864 and LJ 4000 boxes (few bricks: several workgroups
-per brick), Au 32,000 atoms at three brick shapes, and an LJ fcc crystal with
+per brick), Au 32,000 atoms at three brick shapes, an LJ fcc crystal with
 4 x 4 x 2-cell bricks, whose candidate lists overflow one LDS batch (the
-cells then run in two batches).  The sets themselves are checked against
+cells then run in two batches), an Au slab free in z (guard cells) and an Au
+cluster in a large periodic box (empty cells and bricks).  The sets themselves are checked against
 the oracle in test_gpu_parity.py, on the cooperative path.
 """
 import os
@@ -122,6 +123,47 @@ def test_cooperative_rows_match_in_batches():
     for prec in (hip.GH_FP64, hip.GH_MIXED):
         cs = _pair(x, np.zeros(len(x), np.int32), L, "lj", prec, 0.3, tile=np.array([4, 4, 2]))
         res = [c.compute() for c in cs]
+        _check_built(cs)
+        _same(res[0][:2], res[1][:2])
+        assert res[0][2] == res[1][2]
+        for c in cs:
+            c.close()
+
+
+def _slab_and_cluster():
+    a = 4.078
+    x = cm.fcc(16, a, 16, 10)
+    x += np.random.default_rng(11).normal(0.0, 0.05, x.shape)
+    # a slab: periodic in x and y, free in z (guard cells below and above)
+    slab = (x, np.array([16 * a, 16 * a, 10 * a + 12.0]), [1, 1, 0])
+    # a cluster of the same lattice cut to a sphere in a large periodic box:
+    # most cells and whole bricks are empty
+    c0 = x.mean(axis=0)
+    ball = x[np.sum((x - c0) ** 2, axis=1) < 22.0 ** 2] - c0 + 60.0
+    cluster = (ball, np.array([120.0, 120.0, 120.0]), [1, 1, 1])
+    return {"slab": slab, "cluster": cluster}
+
+
+@pytest.mark.parametrize("case", ["slab", "cluster"])
+def test_cooperative_rows_match_slab_and_sparse_box(case):
+    x, L, pbc = _slab_and_cluster()[case]
+    t = np.zeros(len(x), np.int32)
+    for prec in (hip.GH_FP64, hip.GH_MIXED):
+        cs = []
+        for rows in (False, True):
+            os.environ["GEMS_HIP_LIST_BUILD"] = "rows" if rows else "coop"
+            try:
+                c = hip.Context(0, prec)
+            finally:
+                os.environ.pop("GEMS_HIP_LIST_BUILD", None)
+            c.set_box(L, pbc)
+            c.set_atoms(x, t)
+            c.set_masses([cm.MASS_AU])
+            c.set_smatb(1, *cm.matrices(cm.au_params(), 1))
+            c.set_skin(1.0)
+            cs.append(c)
+        res = [c.compute() for c in cs]
+        assert cs[0].stats()["list_mode"] == 1  # the tiled cell path
         _check_built(cs)
         _same(res[0][:2], res[1][:2])
         assert res[0][2] == res[1][2]
