@@ -244,10 +244,15 @@ static int force_pass_threads(const TileGeom &t, int atoms) {
   const int need = std::max(nh, atoms);
   return std::min(1024, (need + 63) / 64 * 64);
 }
+// Below 131,072 atoms (fewer than two wavefronts per SIMD at one lane per
+// atom) four lanes share each atom's row, so a pass has four times the
+// wavefronts to hide its latencies; the lane count depends on the atom count
+// alone, which fixes the order of every FP64 sum whatever the brick shape.
 static void set_force_pass_threads(gh_ctx *c, int max_brick) {
   TileGeom &t = c->tg;
-  t.tpb_max = force_pass_threads(t, max_brick + max_brick / 16 + 1);
-  t.tpb_avg = force_pass_threads(t, (int)((c->n + t.nbrick - 1) / t.nbrick));
+  t.lanes = c->n < 131072 ? 4 : 1;
+  t.tpb_max = force_pass_threads(t, t.lanes * (max_brick + max_brick / 16 + 1));
+  t.tpb_avg = force_pass_threads(t, t.lanes * (int)((c->n + t.nbrick - 1) / t.nbrick));
   t.tpb = c->tpb_mean ? t.tpb_avg : t.tpb_max;
 }
 

@@ -225,6 +225,46 @@ __device__ __forceinline__ void for_each_quad(const unsigned short *__restrict__
 }
 #define GH_QSLOT(w, u) ((int)(((w) >> (16 * (u))) & 0xFFFFull))
 
+// Small systems: G consecutive lanes share one atom's row, lane `sub` taking
+// quads sub, sub + G, ... (loaded four at a time before use, so a lane's list
+// loads are in flight together); the partial sums meet in group_sum.  The
+// order of the FP64 sums is fixed by G, which the host derives from the atom
+// count alone.
+template <int G, class F>
+__device__ __forceinline__ void for_each_quad_g(const unsigned short *__restrict__ list16, int ld,
+                                                int a, int m, int sub, F &&f) {
+  const int ng = (m + 3) >> 2;
+  const unsigned long long *lp = reinterpret_cast<const unsigned long long *>(list16) + a;
+  const size_t sld = (size_t)ld;
+  for (int g0 = sub; g0 < ng; g0 += 4 * G) {
+    unsigned long long w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int g = g0 + u * G;
+      w[u] = g < ng ? ld_quad(lp + (size_t)g * sld) : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (g0 + u * G < ng) f(w[u]);
+  }
+}
+template <int G, class F>
+__device__ __forceinline__ void for_each_slot_g(const unsigned short *__restrict__ list16, int ld,
+                                                int a, int m, int sub, F &&f) {
+  for_each_quad_g<G>(list16, ld, a, m, sub, [&](unsigned long long w) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) f(GH_QSLOT(w, u));
+  });
+}
+// Sum over the G lanes of an atom group (aligned groups of G lanes of one
+// wavefront, all active together); every lane gets the total.
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // SMA-TB FP32 pair forms (the TbF terms; r2 in the TbF unit, FP64).  r is
 // the FP32 r2 times its hardware reciprocal square root y; for the inner
 // branch s2 = 2 (r - r0f) is recovered to FP32 precision of s itself from

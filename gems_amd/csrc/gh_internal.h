@@ -122,6 +122,7 @@ struct TileGeom {
   int slot_cap;   // LDS slots per workgroup (halo atoms)
   int tpb;        // threads per workgroup of the force passes (a multiple of 64)
   int tpb_max, tpb_avg;  // its two candidates: the fullest brick, the mean brick
+  int lanes = 1;  // lanes per atom in the force passes: 4 below 131072 atoms (fixed by n)
   int fxk;        // fixed-point steps per Angstrom: 2^fxk (mixed SMA-TB)
   double fxs;     // 2^fxk
   double L[3];

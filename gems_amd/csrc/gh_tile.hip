@@ -1151,7 +1151,7 @@ __device__ __forceinline__ TbF tbf_vgpr(const TbF &p) {
   return v;
 }
 
-template <bool SINGLE, int S>
+template <bool SINGLE, int S, int G>
 __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1190,7 +1190,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
   }
   __syncthreads();
   const TbF pv = tbf_vgpr(p1);
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+  const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
+  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
@@ -1203,7 +1204,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
                     band))
         acc += (double)band;
     };
-    for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
+    auto quad = [&](unsigned long long w) {
       const int s0 = GH_QSLOT(w, 0), s1 = GH_QSLOT(w, 1), s2 = GH_QSLOT(w, 2), s3 = GH_QSLOT(w, 3);
       const int x0 = qxa[s0], y0 = qya[s0], z0 = qza[s0];
       const int x1 = qxa[s1], y1 = qya[s1], z1 = qza[s1];
@@ -1213,13 +1214,18 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
       pair(x1, y1, z1, s1);
       pair(x2, y2, z2, s2);
       pair(x3, y3, z3, s3);
-    });
-    if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
-    double yr = __builtin_amdgcn_rsq(acc);
-    const double hr = 0.5 * acc;
-    yr = yr * fma(-hr * yr, yr, 1.5);
-    yr = yr * fma(-hr * yr, yr, 1.5);
-    ebd32[a] = (float)(yr - e0);
+    };
+    if (G == 1) for_each_quad(list16, ld, a, nn[a], quad);
+    else for_each_quad_g<G>(list16, ld, a, nn[a], sub, quad);
+    if (G > 1) acc = group_sum<G>(acc);
+    if (sub == 0) {
+      if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
+      double yr = __builtin_amdgcn_rsq(acc);
+      const double hr = 0.5 * acc;
+      yr = yr * fma(-hr * yr, yr, 1.5);
+      yr = yr * fma(-hr * yr, yr, 1.5);
+      ebd32[a] = (float)(yr - e0);
+    }
   }
 }
 
@@ -1229,7 +1235,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
 // LDS of S slots as in pass 1 (S = 0: dynamic).  FUSED: the
 // velocity Verlet update of the atoms (no forces or energies stored, no
 // repulsion summed); otherwise forces and per-atom energies are stored.
-template <bool SINGLE, bool FUSED, int S>
+template <bool SINGLE, bool FUSED, int S, int G>
 __device__ __forceinline__ void tb_force_fx_body(
     GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1270,7 +1276,8 @@ __device__ __forceinline__ void tb_force_fx_body(
   // over integer pair vectors come out in eV/A
   double dmax2 = 0.0;
   const TbF pv = tbf_vgpr(p1);
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+  const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
+  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
@@ -1290,14 +1297,23 @@ __device__ __forceinline__ void tb_force_fx_body(
         if (!FUSED) ei += (double)repul;
       }
     };
-    for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
+    auto quad = [&](unsigned long long w) {
       const int s0 = GH_QSLOT(w, 0), s1 = GH_QSLOT(w, 1), s2 = GH_QSLOT(w, 2), s3 = GH_QSLOT(w, 3);
       const int4 q0 = q4[s0], q1 = q4[s1], q2 = q4[s2], q3 = q4[s3];
       pair(q0, s0);
       pair(q1, s1);
       pair(q2, s2);
       pair(q3, s3);
-    });
+    };
+    if (G == 1) for_each_quad(list16, ld, a, nn[a], quad);
+    else for_each_quad_g<G>(list16, ld, a, nn[a], sub, quad);
+    if (G > 1) {
+      fx = group_sum<G>(fx);
+      fy = group_sum<G>(fy);
+      fz = group_sum<G>(fz);
+      if (!FUSED) ei = group_sum<G>(ei);
+    }
+    if (sub != 0) continue;
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
     if (FUSED) {
       dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
@@ -1310,7 +1326,7 @@ __device__ __forceinline__ void tb_force_fx_body(
   }
   if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
-template <bool SINGLE, bool FUSED, int S>
+template <bool SINGLE, bool FUSED, int S, int G>
 __global__ __launch_bounds__(GH_TBMAX) void k_tb_force_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1318,7 +1334,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_force_fx(
     const double *__restrict__ rho, const float *__restrict__ ebd32, double ev_ui,
     double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, TbF p1, StepArgs sa) {
   if (!halted(st))
-    tb_force_fx_body<SINGLE, FUSED, S>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbf, rho,
+    tb_force_fx_body<SINGLE, FUSED, S, G>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbf, rho,
                                     ebd32, ev_ui, frc, epa, fv, p1, sa.step);
 }
 
@@ -1326,7 +1342,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_force_fx(
 // SMA-TB, plain FP64 (the reference's arithmetic; LDS holds FP64 positions
 // with the image shift applied).
 // ---------------------------------------------------------------------------
-template <bool SINGLE>
+template <bool SINGLE, int G>
 __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1364,16 +1380,21 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho64_tiled(
     if (!SINGLE) ty[t.slot_cap] = 0;
   }
   __syncthreads();
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+  const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
+  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
     double acc = 0.0;
-    for_each_slot(list16, ld, a, nn[a], [&](int s) {
+    auto slot = [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
       const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
       const double dr = sqrt(dist2_rn(vx, vy, vz));
       if (dr <= P.rce) acc += tb_band64(P, dr);
-    });
+    };
+    if (G == 1) for_each_slot(list16, ld, a, nn[a], slot);
+    else for_each_slot_g<G>(list16, ld, a, nn[a], sub, slot);
+    if (G > 1) acc = group_sum<G>(acc);
+    if (sub != 0) continue;
     if (rho) rho[a] = acc;  // null on fused steps: only energy output reads it
     const double eb = 1.0 / sqrt(acc);
     eband[a] = eb;
@@ -1381,7 +1402,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho64_tiled(
   }
 }
 
-template <bool SINGLE>
+template <bool SINGLE, int G>
 __device__ __forceinline__ void tb_force64_body(
     GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1421,12 +1442,13 @@ __device__ __forceinline__ void tb_force64_body(
   }
   __syncthreads();
   double dmax2 = 0.0;
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+  const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
+  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
     const double ebi = eband[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
-    for_each_slot(list16, ld, a, nn[a], [&](int s) {
+    auto slot = [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
       const TbDev &P = SINGLE ? sp[0] : sp[ti * GH_MAXT + ty[s]];
       const double dr = sqrt(dist2_rn(vx, vy, vz));
@@ -1439,7 +1461,16 @@ __device__ __forceinline__ void tb_force64_body(
         fz -= sf * vz;
         ei += repul;
       }
-    });
+    };
+    if (G == 1) for_each_slot(list16, ld, a, nn[a], slot);
+    else for_each_slot_g<G>(list16, ld, a, nn[a], sub, slot);
+    if (G > 1) {
+      fx = group_sum<G>(fx);
+      fy = group_sum<G>(fy);
+      fz = group_sum<G>(fz);
+      ei = group_sum<G>(ei);
+    }
+    if (sub != 0) continue;
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
     if (fv.on) {
       dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
@@ -1452,7 +1483,7 @@ __device__ __forceinline__ void tb_force64_body(
   }
   if (fv.on) block_max_atomic(dmax2, fv.maxd2);
 }
-template <bool SINGLE>
+template <bool SINGLE, int G>
 __global__ __launch_bounds__(GH_TBMAX) void k_tb_force64_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ type,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1460,14 +1491,14 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_force64_tiled(
     int nt, const double *__restrict__ rho, const double *__restrict__ eband, double ev_ui,
     double *__restrict__ frc, double *__restrict__ epa, FuseVV fv, StepArgs sa) {
   if (!halted(st))
-    tb_force64_body<SINGLE>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbd, nt, rho,
+    tb_force64_body<SINGLE, G>(st, t, pos, type, cell_start, nn, list16, ld, ncap, tbd, nt, rho,
                             eband, ev_ui, frc, epa, fv, sa.step);
 }
 
 // Lennard-Jones (lj_interact, Pairs.F90:207-279), tiled, FP64 LDS positions.
 // POT: GH_POT_LJ (eps, sigma as arguments) or a tabulated form (plj, slj,
 // wca: per type pair, types staged in LDS).
-template <bool MIXED, int POT, bool FUSED>
+template <bool MIXED, int POT, bool FUSED, int G>
 __device__ __forceinline__ void lj_tiled_body(
     GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
     const int *__restrict__ cell_start, const int *__restrict__ nn,
@@ -1505,11 +1536,12 @@ __device__ __forceinline__ void lj_tiled_body(
   __syncthreads();
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
   double dmax2 = 0.0;
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+  const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
+  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = type[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
-    for_each_slot(list16, ld, a, nn[a], [&](int s) {
+    auto slot = [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
       if (MIXED) {
         const double r2 = fma(vx, vx, fma(vy, vy, vz * vz));
@@ -1555,7 +1587,16 @@ __device__ __forceinline__ void lj_tiled_body(
           fz -= sf * vz;
         }
       }
-    });
+    };
+    if (G == 1) for_each_slot(list16, ld, a, nn[a], slot);
+    else for_each_slot_g<G>(list16, ld, a, nn[a], sub, slot);
+    if (G > 1) {
+      fx = group_sum<G>(fx);
+      fy = group_sum<G>(fy);
+      fz = group_sum<G>(fz);
+      if (!FUSED) ei = group_sum<G>(ei);
+    }
+    if (sub != 0) continue;
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
     if (FUSED) {
       dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
@@ -1569,7 +1610,7 @@ __device__ __forceinline__ void lj_tiled_body(
   if (FUSED) block_max_atomic(dmax2, fv.maxd2);
 }
 // One pass per step: gated (and publishing) as a first pass.
-template <bool MIXED, int POT, bool FUSED>
+template <bool MIXED, int POT, bool FUSED, int G>
 __global__ __launch_bounds__(GH_TBMAX) void k_lj_tiled(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
@@ -1577,7 +1618,7 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_tiled(
     double *__restrict__ epa, const int *__restrict__ type, FuseVV fv,
     const PljDev *__restrict__ pljd, StepArgs sa) {
   if (!step_gate(st, sa))
-    lj_tiled_body<MIXED, POT, FUSED>(st, t, pos, cell_start, nn, list16, ld, ncap, eps, sigma, rc2,
+    lj_tiled_body<MIXED, POT, FUSED, G>(st, t, pos, cell_start, nn, list16, ld, ncap, eps, sigma, rc2,
                               ev_ui, frc, epa, type, fv, pljd, sa.step);
 }
 
@@ -1593,7 +1634,7 @@ struct LjF {
   float sig2, e4;  // sigma^2 in steps^2, 4 eps
   float e24, e48;  // 24 eps, 48 eps
 };
-template <bool FUSED>
+template <bool FUSED, int G>
 __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
     GhStatus *st, TileGeom t, const double4 *__restrict__ pos, const int *__restrict__ cell_start,
     const int *__restrict__ nn, const unsigned short *__restrict__ list16, int ld, int ncap,
@@ -1621,7 +1662,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
   const float sig2 = in_vgpr(p.sig2), e24 = p.e24, e48 = p.e48, e4 = p.e4;
   const double scale = fxs * ev_ui;
   double dmax2 = 0.0;
-  for (int a = H.a0 + threadIdx.x; a < H.a1; a += blockDim.x) {
+  const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
+  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
@@ -1640,14 +1682,23 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
         fz = fma(-sf, dz, fz);
       }
     };
-    for_each_quad(list16, ld, a, nn[a], [&](unsigned long long w) {
+    auto quad = [&](unsigned long long w) {
       const int4 q0 = lq4[GH_QSLOT(w, 0)], q1 = lq4[GH_QSLOT(w, 1)], q2 = lq4[GH_QSLOT(w, 2)],
                  q3 = lq4[GH_QSLOT(w, 3)];
       pair(q0);
       pair(q1);
       pair(q2);
       pair(q3);
-    });
+    };
+    if (G == 1) for_each_quad(list16, ld, a, nn[a], quad);
+    else for_each_quad_g<G>(list16, ld, a, nn[a], sub, quad);
+    if (G > 1) {
+      fx = group_sum<G>(fx);
+      fy = group_sum<G>(fy);
+      fz = group_sum<G>(fz);
+      if (!FUSED) ei = group_sum<G>(ei);
+    }
+    if (sub != 0) continue;
     const double Fx = fx * scale, Fy = fy * scale, Fz = fz * scale;
     if (FUSED) {
       dmax2 = fmax(dmax2, fused_vv(fv, a, pa, type[a], Fx, Fy, Fz));
@@ -1913,10 +1964,10 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     if (mixed) {
       if (c->tbf_k != c->tg.fxk) GH_TRY(gh_upload_tbf(c, c->tg.fxk));
       const double e0 = c->tb_e0;
-#define RHO_FX(SG, SS)                                                                      \
+#define RHO_FX(SG, SS, GG)                                                                  \
   do {                                                                                      \
     const size_t lds = SS ? 0 : lds1;                                                        \
-    k_tb_rho_fx<SG, SS><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->type,      \
+    k_tb_rho_fx<SG, SS, GG><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->type,  \
                                                      c->cell_start, c->nn, c->list16, ld,    \
                                                      c->tbf, rho1, c->eband32, e0, c->tbf0,  \
                                                      sa);                                    \
@@ -1925,24 +1976,25 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 #define RHO_S(SG)                                                                           \
   do {                                                                                      \
     const int need = c->tg.slot_cap + 1;                                                    \
-    if (need <= 768) RHO_FX(SG, 768);                                                       \
-    else if (need <= 1024) RHO_FX(SG, 1024);                                                \
-    else if (need <= 1280) RHO_FX(SG, 1280);                                                \
-    else if (need <= 1536) RHO_FX(SG, 1536);                                                \
-    else if (need <= 1792) RHO_FX(SG, 1792);                                                \
-    else if (need <= 2048) RHO_FX(SG, 2048);                                                \
-    else if (need <= 2560) RHO_FX(SG, 2560);                                                \
-    else if (need <= 3072) RHO_FX(SG, 3072);                                                \
-    else RHO_FX(SG, 0);                                                                     \
+    if (c->tg.lanes == 4) RHO_FX(SG, 0, 4);                                                 \
+    else if (need <= 768) RHO_FX(SG, 768, 1);                                               \
+    else if (need <= 1024) RHO_FX(SG, 1024, 1);                                             \
+    else if (need <= 1280) RHO_FX(SG, 1280, 1);                                             \
+    else if (need <= 1536) RHO_FX(SG, 1536, 1);                                             \
+    else if (need <= 1792) RHO_FX(SG, 1792, 1);                                             \
+    else if (need <= 2048) RHO_FX(SG, 2048, 1);                                             \
+    else if (need <= 2560) RHO_FX(SG, 2560, 1);                                             \
+    else if (need <= 3072) RHO_FX(SG, 3072, 1);                                             \
+    else RHO_FX(SG, 0, 1);                                                                  \
   } while (0)
       if (single) RHO_S(true);
       else RHO_S(false);
 #undef RHO_S
 #undef RHO_FX
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
-#define TB_FX3(SG, FU, SS)                                                                  \
+#define TB_FX3(SG, FU, SS, GG)                                                              \
   do {                                                                                      \
-    k_tb_force_fx<SG, FU, SS><<<grid, c->tg.tpb, SS ? 0 : lds2, s>>>(                        \
+    k_tb_force_fx<SG, FU, SS, GG><<<grid, c->tg.tpb, SS ? 0 : lds2, s>>>(                    \
         c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,    \
         c->tbf, c->rho, c->eband32, ev_ui, c->frc, c->epa, fv, c->tbf0, sa);                \
     ++c->launches;                                                                          \
@@ -1950,15 +2002,16 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 #define TB_FX(SG, FU)                                                                       \
   do {                                                                                      \
     const int need = c->tg.slot_cap + 1;                                                    \
-    if (need <= 768) TB_FX3(SG, FU, 768);                                                   \
-    else if (need <= 1024) TB_FX3(SG, FU, 1024);                                            \
-    else if (need <= 1280) TB_FX3(SG, FU, 1280);                                            \
-    else if (need <= 1536) TB_FX3(SG, FU, 1536);                                            \
-    else if (need <= 1792) TB_FX3(SG, FU, 1792);                                            \
-    else if (need <= 2048) TB_FX3(SG, FU, 2048);                                            \
-    else if (need <= 2560) TB_FX3(SG, FU, 2560);                                            \
-    else if (need <= 3072) TB_FX3(SG, FU, 3072);                                            \
-    else TB_FX3(SG, FU, 0);                                                                 \
+    if (c->tg.lanes == 4) TB_FX3(SG, FU, 0, 4);                                             \
+    else if (need <= 768) TB_FX3(SG, FU, 768, 1);                                           \
+    else if (need <= 1024) TB_FX3(SG, FU, 1024, 1);                                         \
+    else if (need <= 1280) TB_FX3(SG, FU, 1280, 1);                                         \
+    else if (need <= 1536) TB_FX3(SG, FU, 1536, 1);                                         \
+    else if (need <= 1792) TB_FX3(SG, FU, 1792, 1);                                         \
+    else if (need <= 2048) TB_FX3(SG, FU, 2048, 1);                                         \
+    else if (need <= 2560) TB_FX3(SG, FU, 2560, 1);                                         \
+    else if (need <= 3072) TB_FX3(SG, FU, 3072, 1);                                         \
+    else TB_FX3(SG, FU, 0, 1);                                                              \
   } while (0)
       if (single) {
         if (fuse_vv) TB_FX(true, true);
@@ -1970,38 +2023,52 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
 #undef TB_FX
 #undef TB_FX3
     } else {
-      if (single)
-        { k_tb_rho64_tiled<true><<<grid, c->tg.tpb, lds1, s>>>(
-            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, rho1, c->eband, c->eband32, sa); ++c->launches; }
-      else
-        { k_tb_rho64_tiled<false><<<grid, c->tg.tpb, lds1, s>>>(
-            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd,
-            c->nt, rho1, c->eband, c->eband32, sa); ++c->launches; }
+#define RHO64(SG, GG)                                                                       \
+  k_tb_rho64_tiled<SG, GG><<<grid, c->tg.tpb, lds1, s>>>(                                    \
+      c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->tbd, c->nt, \
+      rho1, c->eband, c->eband32, sa)
+#define FORCE64(SG, GG)                                                                     \
+  k_tb_force64_tiled<SG, GG><<<grid, c->tg.tpb, lds2, s>>>(                                  \
+      c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,       \
+      c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa)
+      const bool g4 = c->tg.lanes == 4;
+      if (single) {
+        if (g4) RHO64(true, 4);
+        else RHO64(true, 1);
+      } else {
+        if (g4) RHO64(false, 4);
+        else RHO64(false, 1);
+      }
+      ++c->launches;
       if (mid) GH_HIP(c, hipEventRecord(mid, s));
-      if (single)
-        { k_tb_force64_tiled<true><<<grid, c->tg.tpb, lds2, s>>>(
-            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa); ++c->launches; }
-      else
-        { k_tb_force64_tiled<false><<<grid, c->tg.tpb, lds2, s>>>(
-            c->status, c->tg, c->pos, c->type, c->cell_start, c->nn, c->list16, ld, c->ncap,
-            c->tbd, c->nt, c->rho, c->eband, ev_ui, c->frc, c->epa, fv, sa); ++c->launches; }
+      if (single) {
+        if (g4) FORCE64(true, 4);
+        else FORCE64(true, 1);
+      } else {
+        if (g4) FORCE64(false, 4);
+        else FORCE64(false, 1);
+      }
+      ++c->launches;
+#undef RHO64
+#undef FORCE64
     }
   } else if (c->pot == GH_POT_LJ || gh_is_table_pot(c->pot)) {
     if (mid) GH_HIP(c, hipEventRecord(mid, s));
     const double rc2 = c->lj_rc * c->lj_rc;
     const size_t lds = gh_pair_slot_bytes(c->pot, c->precision) * slots;
+#define LJ_LAUNCH4(MX, PT, FU, GG)                                                          \
+  k_lj_tiled<MX, PT, FU, GG><<<grid, c->tg.tpb, lds, s>>>(                                   \
+      c->status, c->tg, c->pos, c->cell_start, c->nn, c->list16, ld, c->ncap, c->lj_eps,     \
+      c->lj_sigma, rc2, ev_ui, c->frc, c->epa, c->type, fv, c->pljd, sa)
 #define LJ_LAUNCH(MX, PT)                                                                   \
   do {                                                                                      \
-    if (fuse_vv)                                                                            \
-      k_lj_tiled<MX, PT, true><<<grid, c->tg.tpb, lds, s>>>(                                \
-          c->status, c->tg, c->pos, c->cell_start, c->nn, c->list16, ld, c->ncap, c->lj_eps, \
-          c->lj_sigma, rc2, ev_ui, c->frc, c->epa, c->type, fv, c->pljd, sa);               \
-    else                                                                                    \
-      k_lj_tiled<MX, PT, false><<<grid, c->tg.tpb, lds, s>>>(                               \
-          c->status, c->tg, c->pos, c->cell_start, c->nn, c->list16, ld, c->ncap, c->lj_eps, \
-          c->lj_sigma, rc2, ev_ui, c->frc, c->epa, c->type, fv, c->pljd, sa);               \
+    if (c->tg.lanes == 4) {                                                                 \
+      if (fuse_vv) LJ_LAUNCH4(MX, PT, true, 4);                                             \
+      else LJ_LAUNCH4(MX, PT, false, 4);                                                    \
+    } else {                                                                                \
+      if (fuse_vv) LJ_LAUNCH4(MX, PT, true, 1);                                             \
+      else LJ_LAUNCH4(MX, PT, false, 1);                                                    \
+    }                                                                                       \
     ++c->launches;                                                                          \
   } while (0)
 #define LJ_POTS(MX)                                                                         \
@@ -2020,14 +2087,18 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
       p.e4 = (float)(4.0 * c->lj_eps);
       p.e24 = (float)(24.0 * c->lj_eps);
       p.e48 = (float)(48.0 * c->lj_eps);
-      if (fuse_vv)
-        k_lj_fx<true><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
-                                                   c->nn, c->list16, ld, c->ncap, p, ev_ui,
-                                                   c->frc, c->epa, c->type, fv, sa);
-      else
-        k_lj_fx<false><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start,
-                                                    c->nn, c->list16, ld, c->ncap, p, ev_ui,
-                                                    c->frc, c->epa, c->type, fv, sa);
+#define LJFX(FU, GG)                                                                        \
+  k_lj_fx<FU, GG><<<grid, c->tg.tpb, lds, s>>>(c->status, c->tg, c->pos, c->cell_start, c->nn, \
+                                             c->list16, ld, c->ncap, p, ev_ui, c->frc,       \
+                                             c->epa, c->type, fv, sa)
+      if (c->tg.lanes == 4) {
+        if (fuse_vv) LJFX(true, 4);
+        else LJFX(false, 4);
+      } else {
+        if (fuse_vv) LJFX(true, 1);
+        else LJFX(false, 1);
+      }
+#undef LJFX
       ++c->launches;
     } else if (mixed) {
       LJ_POTS(true)
@@ -2036,6 +2107,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
     }
 #undef LJ_POTS
 #undef LJ_LAUNCH
+#undef LJ_LAUNCH4
   } else {
     return gh_fail(c, GH_ESTATE, "no potential set");
   }
