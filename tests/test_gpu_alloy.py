@@ -1,13 +1,16 @@
-"""Two-type SMA-TB in a periodic box: the tiled passes with per-type-pair
-parameters (the multi-type kernels, types staged in LDS) against the oracle,
-at a size that runs four lanes per atom (4,000 atoms) and at one that runs
-one lane per atom (131,072 atoms).  Bars as in test_gpu_parity.py: FP64
-forces and per-atom energies within 1e-10, mixed max|dF|/rms|F| <= 1e-5.
+"""The tiled passes at both lane counts against the oracle.
 
-The alloy is an fcc Ag-Cu solid solution (a = 3.87 A, the mean of the two
-lattice constants), 50:50 labels from a seeded generator, 0.05 A Gaussian
-jitter; parameters are the reference's Ag/Cu/AgCu rows with internal types
-in order of first appearance (Ag first).
+* Two-type SMA-TB in a periodic box (the multi-type kernels, per-type-pair
+  parameters, types staged in LDS) at 4,000 atoms (four lanes per atom) and
+  131,072 atoms (one lane per atom): an fcc Ag-Cu solid solution (a = 3.87 A,
+  the mean of the two lattice constants), 50:50 labels from a seeded
+  generator, 0.05 A Gaussian jitter; the reference's Ag/Cu/AgCu rows with
+  internal types in order of first appearance (Ag first).
+* LJ at 131,072 atoms (one lane per atom; the 4,000-atom fixtures of
+  test_gpu_parity.py run four).
+
+Bars as in test_gpu_parity.py: FP64 forces (and per-atom energies) within
+1e-10, mixed max|dF|/rms|F| <= 1e-5.
 """
 import numpy as np
 import pytest
@@ -57,5 +60,36 @@ def test_alloy_forces_match_oracle(ncell):
             else:
                 err_max = np.max(np.abs(F - Fo)) / rms
                 print("alloy %d mixed max|dF|/rms %.3e" % (len(x), err_max))
+                assert err_max <= 1e-5
+                assert et == pytest.approx(epo, rel=1e-7)
+
+
+def test_lj_one_lane_size_matches_oracle():
+    # the LJ passes at one lane per atom (131,072 atoms; the 4,000-atom
+    # fixtures of test_gpu_parity.py run four): fcc 32^3 at rho* 0.8442 with
+    # 0.05 sigma jitter, rc 2.5 sigma, skin 0.3 sigma
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    x = cm.fcc(32, a)
+    x += np.random.default_rng(9).normal(0.0, 0.05, x.shape)
+    L = np.array([32 * a] * 3)
+    box = O.make_box(L, [1, 1, 1])
+    nn, lst, _ = O.neighbours(x, box, 2.5 + 0.3)
+    Fo, eo, epo = O.lj_force(x, box, 1.0, 1.0, 2.5, nn, lst)
+    rms = np.sqrt(np.mean(Fo ** 2))
+    for prec in (hip.GH_FP64, hip.GH_MIXED):
+        with hip.Context(0, prec) as c:
+            c.set_box(L, [1, 1, 1])
+            c.set_atoms(x, np.zeros(len(x), np.int32))
+            c.set_masses([1.0])
+            c.set_lj(1.0, 1.0, 2.5)
+            c.set_skin(0.3)
+            F, e, et = c.compute()
+            assert c.stats()["list_mode"] == 1
+            if prec == hip.GH_FP64:
+                assert cm.force_err(F, Fo) <= 1e-10
+                assert et == pytest.approx(epo, rel=1e-12)
+            else:
+                err_max = np.max(np.abs(F - Fo)) / rms
+                print("lj %d mixed max|dF|/rms %.3e" % (len(x), err_max))
                 assert err_max <= 1e-5
                 assert et == pytest.approx(epo, rel=1e-7)
