@@ -872,10 +872,15 @@ __global__ __launch_bounds__(GH_CBLOCK) void k_build_coop(GhStatus *st,
         if (lane >= o) inc += y;
       }
       const int nfit = max(__popcll(__ballot(X < X1 && inc <= A.ncand)), 1);
-      if (lane < nfit) coff[lane + 1] = inc;
-      if (lane <= nfit) cst[lane] = R.start[H.key0 + xb + lane];  // the cells' first atoms
+      // offsets of the batch's lists, and its cells' first atoms up to the
+      // one past its last cell (cst[nfit], which lane 0 writes: nfit may be 64)
+      if (lane < nfit) {
+        coff[lane + 1] = inc;
+        cst[lane] = R.start[H.key0 + xb + lane];
+      }
       if (lane == 0) {
         coff[0] = 0;
+        cst[nfit] = R.start[H.key0 + xb + nfit];
         bend = xb + nfit;
       }
     }

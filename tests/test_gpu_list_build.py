@@ -144,8 +144,11 @@ def _slab_and_cluster():
     return {"slab": slab, "cluster": cluster}
 
 
-@pytest.mark.parametrize("case", ["slab", "cluster"])
-def test_cooperative_rows_match_slab_and_sparse_box(case):
+@pytest.mark.parametrize("case,tile", [("slab", None), ("cluster", None), ("cluster", (4, 4, 4))],
+                         ids=["slab", "cluster", "cluster-4x4x4"])
+def test_cooperative_rows_match_slab_and_sparse_box(case, tile):
+    # 4 x 4 x 4-cell bricks at the cluster's surface are mostly empty, so all
+    # 64 cells' candidate lists fit one batch (the batch bounds at their limit)
     x, L, pbc = _slab_and_cluster()[case]
     t = np.zeros(len(x), np.int32)
     for prec in (hip.GH_FP64, hip.GH_MIXED):
@@ -161,6 +164,8 @@ def test_cooperative_rows_match_slab_and_sparse_box(case):
             c.set_masses([cm.MASS_AU])
             c.set_smatb(1, *cm.matrices(cm.au_params(), 1))
             c.set_skin(1.0)
+            if tile is not None:
+                c.set_tile(np.array(tile))
             cs.append(c)
         res = [c.compute() for c in cs]
         assert cs[0].stats()["list_mode"] == 1  # the tiled cell path
