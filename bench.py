@@ -367,11 +367,12 @@ def lj_bulk(hip, nx, steps, warmup=10, jitter=0.0, Tstar=0.72, tiles=None):
         F[name] = c.compute()[0]
         dt, _, nupd, launches = timed_run(c, steps, warmup)
         _, tm, _, _ = timed_run(c, steps, 0, timing=True)
+        wpb = c.stats()["chunks"]
         c.close()
         out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
                      "rebuilds": nupd, "launches_per_step": launches / steps,
                      "force_pass_us": tm["force"][0] / max(tm["force"][1], 1) * 1e3,
-                     "brick_cells": shape}
+                     "brick_cells": shape, "workgroups_per_brick": wpb}
     # the fused pass moves x 24 + v 48 + the next x 24 = 96 algorithmic B/atom
     t_us = out["mixed"]["force_pass_us"]
     ach = 96.0 * len(x) / (t_us * 1e-6) / 1e9
@@ -398,10 +399,11 @@ def au_32k(hip, steps=2000, warmup=20):
         c = make_ctx(hip, prec, x, L, 1.0)
         shape, _ = c.autotune(reps=3)
         dt, _, nupd, launches = timed_run(c, steps, warmup)
+        wpb = c.stats()["chunks"]
         c.close()
         out[name] = {"atom_steps_per_s": len(x) * steps / dt, "ms_per_step": dt / steps * 1e3,
                      "rebuilds": nupd, "launches_per_step": launches / steps,
-                     "brick_cells": shape}
+                     "brick_cells": shape, "workgroups_per_brick": wpb}
     return out
 
 

@@ -37,7 +37,7 @@ module gems_hip
     integer(c_int32_t) :: maxnn, cap, ncells(3), list_mode
     integer(c_int64_t) :: steps
     integer(c_int64_t) :: launches
-    integer(c_int32_t) :: list_build
+    integer(c_int32_t) :: list_build, chunks
   end type
 
   interface

@@ -248,11 +248,16 @@ static int force_pass_threads(const TileGeom &t, int atoms) {
 // atom) four lanes share each atom's row, so a pass has four times the
 // wavefronts to hide its latencies; the lane count depends on the atom count
 // alone, which fixes the order of every FP64 sum whatever the brick shape.
-static void set_force_pass_threads(gh_ctx *c, int max_brick) {
+// With t.chunks workgroups per brick, each takes a contiguous share of the
+// brick's atoms (the halo staged by each).
+static void set_force_pass_threads(gh_ctx *c) {
   TileGeom &t = c->tg;
   t.lanes = c->n < 131072 ? 4 : 1;
-  t.tpb_max = force_pass_threads(t, t.lanes * (max_brick + max_brick / 16 + 1));
-  t.tpb_avg = force_pass_threads(t, t.lanes * (int)((c->n + t.nbrick - 1) / t.nbrick));
+  t.chunks = std::max(1, c->chunks_req);
+  const int mb = (c->max_brick + t.chunks - 1) / t.chunks;
+  const long long units = (long long)t.nbrick * t.chunks;
+  t.tpb_max = force_pass_threads(t, t.lanes * (mb + mb / 16 + 1));
+  t.tpb_avg = force_pass_threads(t, t.lanes * (int)((c->n + units - 1) / units));
   t.tpb = c->tpb_mean ? t.tpb_avg : t.tpb_max;
 }
 
@@ -386,7 +391,8 @@ static int ensure_ready(gh_ctx *c) {
       slot_cap = roundup(slot_cap, 32);
       if (slot_cap <= max_slots(c) && !h.tile_overflow) {
         c->tg.slot_cap = slot_cap;
-        set_force_pass_threads(c, max_brick);
+        c->max_brick = max_brick;
+        set_force_pass_threads(c);
         maxnn = h.maxnn;
         break;
       }
@@ -626,6 +632,9 @@ int gh_create(gh_ctx **out, int device, int precision) {
   // choice of coop_args
   const char *lb = getenv("GEMS_HIP_LIST_BUILD");
   c->list_rows = !lb ? 0 : strcmp(lb, "rows") == 0 ? 1 : strcmp(lb, "coop") == 0 ? 2 : 0;
+  // force-pass workgroups per brick, 1..4 (the autotuner then keeps it)
+  if (const char *ck = getenv("GEMS_HIP_CHUNKS")) c->chunks_env = std::min(4, std::max(0, atoi(ck)));
+  c->chunks_req = c->chunks_env;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     delete c;
@@ -683,6 +692,7 @@ int gh_set_tile(gh_ctx *c, const int *bd) {
     if (bd && (bd[d] < 0 || bd[d] > 4)) return gh_fail(c, GH_EINVAL, "brick edges must be 0..4");
   for (int d = 0; d < 3; ++d) c->bd_req[d] = bd ? bd[d] : 0;
   c->tpb_mean = 0;
+  c->chunks_req = c->chunks_env;
   c->ready = false;
   return 0;
 }
@@ -722,7 +732,7 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
   hipEvent_t e0, e1;
   GH_HIP(c, hipEventCreate(&e0));
   GH_HIP(c, hipEventCreate(&e1));
-  int best[3] = {0, 0, 0}, best_mean = 0;
+  int best[3] = {0, 0, 0}, best_mean = 0, best_chunks = 1;
   double best_us = 1e30, best_score = 1e30;
   for (int bz = 1; bz <= 4; ++bz)
     for (int by = bz; by <= 4; ++by)
@@ -751,24 +761,35 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
           if (check_status(c) != 0) continue;
           chain_us = 1e3 * ms;
         }
-        const int cand[2] = {c->tg.tpb_max, c->tg.tpb_avg};
-        for (int m = 0; m < 2; ++m) {
-          if (m == 1 && cand[1] == cand[0]) break;
-          c->tg.tpb = cand[m];
-          GH_TRY(gh_launch_forces(c, nullptr));
-          GH_HIP(c, hipEventRecord(e0, c->stream));
-          for (int r = 0; r < reps; ++r) GH_TRY(gh_launch_forces(c, nullptr));
-          GH_HIP(c, hipEventRecord(e1, c->stream));
-          GH_HIP(c, hipEventSynchronize(e1));
-          float ms = 0.f;
-          GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-          if (check_status(c) != 0) break;
-          const double us = 1e3 * ms / reps;
-          if (us + rate * chain_us < best_score) {
-            best_score = us + rate * chain_us;
-            best_us = us;
-            best_mean = m;
-            for (int d = 0; d < 3; ++d) best[d] = b[d];
+        // workgroups per brick: 1, and 2 and 4 where the bricks are too few
+        // to fill the chip (the grid stays within two workgroups per CU);
+        // GEMS_HIP_CHUNKS fixes it
+        for (int ki = 0; ki < 3; ++ki) {
+          const int k = c->chunks_env ? c->chunks_env : 1 << ki;
+          if (c->chunks_env && ki > 0) break;
+          if (k > 1 && !c->chunks_env && c->tg.nbrick * k > 512) continue;
+          c->chunks_req = k;
+          set_force_pass_threads(c);
+          const int cand[2] = {c->tg.tpb_max, c->tg.tpb_avg};
+          for (int m = 0; m < 2; ++m) {
+            if (m == 1 && cand[1] == cand[0]) break;
+            c->tg.tpb = cand[m];
+            GH_TRY(gh_launch_forces(c, nullptr));
+            GH_HIP(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; ++r) GH_TRY(gh_launch_forces(c, nullptr));
+            GH_HIP(c, hipEventRecord(e1, c->stream));
+            GH_HIP(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            GH_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+            if (check_status(c) != 0) break;
+            const double us = 1e3 * ms / reps;
+            if (us + rate * chain_us < best_score) {
+              best_score = us + rate * chain_us;
+              best_us = us;
+              best_mean = m;
+              best_chunks = k;
+              for (int d = 0; d < 3; ++d) best[d] = b[d];
+            }
           }
         }
       }
@@ -776,6 +797,7 @@ int gh_autotune(gh_ctx *c, int reps, int *bd_out, double *us_out) {
   (void)hipEventDestroy(e1);
   for (int d = 0; d < 3; ++d) c->bd_req[d] = best[d];
   c->tpb_mean = best_mean;
+  c->chunks_req = best_chunks;
   c->ready = false;
   c->err.clear();
   GH_TRY(ensure_ready(c));
@@ -1665,6 +1687,7 @@ int gh_get_stats(gh_ctx *c, gh_stats *s) {
   s->steps = c->step;
   s->launches = c->launches;
   s->list_build = c->list_build;
+  s->chunks = c->list_mode == 1 ? c->tg.chunks : 0;
   return 0;
 }
 

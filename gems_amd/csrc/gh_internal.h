@@ -123,6 +123,7 @@ struct TileGeom {
   int tpb;        // threads per workgroup of the force passes (a multiple of 64)
   int tpb_max, tpb_avg;  // its two candidates: the fullest brick, the mean brick
   int lanes = 1;  // lanes per atom in the force passes: 4 below 131072 atoms (fixed by n)
+  int chunks = 1; // workgroups per brick in the force passes (each a share of its atoms)
   int fxk;        // fixed-point steps per Angstrom: 2^fxk (mixed SMA-TB)
   double fxs;     // 2^fxk
   double L[3];
@@ -220,6 +221,9 @@ struct gh_ctx {
   TileGeom tg;                // brick grouping (cell mode)
   int bd_req[3] = {0, 0, 0};  // requested brick edge (0 = automatic)
   int tpb_mean = 0;           // 1: force-pass workgroups sized to the mean brick (autotuned)
+  int chunks_req = 0;         // force-pass workgroups per brick (autotuned; 0 = 1)
+  int chunks_env = 0;         // GEMS_HIP_CHUNKS: forced workgroups per brick (0 = free)
+  int max_brick = 0;          // atoms of the fullest brick at the last sizing
   int list_rows = 0;          // GEMS_HIP_LIST_BUILD: 1 = rows (one thread per atom), 2 = coop, 0 auto
   int list_build = 0;         // kernel of the last enqueued build: 1 per-thread, 2 cooperative
   int cap = 0;                // list row capacity

@@ -1106,6 +1106,22 @@ __global__ __launch_bounds__(GH_TBLOCK) void k_tiled_to_global(
     }
 }
 
+// Brick of a force-pass workgroup: t.chunks consecutive workgroups (one
+// XCD's) share a brick, each staging its halo and taking the contiguous share
+// kc of its atoms (chunk_range).
+__device__ __forceinline__ int pass_brick(const TileGeom &t, int &kc) {
+  const int lin = xcd_block();
+  const int b = lin / t.chunks;
+  kc = lin - b * t.chunks;
+  return b;
+}
+__device__ __forceinline__ void chunk_range(const TileGeom &t, const Halo &H, int kc, int &a0,
+                                            int &a1) {
+  const int na = H.a1 - H.a0;
+  a0 = H.a0 + na * kc / t.chunks;
+  a1 = H.a0 + na * (kc + 1) / t.chunks;
+}
+
 // Per-step arguments of the force passes.  The first pass of a step runs
 // only if the run is not halted and the step's rebuild is not due (when the
 // host has not enqueued the rebuild chain for the step, a due rebuild stops
@@ -1170,7 +1186,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
   if (!SINGLE)
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) sp[k] = tbf[k];
-  const int b = xcd_block();
+  int kc;  // this workgroup's share of the brick's atoms
+  const int b = pass_brick(t, kc);
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
@@ -1196,7 +1213,9 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho_fx(
   __syncthreads();
   const TbF pv = tbf_vgpr(p1);
   const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
-  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
+  int ca0, ca1;
+  chunk_range(t, H, kc, ca0, ca1);
+  for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
@@ -1255,7 +1274,8 @@ __device__ __forceinline__ void tb_force_fx_body(
   __shared__ TbF sp[SINGLE ? 1 : GH_MAXT * GH_MAXT];  // one type: the parameters come in p1
   if (!SINGLE)
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) sp[k] = tbf[k];
-  const int b = xcd_block();
+  int kc;  // this workgroup's share of the brick's atoms
+  const int b = pass_brick(t, kc);
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
@@ -1282,7 +1302,9 @@ __device__ __forceinline__ void tb_force_fx_body(
   double dmax2 = 0.0;
   const TbF pv = tbf_vgpr(p1);
   const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
-  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
+  int ca0, ca1;
+  chunk_range(t, H, kc, ca0, ca1);
+  for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
@@ -1363,7 +1385,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho64_tiled(
   } else {
     load_params(sp, tbd, nt);
   }
-  const int b = xcd_block();
+  int kc;  // this workgroup's share of the brick's atoms
+  const int b = pass_brick(t, kc);
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
@@ -1386,7 +1409,9 @@ __global__ __launch_bounds__(GH_TBMAX) void k_tb_rho64_tiled(
   }
   __syncthreads();
   const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
-  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
+  int ca0, ca1;
+  chunk_range(t, H, kc, ca0, ca1);
+  for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
     double acc = 0.0;
@@ -1422,7 +1447,8 @@ __device__ __forceinline__ void tb_force64_body(
   } else {
     load_params(sp, tbd, nt);
   }
-  const int b = xcd_block();
+  int kc;  // this workgroup's share of the brick's atoms
+  const int b = pass_brick(t, kc);
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
@@ -1448,7 +1474,9 @@ __device__ __forceinline__ void tb_force64_body(
   __syncthreads();
   double dmax2 = 0.0;
   const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
-  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
+  int ca0, ca1;
+  chunk_range(t, H, kc, ca0, ca1);
+  for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = SINGLE ? 0 : type[a];
     const double ebi = eband[a];
@@ -1517,7 +1545,8 @@ __device__ __forceinline__ void lj_tiled_body(
   __shared__ PljDev pp[TAB ? GH_MAXT * GH_MAXT : 1];
   if (TAB)
     for (int k = threadIdx.x; k < GH_MAXT * GH_MAXT; k += blockDim.x) pp[k] = pljd[k];
-  const int b = xcd_block();
+  int kc;  // this workgroup's share of the brick's atoms
+  const int b = pass_brick(t, kc);
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, step)) return;
   const int cap = t.slot_cap + 1;  // slots + the sentinel
@@ -1542,7 +1571,9 @@ __device__ __forceinline__ void lj_tiled_body(
   const float epsf = (float)eps, sig2f = (float)(sigma * sigma);
   double dmax2 = 0.0;
   const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
-  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
+  int ca0, ca1;
+  chunk_range(t, H, kc, ca0, ca1);
+  for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = type[a];
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
@@ -1648,7 +1679,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
   if (step_gate(st, sa)) return;
   extern __shared__ int4 lq4[];
   __shared__ Halo H;
-  const int b = xcd_block();
+  int kc;  // this workgroup's share of the brick's atoms
+  const int b = pass_brick(t, kc);
   if (b >= t.nbrick) return;
   if (!halo_setup(t, b, cell_start, H, st, true, sa.step)) return;
   const int ns = t.slot_cap + 1;  // slots + the sentinel
@@ -1668,7 +1700,9 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
   const double scale = fxs * ev_ui;
   double dmax2 = 0.0;
   const int sub = (int)(threadIdx.x & (G - 1));  // G lanes per atom (small systems)
-  for (int a = H.a0 + (int)(threadIdx.x / G); a < H.a1; a += (int)(blockDim.x / G)) {
+  int ca0, ca1;
+  chunk_range(t, H, kc, ca0, ca1);
+  for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
@@ -1904,9 +1938,14 @@ int gh_launch_rebuild_cells(gh_ctx *c) {
   { k_cell_rank_gather<<<ga, GH_TBLOCK, 0, s>>>(c->status, C, c->pos); ++c->launches; }
   c->list_build = coop ? 2 : 1;
   if (coop) {
-    // about 24 atoms per wavefront of a workgroup's mean share, 4 to 16 wavefronts
-    const long long share = (long long)c->n / ((long long)c->tg.nbrick * h.A.chunks) + 1;
-    const int waves = (int)std::min<long long>(GH_CBLOCK / 64, std::max<long long>(4, (share + 23) / 24));
+    // 4 to 16 wavefronts: about 24 atoms each of a workgroup's mean share
+    // where the workgroups fill the CUs, about two where they do not (each
+    // CU then holds one workgroup, whose rows wait on their latencies)
+    const long long units = (long long)c->tg.nbrick * h.A.chunks;
+    const long long share = (long long)c->n / units + 1;
+    const int apw = units < 256 ? 2 : 24;
+    const int waves = (int)std::min<long long>(GH_CBLOCK / 64,
+                                               std::max<long long>(4, (share + apw - 1) / apw));
     const int grid = xgrid(c->tg.nbrick * h.A.chunks);
     k_build_coop<<<grid, 64 * waves, lds_coop, s>>>(c->status, C, c->pos, c->step);
   } else {
@@ -1954,7 +1993,7 @@ int gh_launch_forces_tiled(gh_ctx *c, hipEvent_t mid, bool fuse_vv) {
   fv.pos_next = c->pos_next;
   fv.maxd2 = gh_maxd2(c);
   fv.ncap = c->ncap;
-  const int grid = xgrid(c->tg.nbrick);
+  const int grid = xgrid(c->tg.nbrick * c->tg.chunks);
   const int ld = c->ncap;
   const double ev_ui = ghc::ev_ui();
   const bool single = c->nt == 1;

@@ -43,7 +43,8 @@ class GhError(RuntimeError):
 class Stats(C.Structure):
     _fields_ = [("nupd", C.c_int64), ("maxnn", C.c_int32), ("cap", C.c_int32),
                 ("ncells", C.c_int32 * 3), ("list_mode", C.c_int32), ("steps", C.c_int64),
-                ("launches", C.c_int64), ("list_build", C.c_int32)]
+                ("launches", C.c_int64), ("list_build", C.c_int32),
+                ("chunks", C.c_int32)]
 
 
 _lib = None
@@ -332,7 +333,7 @@ class Context:
         self._ck(self.L.gh_get_stats(self.h, C.byref(s)))
         return dict(nupd=s.nupd, maxnn=s.maxnn, cap=s.cap, ncells=tuple(s.ncells),
                     list_mode=s.list_mode, steps=s.steps, launches=s.launches,
-                    list_build=s.list_build)
+                    list_build=s.list_build, chunks=s.chunks)
 
     def set_timing(self, on=True):
         self._ck(self.L.gh_set_timing(self.h, int(on)))

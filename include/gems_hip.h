@@ -157,10 +157,16 @@ int gh_set_skin(gh_ctx *ctx, double skin);
 /* Cell-tile shape of the tiled passes.  The reference fixes only the cell
  * grid (intergroup_setcells, Neighbour.F90:587-664); the device groups cells
  * into bricks of bd[0] x bd[1] x bd[2] cells (1..4 each, 0 = automatic), one
- * workgroup per brick.  No counterpart in the reference. */
+ * workgroup per brick (or per share of a brick's atoms where gh_autotune
+ * chose several; setting the shape resets that to one).  No counterpart in
+ * the reference. */
 int gh_set_tile(gh_ctx *ctx, const int bd[3]);
 /* Autotuner: times `reps` force evaluations for each brick shape whose halo
- * fits the LDS budget and keeps the fastest.  With velocities and a time step
+ * fits the LDS budget and keeps the fastest; where the bricks are too few to
+ * fill the chip it also tries two and four workgroups per brick, each
+ * staging the brick's halo and taking a share of its atoms (results do not
+ * depend on it; GEMS_HIP_CHUNKS=1|2|3|4 at context creation fixes the
+ * count).  With velocities and a time step
  * set, each shape's time is that of a force evaluation plus its rebuild
  * chain times the expected rebuilds per step (2 v_max dt / skin, at most 1).
  * Writes the chosen shape and its time per force evaluation (us); bd_out =
@@ -245,6 +251,8 @@ typedef struct {
   int64_t launches;  /* kernels the context has enqueued (no counterpart) */
   int32_t list_build; /* kernel of the last enqueued list build: 0 none, 1 one
                          thread per atom, 2 cooperative (no counterpart) */
+  int32_t chunks;     /* workgroups per brick of the force passes, each taking
+                         a share of its atoms (cell mode; no counterpart) */
 } gh_stats;
 int gh_get_stats(gh_ctx *ctx, gh_stats *s);
 
