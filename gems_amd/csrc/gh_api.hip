@@ -239,6 +239,9 @@ static int max_slots(const gh_ctx *c) {
 // a few bricks run a short second round).  Rounded up to whole wavefronts,
 // at least one thread per halo cell (halo_setup), at most 1024.  The
 // autotuner times both; the fullest-brick rule is the default.
+#ifndef GH_STAGE_ROUNDS
+#define GH_STAGE_ROUNDS 3
+#endif
 static int force_pass_threads(const TileGeom &t, int atoms) {
   const int nh = (t.bd[0] + 2) * (t.bd[1] + 2) * (t.bd[2] + 2);
   const int need = std::max(nh, atoms);
@@ -249,15 +252,18 @@ static int force_pass_threads(const TileGeom &t, int atoms) {
 // wavefronts to hide its latencies; the lane count depends on the atom count
 // alone, which fixes the order of every FP64 sum whatever the brick shape.
 // With t.chunks workgroups per brick, each takes a contiguous share of the
-// brick's atoms (the halo staged by each).
+// brick's atoms and stages the whole halo: it gets at least a third as many
+// threads as halo slots, so that the staging takes at most three rounds of
+// loads (the atom loop's spare threads return at once).
 static void set_force_pass_threads(gh_ctx *c) {
   TileGeom &t = c->tg;
   t.lanes = c->n < 131072 ? 4 : 1;
   t.chunks = std::max(1, c->chunks_req);
   const int mb = (c->max_brick + t.chunks - 1) / t.chunks;
   const long long units = (long long)t.nbrick * t.chunks;
-  t.tpb_max = force_pass_threads(t, t.lanes * (mb + mb / 16 + 1));
-  t.tpb_avg = force_pass_threads(t, t.lanes * (int)((c->n + units - 1) / units));
+  const int stage = t.chunks > 1 ? (t.slot_cap + GH_STAGE_ROUNDS - 1) / GH_STAGE_ROUNDS : 0;
+  t.tpb_max = force_pass_threads(t, std::max(stage, t.lanes * (mb + mb / 16 + 1)));
+  t.tpb_avg = force_pass_threads(t, std::max(stage, t.lanes * (int)((c->n + units - 1) / units)));
   t.tpb = c->tpb_mean ? t.tpb_avg : t.tpb_max;
 }
 

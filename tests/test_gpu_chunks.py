@@ -117,3 +117,27 @@ def test_workgroups_per_brick_one_lane(pot):
                 assert c.stats()["chunks"] == k
         _same(res[0][:2], res[1][:2])
         assert res[0][2] == res[1][2]
+
+
+def test_autotuned_workgroups_per_brick():
+    # the autotuner may split the LJ 4000 box's 45 bricks over 2 or 4
+    # workgroups each (its 1M-atom boxes keep one); whatever it keeps, the
+    # forces are those of its brick shape on one workgroup per brick, bit for
+    # bit (mixed forces depend on the shape: fixed point about the brick centre)
+    f = cm.load("lj4000_pbc")
+    for prec in (hip.GH_FP64, hip.GH_MIXED):
+        with hip.Context(0, prec) as a, _ctx(prec, 1) as b:
+            _setup(a, f["x"], f["types"], f["L"], "lj")
+            _setup(b, f["x"], f["types"], f["L"], "lj")
+            shape, _ = a.autotune(reps=2)
+            k = a.stats()["chunks"]
+            assert k in (1, 2, 4)
+            print("precision %d: %s cells, %d workgroups per brick" % (prec, shape, k))
+            b.set_tile(shape)
+            ra, rb = a.compute(), b.compute()
+            assert b.stats()["chunks"] == 1
+            _same(ra[:2], rb[:2])
+            assert ra[2] == rb[2]
+            a.set_tile(shape)  # a shape request resets the split
+            a.compute()
+            assert a.stats()["chunks"] == 1
