@@ -167,8 +167,24 @@ __device__ __forceinline__ int halo_of_slot(const Halo &H, int s) {
 // operations in the same order as k_vv_ba, on the force just summed.
 // New positions go to pos_next (other workgroups still stage pos); returns
 // |x_new - x_old|^2 for the skin test.
+// The update's loads of atom a (velocity, build position), issued together
+// before its arithmetic; the mixed SMA-TB and the LJ passes over small
+// systems issue them before the pair loop (those passes wait on each memory
+// round trip), the others after it.
+struct VvLoad {
+  double v[3], xo[3];
+};
+__device__ __forceinline__ VvLoad vv_load(const FuseVV &f, int a) {
+  VvLoad r;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    r.v[d] = f.vel[d * f.ncap + a];
+    r.xo[d] = f.xold[d * f.ncap + a];
+  }
+  return r;
+}
 __device__ __forceinline__ double fused_vv(const FuseVV &f, int a, const double4 &p, int t,
-                                           double Fx, double Fy, double Fz) {
+                                           double Fx, double Fy, double Fz, const VvLoad &ld) {
   const double om = f.om[t];
   const double F[3] = {Fx, Fy, Fz};
   double x[3] = {p.x, p.y, p.z};
@@ -176,14 +192,18 @@ __device__ __forceinline__ double fused_vv(const FuseVV &f, int a, const double4
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double acc = F[d] * om;
-    const double v = f.vel[d * f.ncap + a] + f.acel_dv * acc;  // stepb
-    x[d] = x[d] + f.dt * v + f.acel_dx * acc;                   // stepa
+    const double v = ld.v[d] + f.acel_dv * acc;  // stepb
+    x[d] = x[d] + f.dt * v + f.acel_dx * acc;    // stepa
     f.vel[d * f.ncap + a] = v + f.acel_dv * acc;
-    const double dd = x[d] - f.xold[d * f.ncap + a];
+    const double dd = x[d] - ld.xo[d];
     d2 += dd * dd;
   }
   f.pos_next[a] = make_double4(x[0], x[1], x[2], p.w);
   return d2;
+}
+__device__ __forceinline__ double fused_vv(const FuseVV &f, int a, const double4 &p, int t,
+                                           double Fx, double Fy, double Fz) {
+  return fused_vv(f, a, p, t, Fx, Fy, Fz, vv_load(f, a));
 }
 
 // Workgroup maximum of d2, stored at dst[b] (all threads call it).
@@ -1319,6 +1339,8 @@ __device__ __forceinline__ void tb_force_fx_body(
               qz = fxq(pa.z - H.org[2], fxs);
     const int ti = SINGLE ? 0 : type[a];
     const float di = ebd32[a];
+    VvLoad vl{};
+    if (FUSED && G > 1 && sub == 0) vl = vv_load(fv, a);
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
     auto pair = [&](const int4 &q, int s) {
       const double dx = fxd(q.x, qx), dy = fxd(q.y, qy), dz = fxd(q.z, qz);
@@ -1352,7 +1374,8 @@ __device__ __forceinline__ void tb_force_fx_body(
     if (sub != 0) continue;
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
     if (FUSED) {
-      dmax2 = fmax(dmax2, fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
+      dmax2 = fmax(dmax2, G > 1 ? fused_vv(fv, a, pa, ti, Fx, Fy, Fz, vl)
+                                : fused_vv(fv, a, pa, ti, Fx, Fy, Fz));
     } else {
       frc[a] = Fx;
       frc[ncap + a] = Fy;
@@ -1585,6 +1608,8 @@ __device__ __forceinline__ void lj_tiled_body(
   for (int a = ca0 + (int)(threadIdx.x / G); a < ca1; a += (int)(blockDim.x / G)) {
     const double4 pi = pos[a];
     const int ti = type[a];
+    VvLoad vl{};
+    if (FUSED && G > 1 && sub == 0) vl = vv_load(fv, a);
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
     auto slot = [&](int s) {
       const double vx = px[s] - pi.x, vy = py[s] - pi.y, vz = pz[s] - pi.z;
@@ -1644,7 +1669,8 @@ __device__ __forceinline__ void lj_tiled_body(
     if (sub != 0) continue;
     const double Fx = fx * ev_ui, Fy = fy * ev_ui, Fz = fz * ev_ui;
     if (FUSED) {
-      dmax2 = fmax(dmax2, fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
+      dmax2 = fmax(dmax2, G > 1 ? fused_vv(fv, a, pi, ti, Fx, Fy, Fz, vl)
+                                : fused_vv(fv, a, pi, ti, Fx, Fy, Fz));
     } else {
       frc[a] = Fx;
       frc[ncap + a] = Fy;
@@ -1715,6 +1741,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
     const double4 pa = pos[a];
     const int qx = fxq(pa.x - H.org[0], fxs), qy = fxq(pa.y - H.org[1], fxs),
               qz = fxq(pa.z - H.org[2], fxs);
+    VvLoad vl{};
+    if (FUSED && G > 1 && sub == 0) vl = vv_load(fv, a);
     double fx = 0.0, fy = 0.0, fz = 0.0, ei = 0.0;
     auto pair = [&](const int4 &q) {
       const double dx = fxd(q.x, qx), dy = fxd(q.y, qy), dz = fxd(q.z, qz);
@@ -1753,7 +1781,8 @@ __global__ __launch_bounds__(GH_TBMAX) void k_lj_fx(
     if (sub != 0) continue;
     const double Fx = fx * scale, Fy = fy * scale, Fz = fz * scale;
     if (FUSED) {
-      dmax2 = fmax(dmax2, fused_vv(fv, a, pa, type[a], Fx, Fy, Fz));
+      dmax2 = fmax(dmax2, G > 1 ? fused_vv(fv, a, pa, type[a], Fx, Fy, Fz, vl)
+                                : fused_vv(fv, a, pa, type[a], Fx, Fy, Fz));
     } else {
       frc[a] = Fx;
       frc[ncap + a] = Fy;
