@@ -751,6 +751,29 @@ __device__ __forceinline__ void stage_local(const Halo &H, const TileGeom &t,
   __syncthreads();
 }
 
+// The workgroup's longest row folded into st->maxnn, one atomic per
+// workgroup (one per wavefront queued thousands of atomics on one address at
+// the end of a build); a row past the capacity flags the overflow and, when
+// halt, stops the run.  All threads call it.
+__device__ __forceinline__ void rows_max_atomic(int m, GhStatus *st, int cap, bool halt,
+                                                long long step) {
+  __shared__ int part[GH_TBMAX / 64];
+  m = wave_imax(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 1; w < nw; ++w) m = max(m, part[w]);
+    if (m > 0) {
+      atomicMax(&st->maxnn, m);
+      if (m > cap) {
+        st->overflow = 1;
+        if (halt) raise_halt(st, GH_HALT_OVERFLOW, step);
+      }
+    }
+  }
+}
+
 // Rows of the atoms of brick b (all threads of the workgroup call it; LDS
 // is free again on return).
 __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__restrict__ pos,
@@ -788,14 +811,7 @@ __device__ void build_brick(GhStatus *st, const TileGeom &t, const double4 *__re
       build_rows<false>(H, t, pos, cellof, nullptr, nullptr, nullptr, far, A, pad, mmax);
     }
   }
-  mmax = wave_imax(mmax);
-  if ((threadIdx.x & 63) == 0 && mmax > 0) {
-    atomicMax(&st->maxnn, mmax);
-    if (mmax > cap) {
-      st->overflow = 1;
-      if (cap > 0) raise_halt(st, GH_HALT_OVERFLOW, step);  // cap 0: the sizing count
-    }
-  }
+  rows_max_atomic(mmax, st, cap, cap > 0, step);  // cap 0: the sizing count
   __syncthreads();
 }
 
@@ -1096,13 +1112,7 @@ __global__ __launch_bounds__(GH_CBLOCK) void k_build_coop(GhStatus *st,
     }
     xb = xe;
   }
-  if (lane == 0 && mmax > 0) {
-    atomicMax(&st->maxnn, mmax);
-    if (mmax > cap) {
-      st->overflow = 1;
-      raise_halt(st, GH_HALT_OVERFLOW, step);
-    }
-  }
+  rows_max_atomic(mmax, st, cap, true, step);
 }
 
 // Largest halo and brick (atoms) over all bricks; sizes slot_cap.
