@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <new>
+#include <deque>
 #include <thread>
 #include <vector>
 
@@ -1370,6 +1371,9 @@ static int finish_mid_step(gh_ctx *c) {
   return 0;
 }
 
+#ifndef GH_CHAIN_AHEAD
+#define GH_CHAIN_AHEAD 2
+#endif
 int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
   if (!c || nsteps < 0) return GH_EINVAL;
   if (each > 0 && !energy) return gh_fail(c, GH_EINVAL, "energy buffer required when each > 0");
@@ -1426,6 +1430,12 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
   // spare.  A rebuild it misses still stops the run at its step (resume).
   long long seen_step = -1;
   double seen_r = 0.0, rate = 0.0;
+  // Chained steps the device has not yet started (published): the host,
+  // up to kLead steps ahead, would otherwise chain again the steps after a
+  // rebuild from the displacement it last saw; at most GH_CHAIN_AHEAD
+  // unseen chains (a rebuild that none of them catches stops the run at its
+  // step, and the step is re-enqueued with the chain).
+  std::deque<long long> unseen;
   auto rebuild_expected = [&](long long step) {
     const long long p = pg->step;
     const double r = sqrt(__builtin_bit_cast(double, (unsigned long long)pg->maxd2));
@@ -1465,6 +1475,7 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
     stepa_done = true;  // the stop came after the step's stepa
     pending_b = false;
     chain = true;
+    unseen.clear();
     return 0;
   };
   for (;;) {
@@ -1487,7 +1498,10 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
           continue;
         }
       }
-      if (lazy && !chain) chain = rebuild_expected(s0 + ns);
+      if (lazy && !chain) {
+        while (!unseen.empty() && unseen.front() <= pg->step) unseen.pop_front();
+        chain = (int)unseen.size() < GH_CHAIN_AHEAD && rebuild_expected(s0 + ns);
+      }
       c->step = s0 + ns;
       ring[ns % kRing] = StepRec{c->pos, c->pos_next};
       if (lcg) GH_TRY(lcg_step_normals(c));
@@ -1496,7 +1510,10 @@ int gh_run(gh_ctx *c, int nsteps, int each, double *energy) {
       else if (!stepa_done) GH_TRY(ermak ? gh_launch_ermak_a(c) : gh_launch_vv_a(c));
       pending_b = stepa_done = false;
       GH_TRY(rec(c, 1));
-      if (!lazy || chain) GH_TRY(gh_launch_rebuild(c, false));
+      if (!lazy || chain) {
+        GH_TRY(gh_launch_rebuild(c, false));
+        if (lazy) unseen.push_back(c->step);
+      }
       chain = false;
       GH_TRY(rec(c, 2));
       const bool observed = ns == nsteps || (each > 0 && ns % each == 0);
