@@ -541,8 +541,8 @@ def main():
                           "lj_1m": lj_bulk(hip, 64, 100, jitter=0.01, Tstar=1.44, tiles=tiles)}))
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not run)" % args.gpus,
+        print("bench.py: runs on one GPU only; --gpus %d is not supported (the path's "
+              "sharding over several GPUs is designed, not built)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 

@@ -208,7 +208,7 @@ def main():
     rng = np.random.default_rng(20260101)
     meta = {}
 
-    # --- Au, Cleri/BASELINE parameters, rci = a, rce = a*sqrt(1.5) ----------
+    # --- Au, Cleri-Rosato parameters, rci = a, rce = a*sqrt(1.5) -----------
     a = 4.078
     au_rows = [("Au", "Au", AU, a, a * np.sqrt(1.5))]
     meta["au_prm"] = dict(AU, rci=a, rce=a * np.sqrt(1.5))
